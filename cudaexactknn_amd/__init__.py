@@ -1,0 +1,237 @@
+"""cudaexactknn_amd -- exact kNN on AMD MI355X (gfx950).
+
+Python mirror of the reference's KnnTrees GPU API (README.md:52-114 of
+rezvoj/CudaExactKnn), driving hand-written HIP kernels through the C ABI of
+libknn_mi355x.so.  PyTorch is used only to hold device memory: points,
+queries and results are CUDA/HIP tensors (or raw device pointers).
+
+    bf = BruteForceGpu(d_points)                       # BruteForceGpu<D>(points, n)
+    kd = KdTreeGpu(d_points, depth_skips=0, split_dimensions=D, block_size=256)
+    ball = BallTreeGpu(d_points, depth_skips=1, block_size=256)
+    q = Query(d_queries, count, max_distance2, r_indexes, r_distances)   # Query<D,K>
+    kd.batchKnn(q, K, BLOCK_SIZE=256, USE_HEAP_TRACKING=False,
+                USE_TRACKING_SHMEM=False, USE_STACK_SHMEM=False)
+
+Semantics follow the reference's CPU BruteForce bit for bit (see
+include/knn_mi355x.h): indices and squared distances are identical for every
+structure and option; heap mode emits keys in descending order.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Any
+
+from . import _lib
+from ._lib import KnnError, QueryOpts, TreeInfo  # noqa: F401
+
+__all__ = [
+    "BruteForceGpu", "KdTreeGpu", "BallTreeGpu", "Query", "KnnError", "pinned_malloc", "pinned_free",
+    "FORM_PLAIN", "FORM_BF16", "FORM_LEARNED", "library_path", "version",
+]
+
+FORM_PLAIN, FORM_BF16, FORM_LEARNED = _lib.KNN_FORM_PLAIN, _lib.KNN_FORM_BF16, _lib.KNN_FORM_LEARNED
+
+
+def library_path() -> str:
+    return _lib.LIB_PATH
+
+
+def version() -> int:
+    return int(_lib.load().knn_version())
+
+
+def _ptr(x: Any) -> int:
+    """Device pointer of a torch tensor (checked contiguous float32/int32 on a GPU) or an int."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if not x.is_cuda:
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, "tensor must live on the GPU")
+        if not x.is_contiguous():
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, "tensor must be contiguous")
+        return int(x.data_ptr())
+    raise TypeError(f"expected a device tensor or pointer, got {type(x)!r}")
+
+
+def _shape2(x: Any, what: str):
+    if not hasattr(x, "shape") or len(x.shape) != 2:
+        raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{what} must be a 2-D [count, dims] tensor")
+    return int(x.shape[0]), int(x.shape[1])
+
+
+@dataclass
+class Query:
+    """Query<D, K> = {points, count, maxDistance2, rIndexes, rDistances}
+    (src/KnnTrees.cuh:224-231): `points` [count, D] float32, `max_distance2` a
+    SQUARED radius, results [count, K] int32 / float32.  `count` may be None
+    (= rows of `points`)."""
+    points: Any
+    count: int | None
+    max_distance2: float
+    r_indexes: Any
+    r_distances: Any
+
+    def n(self) -> int:
+        if self.count is not None:
+            return int(self.count)
+        return int(self.points.shape[0])
+
+
+class _Structure:
+    _kind = None
+
+    def __init__(self):
+        self._h = ctypes.c_void_p(0)
+
+    @property
+    def handle(self) -> int:
+        return self._h.value or 0
+
+    def info(self) -> dict:
+        ti = TreeInfo()
+        _lib.check(_lib.load().knn_tree_get_info(self._h, ctypes.byref(ti)))
+        return ti.as_dict()
+
+    def _query(self, query: Query, K: int, opts: QueryOpts) -> None:
+        q = query.n()
+        if hasattr(query.points, "shape") and q:
+            _, d = _shape2(query.points, "query points")
+            if d != self.dims:
+                raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"query dims {d} != structure dims {self.dims}")
+        for name, t in (("r_indexes", query.r_indexes), ("r_distances", query.r_distances)):
+            if hasattr(t, "numel") and t.numel() < q * K:
+                raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{name} holds fewer than count*K elements")
+        _lib.check(_lib.load().knn_query(self._h, _ptr(query.points), q, float(query.max_distance2),
+                                         _ptr(query.r_indexes), _ptr(query.r_distances), int(K),
+                                         ctypes.byref(opts)))
+
+    def last_query_stats(self) -> dict:
+        buf = (ctypes.c_float * 4)()
+        _lib.check(_lib.load().knn_last_query_stats(self._h, buf))
+        return {"rerun_queries": buf[0], "bf16_survivor_fraction": buf[1], "radius_ratio": buf[2],
+                "query_ms": buf[3]}
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            _lib.load().knn_destroy(self._h)
+            self._h = ctypes.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, leaf_span=0, stream=None):
+    o = QueryOpts()
+    o.block = int(BLOCK_SIZE or 0)
+    o.heap = int(bool(heap))
+    o.track_lds = int(bool(track))
+    o.stack_lds = int(bool(stack))
+    o.form = int(form)
+    o.flags = 0 if sort_queries else _lib.KNN_FLAG_NO_QUERY_SORT
+    o.leaf_span = int(leaf_span)
+    o.stream = stream
+    return o
+
+
+class BruteForceGpu(_Structure):
+    """BruteForceGpu<D>(const Array<float,D>* points, uint size)
+    (src/BruteForceGpu.cuh:64-68): copies the device points."""
+
+    def __init__(self, points: Any, size: int | None = None):
+        super().__init__()
+        n, d = _shape2(points, "points")
+        n = n if size is None else int(size)
+        self.dims, self.size = d, n
+        _lib.check(_lib.load().knn_bf_create(_ptr(points), n, d, ctypes.byref(self._h)))
+
+    def batchKnn(self, query: Query, K: int, BLOCK_SIZE: int = 0, USE_HEAP_TRACKING: bool = False,
+                 USE_TRACKING_SHMEM: bool = False, form: int = FORM_PLAIN, stream=None) -> None:
+        """batchKnn<K, BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM>(Query&)
+        (src/BruteForceGpu.cuh:74-83)."""
+        self._query(query, K, _opts(BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, False, form,
+                                    stream=stream))
+
+
+class _Tree(_Structure):
+    def batchKnn(self, query: Query, K: int, BLOCK_SIZE: int = 0, USE_HEAP_TRACKING: bool = False,
+                 USE_TRACKING_SHMEM: bool = False, USE_STACK_SHMEM: bool = False, form: int = FORM_PLAIN,
+                 sort_queries: bool = True, leaf_span: int = 0, stream=None) -> None:
+        """batchKnn<K, BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM>(Query&)
+        (src/KdTreeGpu.cuh:393-410, src/BallTreeGpu.cuh:631-648)."""
+        self._query(query, K, _opts(BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM, form,
+                                    sort_queries, leaf_span, stream))
+
+    def build_ms(self) -> float:
+        return float(_lib.load().knn_last_build_ms(self._h))
+
+    def export(self):
+        """Copy the tree to host numpy arrays: (nodes, leaf_index, leaf_points[, dims])."""
+        import numpy as np
+        i = self.info()
+        nodes = np.empty(i["node_size"] * i["node_bytes"] // 4, np.float32)
+        leaves = np.empty(i["index_size"] * (1 + i["dims"]), np.float32)
+        dims = np.empty(max(1, i["node_size"]), np.uint32)
+        _lib.check(_lib.load().knn_tree_export(self._h, nodes.ctypes.data, leaves.ctypes.data, dims.ctypes.data))
+        leaves = leaves.reshape(i["index_size"], 1 + i["dims"])
+        leaf_index = leaves[:, 0].copy().view(np.int32)
+        leaf_points = leaves[:, 1:].copy()
+        return nodes, leaf_index, leaf_points, dims[: i["node_size"]]
+
+    def train_radius(self, sample: Any, K: int, epochs: int = 0, lr: float = 0.0) -> dict:
+        """Train the learned-radius predictor (KNN_FORM_LEARNED) on sample queries."""
+        n, d = _shape2(sample, "sample")
+        stats = (ctypes.c_float * 2)()
+        _lib.check(_lib.load().knn_learned_train(self._h, _ptr(sample), n, int(K), int(epochs), float(lr), stats))
+        return {"loss": stats[0], "train_ms": stats[1]}
+
+
+class KdTreeGpu(_Tree):
+    """KdTreeGpu<D, SPLIT_DIMENSIONS, BLOCK_SIZE>(points, size, depthSkips = 0)
+    (src/KdTreeGpu.cuh:375-385)."""
+
+    def __init__(self, points: Any, size: int | None = None, depth_skips: int = 0,
+                 split_dimensions: int | None = None, block_size: int = 256):
+        super().__init__()
+        n, d = _shape2(points, "points")
+        n = n if size is None else int(size)
+        sd = d if split_dimensions is None else int(split_dimensions)
+        self.dims, self.size = d, n
+        _lib.check(_lib.load().knn_kd_build(_ptr(points), n, d, sd, int(depth_skips), int(block_size),
+                                            ctypes.byref(self._h)))
+
+
+class BallTreeGpu(_Tree):
+    """BallTreeGpu<D, BLOCK_SIZE>(points, size, depthSkips = 1)
+    (src/BallTreeGpu.cuh:614-624)."""
+
+    def __init__(self, points: Any, size: int | None = None, depth_skips: int = 1, block_size: int = 256):
+        super().__init__()
+        n, d = _shape2(points, "points")
+        n = n if size is None else int(size)
+        self.dims, self.size = d, n
+        _lib.check(_lib.load().knn_ball_build(_ptr(points), n, d, int(depth_skips), int(block_size),
+                                              ctypes.byref(self._h)))
+
+
+def pinned_malloc(nbytes: int) -> int:
+    """pinnedMalloc (src/KnnTrees.cuh:233-238): page-locked host memory."""
+    p = ctypes.c_void_p(0)
+    _lib.check(_lib.load().knn_pinned_alloc(int(nbytes), ctypes.byref(p)))
+    return p.value or 0
+
+
+def pinned_free(ptr: int) -> None:
+    """pinnedFree (src/KnnTrees.cuh:240-243)."""
+    _lib.check(_lib.load().knn_pinned_free(ctypes.c_void_p(ptr)))

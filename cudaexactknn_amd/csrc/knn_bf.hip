@@ -1,0 +1,191 @@
+// Plain exact brute force (replaces BruteForceKernels::knnDevice / knnKernel /
+// knnShKernel, src/BruteForceGpu.cuh:9-50, dispatch :74-83).
+//
+// One query per lane, as in the reference, but the point stream is staged
+// through LDS: each workgroup copies a tile of points with coalesced loads,
+// then every lane reads the same LDS address (a broadcast, conflict-free) for
+// each point.  The K-buffer is a static-index sorted array in VGPRs, or a
+// lane-interleaved array / heap in LDS, or a heap in private memory, as the
+// USE_HEAP_TRACKING / USE_TRACKING_SHMEM options ask.  A candidate is first
+// screened with a 32-bit compare of its distance bits against the high word
+// of the K-th key; only then is the 64-bit key formed.
+#include "knn_device.hpp"
+#include "knn_internal.hpp"
+
+namespace knn {
+
+template <int D, int KIND>
+__global__ void __launch_bounds__(1024) bf_kernel(const float* __restrict__ pts, uint32_t n,
+                                                  const float* __restrict__ qs, uint32_t nq, float maxd2,
+                                                  uint32_t K, int32_t* __restrict__ oidx,
+                                                  float* __restrict__ odist, uint32_t tile_pts) {
+    constexpr int DP = (D + 3) & ~3;
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    float* tile = reinterpret_cast<float*>(lds_raw);
+    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + (size_t)tile_pts * DP * 4);
+    const uint32_t BS = blockDim.x, tid = threadIdx.x;
+    const uint32_t qi = blockIdx.x * BS + tid;
+    const bool active = qi < nq;
+    float q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = active ? qs[(size_t)qi * D + d] : 0.0f;
+    typename BufOf<KIND>::T buf;
+    bind_lds(buf, kbuf + tid, BS);
+    const uint64_t empty = ((uint64_t)f2u(maxd2) << 32) | kEmptyLo;
+    uint64_t worst = empty;
+    if (active) worst = buf.init(empty, K);
+    uint32_t whi = key_hi(worst);
+    for (uint32_t base = 0; base < n; base += tile_pts) {
+        const uint32_t cnt = min(tile_pts, n - base);
+        __syncthreads();
+        const float* src = pts + (size_t)base * D;
+        for (uint32_t e = tid; e < cnt * D; e += BS) {
+            const uint32_t p = e / D, d = e - p * D;
+            tile[p * DP + d] = src[e];
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll 4
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const float d2 = dist2<D>(q, tile + j * DP);
+                if (f2u(d2) <= whi) {
+                    const uint64_t key = make_key(d2, base + j);
+                    if (key < worst) {
+                        worst = buf.consider(key);
+                        whi = key_hi(worst);
+                    }
+                }
+            }
+        }
+    }
+    if (active) buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
+}
+
+// Runtime-D fallback: the query lives in LDS (lane-interleaved).
+template <int KIND>
+__global__ void __launch_bounds__(1024) bf_kernel_rt(const float* __restrict__ pts, uint32_t n, uint32_t D,
+                                                     const float* __restrict__ qs, uint32_t nq, float maxd2,
+                                                     uint32_t K, int32_t* __restrict__ oidx,
+                                                     float* __restrict__ odist, uint32_t tile_pts) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const uint32_t DP = (D + 3) & ~3u;
+    const uint32_t BS = blockDim.x, tid = threadIdx.x;
+    float* tile = reinterpret_cast<float*>(lds_raw);
+    float* qlds = tile + (size_t)tile_pts * DP;
+    uint64_t* kbuf = reinterpret_cast<uint64_t*>(qlds + (size_t)BS * D + ((BS * D) & 1));
+    const uint32_t qi = blockIdx.x * BS + tid;
+    const bool active = qi < nq;
+    for (uint32_t d = 0; d < D; ++d) qlds[(size_t)d * BS + tid] = active ? qs[(size_t)qi * D + d] : 0.0f;
+    typename BufOf<KIND>::T buf;
+    bind_lds(buf, kbuf + tid, BS);
+    const uint64_t empty = ((uint64_t)f2u(maxd2) << 32) | kEmptyLo;
+    uint64_t worst = empty;
+    if (active) worst = buf.init(empty, K);
+    uint32_t whi = key_hi(worst);
+    for (uint32_t base = 0; base < n; base += tile_pts) {
+        const uint32_t cnt = min(tile_pts, n - base);
+        __syncthreads();
+        const float* src = pts + (size_t)base * D;
+        for (uint32_t e = tid; e < cnt * D; e += BS) {
+            const uint32_t p = e / D, d = e - p * D;
+            tile[p * DP + d] = src[e];
+        }
+        __syncthreads();
+        if (active) {
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const float d2 = dist2_rt(qlds + tid, BS, tile + (size_t)j * DP, D);
+                if (f2u(d2) <= whi) {
+                    const uint64_t key = make_key(d2, base + j);
+                    if (key < worst) {
+                        worst = buf.consider(key);
+                        whi = key_hi(worst);
+                    }
+                }
+            }
+        }
+    }
+    if (active) buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
+}
+
+using BfFn = void (*)(const float*, uint32_t, const float*, uint32_t, float, uint32_t, int32_t*, float*, uint32_t);
+using BfRtFn = void (*)(const float*, uint32_t, uint32_t, const float*, uint32_t, float, uint32_t, int32_t*, float*,
+                        uint32_t);
+
+template <int D>
+static BfFn bf_pick(int kind) {
+    switch (kind) {
+        case BUF_REG8: return bf_kernel<D, BUF_REG8>;
+        case BUF_REG16: return bf_kernel<D, BUF_REG16>;
+        case BUF_REG32: return bf_kernel<D, BUF_REG32>;
+        case BUF_LDS_ARRAY: return bf_kernel<D, BUF_LDS_ARRAY>;
+        case BUF_LDS_HEAP: return bf_kernel<D, BUF_LDS_HEAP>;
+        default: return bf_kernel<D, BUF_PRIV_HEAP>;
+    }
+}
+
+static BfRtFn bf_pick_rt(int kind) {
+    switch (kind) {
+        case BUF_REG8: return bf_kernel_rt<BUF_REG8>;
+        case BUF_REG16: return bf_kernel_rt<BUF_REG16>;
+        case BUF_REG32: return bf_kernel_rt<BUF_REG32>;
+        case BUF_LDS_ARRAY: return bf_kernel_rt<BUF_LDS_ARRAY>;
+        case BUF_LDS_HEAP: return bf_kernel_rt<BUF_LDS_HEAP>;
+        default: return bf_kernel_rt<BUF_PRIV_HEAP>;
+    }
+}
+
+// Buffer kind from (USE_HEAP_TRACKING, USE_TRACKING_SHMEM, K).  Register
+// buffers hold at most 32 slots; larger K moves the buffer to LDS (same
+// results).
+int pick_buf_kind(bool heap, bool lds, uint32_t K) {
+    if (heap) return (lds || K > 32) ? BUF_LDS_HEAP : BUF_PRIV_HEAP;
+    if (lds || K > 32) return BUF_LDS_ARRAY;
+    if (K <= 8) return BUF_REG8;
+    if (K <= 16) return BUF_REG16;
+    return BUF_REG32;
+}
+
+static bool buf_in_lds(int kind) { return kind == BUF_LDS_ARRAY || kind == BUF_LDS_HEAP; }
+
+void query_bf(Handle& h, const QueryArgs& a) {
+    const uint32_t D = h.info.dims;
+    const int kind = pick_buf_kind(a.heap, a.track_lds, a.K);
+    const uint32_t BS = a.block;
+    const uint32_t DP = (D + 3) & ~3u;
+    uint32_t tile_pts = 8192 / DP;
+    if (tile_pts > 1024) tile_pts = 1024;
+    if (tile_pts < 8) tile_pts = 8;
+    BfFn fn = nullptr;
+    switch (D) {
+        case 2: fn = bf_pick<2>(kind); break;
+        case 3: fn = bf_pick<3>(kind); break;
+        case 4: fn = bf_pick<4>(kind); break;
+        case 5: fn = bf_pick<5>(kind); break;
+        case 6: fn = bf_pick<6>(kind); break;
+        case 8: fn = bf_pick<8>(kind); break;
+        case 16: fn = bf_pick<16>(kind); break;
+        case 32: fn = bf_pick<32>(kind); break;
+        case 64: fn = bf_pick<64>(kind); break;
+        case 128: fn = bf_pick<128>(kind); break;
+        default: break;
+    }
+    size_t lds = (size_t)tile_pts * DP * 4;
+    if (!fn) lds += (size_t)BS * D * 4 + 8;
+    if (buf_in_lds(kind)) lds += (size_t)BS * a.K * 8;
+    require(lds <= device_limits().max_lds_per_block,
+            "LDS request of " + std::to_string(lds) + " bytes exceeds the per-workgroup limit of " +
+                std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",
+            KNN_ERR_INVALID_ARG);
+    const uint32_t grid = (a.nq + BS - 1) / BS;
+    if (fn) {
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(BS), lds, a.stream, h.pts.get(), h.info.n, a.q, a.nq, a.maxd2, a.K,
+                           a.oidx, a.odist, tile_pts);
+    } else {
+        BfRtFn rf = bf_pick_rt(kind);
+        hipLaunchKernelGGL(rf, dim3(grid), dim3(BS), lds, a.stream, h.pts.get(), h.info.n, D, a.q, a.nq, a.maxd2,
+                           a.K, a.oidx, a.odist, tile_pts);
+    }
+    KNN_CHECK_LAUNCH();
+}
+
+}  // namespace knn

@@ -1,0 +1,616 @@
+// GPU tree construction for KdTreeGpu / BallTreeGpu (replaces mSplitPoints,
+// src/KdTreeGpu.cuh:297-372 and src/BallTreeGpu.cuh:480-611, and their CUB
+// calls).
+//
+// Layout and algorithm
+//  - For every split dimension d the builder keeps a list P_d of point ids in
+//    key order (ordered coordinate bits, then id), produced once by an LSD
+//    radix sort.  All lists share one segmentation: at tree level l, node s
+//    owns positions [start_s, start_s + n_s) in every list.
+//  - A level needs no sort: in P_dim the left child is simply the first
+//    ceil(n/2) positions of each segment (src/KdTreeGpu.cuh:231-238 rule) and
+//    the split value is the coordinate at rank ceil(n/2)-1.  Every other list
+//    is re-segmented by a stable 1-bit partition (one LSD radix pass on the
+//    left/right digit), computed from a per-point flag with a block-local
+//    ballot scan plus a device-wide scan of block counts.
+//  - Ball trees pick each node's dim as the first maximum of the per-dim sum
+//    of squared deviations from the node centroid (src/BallTreeGpu.cuh:296-307)
+//    and store, per child, the FP64 mean rounded to FP32 and a certified upper
+//    bound of the radius.  These per-segment sums use a fixed-order reduction
+//    (lane stride, butterfly, chunk order), so builds are deterministic.
+//  - Leaves are IndexPoint<D> buckets of 2^depthSkips slots padded with -1,
+//    in leaf order; inside a bucket points are in key order of the last
+//    split dimension.
+#include "knn_device.hpp"
+#include "knn_internal.hpp"
+
+namespace knn {
+
+static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+__global__ void ordered_keys_kernel(const float* __restrict__ pts, uint32_t n, uint32_t D, uint32_t dim,
+                                    uint32_t* __restrict__ keys) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = ordered_bits(pts[i * D + dim]);
+}
+
+// --- bounding box (query ordering) -----------------------------------------
+__device__ __forceinline__ float from_ordered(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__global__ void bbox_kernel(const float* __restrict__ pts, uint32_t n, uint32_t D, uint32_t* __restrict__ omin,
+                            uint32_t* __restrict__ omax) {
+    const uint32_t d = blockIdx.y;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t o = ordered_bits(pts[i * D + d]);
+        lo = min(lo, o);
+        hi = max(hi, o);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&omin[d], lo);
+        atomicMax(&omax[d], hi);
+    }
+}
+
+__global__ void bbox_finish_kernel(const uint32_t* __restrict__ omin, const uint32_t* __restrict__ omax, uint32_t D,
+                                   float* __restrict__ bbox) {
+    uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < D) {
+        bbox[d] = from_ordered(omin[d]);
+        bbox[D + d] = from_ordered(omax[d]);
+    }
+}
+
+void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hipStream_t st) {
+    DevBuf<uint32_t> ext;
+    ext.alloc(2 * (size_t)D);
+    launch_fill_u32(ext.get(), D, 0xFFFFFFFFu, st);
+    launch_fill_u32(ext.get() + D, D, 0u, st);
+    if (n) {
+        uint32_t gx = nblk(n, 256);
+        if (gx > 1024) gx = 1024;
+        hipLaunchKernelGGL(bbox_kernel, dim3(gx, D), dim3(256), 0, st, d_pts, n, D, ext.get(), ext.get() + D);
+    }
+    hipLaunchKernelGGL(bbox_finish_kernel, dim3(nblk(D, 64)), dim3(64), 0, st, ext.get(), ext.get() + D, D, d_bbox);
+    KNN_CHECK_LAUNCH();
+    KNN_HIP(hipStreamSynchronize(st));
+}
+
+// --- per-level node kernels --------------------------------------------------
+// kd: split value at rank ceil(n/2)-1 of the segment in P_dim
+// (src/KdTreeGpu.cuh:217-239).
+__global__ void kd_node_kernel(const float* __restrict__ pts, uint32_t D, const uint32_t* __restrict__ Pdim,
+                               uint32_t dim, const uint32_t* __restrict__ segStart,
+                               const uint32_t* __restrict__ segN, uint32_t S, float* __restrict__ nodes,
+                               uint32_t* __restrict__ nStart, uint32_t* __restrict__ nN, uint32_t* __restrict__ Lcount) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint32_t st = segStart[s], n = segN[s];
+    const uint32_t L = (n + 1) / 2;
+    nodes[S - 1 + s] = L ? pts[(size_t)Pdim[st + L - 1] * D + dim] : 0.0f;
+    nStart[2 * s] = st;
+    nN[2 * s] = L;
+    nStart[2 * s + 1] = st + L;
+    nN[2 * s + 1] = n - L;
+    Lcount[s] = L;
+}
+
+// ball: dim = first maximum of the deviation sums (strict >, from 0).
+__global__ void ball_node_kernel(const double* __restrict__ var, uint32_t D, const uint32_t* __restrict__ segStart,
+                                 const uint32_t* __restrict__ segN, uint32_t S, uint32_t* __restrict__ nodeDim,
+                                 uint32_t* __restrict__ segDim, uint32_t* __restrict__ nStart,
+                                 uint32_t* __restrict__ nN, uint32_t* __restrict__ Lcount) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    uint32_t best = 0;
+    double bv = 0.0;
+    for (uint32_t d = 0; d < D; ++d) {
+        const double v = var[(size_t)s * D + d];
+        if (v > bv) { bv = v; best = d; }
+    }
+    nodeDim[S - 1 + s] = best;
+    segDim[s] = best;
+    const uint32_t st = segStart[s], n = segN[s];
+    const uint32_t L = (n + 1) / 2;
+    nStart[2 * s] = st;
+    nN[2 * s] = L;
+    nStart[2 * s + 1] = st + L;
+    nN[2 * s + 1] = n - L;
+    Lcount[s] = L;
+}
+
+// Left/right flag per point id and next-level segment per position.
+__global__ void flag_kernel(const uint32_t* __restrict__ P, uint32_t N, uint32_t uniformDim,
+                            const uint32_t* __restrict__ segDim, const uint32_t* __restrict__ segOf,
+                            const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
+                            uint8_t* __restrict__ flag, uint32_t* __restrict__ segOfNext) {
+    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const uint32_t s = segOf[j];
+    const uint32_t st = segStart[s];
+    const uint32_t L = (segN[s] + 1) / 2;
+    const uint32_t d = segDim ? segDim[s] : uniformDim;
+    const uint32_t id = P[(size_t)d * N + j];
+    const bool left = (uint32_t)j - st < L;
+    flag[id] = left ? 1 : 0;
+    segOfNext[j] = 2 * s + (left ? 0 : 1);
+}
+
+// --- stable segmented partition of the lists ---------------------------------
+constexpr int PT_THREADS = 256;
+constexpr int PT_ITEMS = 16;
+constexpr uint32_t PT_TILE = PT_THREADS * PT_ITEMS;
+
+__device__ __forceinline__ uint32_t list_of(uint32_t y, uint32_t skip) { return y + (y >= skip ? 1u : 0u); }
+
+__global__ void __launch_bounds__(PT_THREADS) part_count_kernel(const uint32_t* __restrict__ P, uint32_t N, uint32_t skip,
+                                                                const uint8_t* __restrict__ flag,
+                                                                uint32_t* __restrict__ cnt, uint32_t nb) {
+    __shared__ uint32_t red[PT_THREADS / 64];
+    const uint32_t* L = P + (size_t)list_of(blockIdx.y, skip) * N;
+    const size_t base = (size_t)blockIdx.x * PT_TILE;
+    uint32_t c = 0;
+    for (uint32_t i = threadIdx.x; i < PT_TILE; i += PT_THREADS) {
+        size_t j = base + i;
+        if (j < N) c += flag[L[j]];
+    }
+    for (int off = 32; off >= 1; off >>= 1) c += (uint32_t)__shfl_xor((int)c, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < PT_THREADS / 64; ++w) t += red[w];
+        cnt[(size_t)blockIdx.y * nb + blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ Pout,
+                                                                  uint32_t N, uint32_t skip,
+                                                                  const uint8_t* __restrict__ flag,
+                                                                  const uint32_t* __restrict__ off, uint32_t nb,
+                                                                  const uint32_t* __restrict__ segOf,
+                                                                  const uint32_t* __restrict__ segStart,
+                                                                  const uint32_t* __restrict__ segN,
+                                                                  const uint32_t* __restrict__ leftBefore,
+                                                                  const uint32_t* __restrict__ Lcount, uint32_t S) {
+    __shared__ uint32_t lw[PT_THREADS / 64 + 1];
+    const uint32_t y = blockIdx.y;
+    const size_t lo = (size_t)list_of(y, skip) * N;
+    const uint32_t* L = P + lo;
+    uint32_t* O = Pout + lo;
+    // Every list holds the same number of left points, so list y's block
+    // offsets in the concatenated scan start at y * totalLeft.
+    const uint32_t totalLeft = leftBefore[S - 1] + Lcount[S - 1];
+    const size_t base = (size_t)blockIdx.x * PT_TILE + (size_t)threadIdx.x * PT_ITEMS;
+    uint32_t id[PT_ITEMS];
+    uint32_t fl = 0, cnt = 0;
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + i;
+        id[i] = j < N ? L[j] : 0u;
+        const uint32_t f = j < N ? flag[id[i]] : 0u;
+        fl |= f << i;
+        cnt += f;
+    }
+    uint32_t total;
+    uint32_t g = block_excl_scan(cnt, lw, &total);
+    g += off[(size_t)y * nb + blockIdx.x] - y * totalLeft;
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + i;
+        if (j < N) {
+            const uint32_t s = segOf[j];
+            const uint32_t st = segStart[s];
+            const uint32_t Ls = (segN[s] + 1) / 2;
+            const uint32_t lb = g - leftBefore[s];
+            const bool f = (fl >> i) & 1u;
+            const uint32_t pos = f ? st + lb : st + Ls + ((uint32_t)j - st - lb);
+            O[pos] = id[i];
+            g += f ? 1u : 0u;
+        }
+    }
+}
+
+// --- segmented reductions (ball centroids, deviation sums, radii) ----------
+constexpr int DB = 4;  // dims per reduction pass
+
+// Work item w = (segment s = w / chunks, chunk k = w % chunks), handled by a
+// group of G lanes (G a power of two <= 64).  Lanes stride over the chunk,
+// then a butterfly over the group gives a fixed summation order.
+__global__ void seg_reduce_kernel(const uint32_t* __restrict__ P0, const float* __restrict__ pts, uint32_t D,
+                                  uint32_t dbase, const uint32_t* __restrict__ segStart,
+                                  const uint32_t* __restrict__ segN, uint32_t S, uint32_t chunks, uint32_t G,
+                                  const float* __restrict__ cent, double* __restrict__ partial,
+                                  float* __restrict__ pmax) {
+#pragma clang fp contract(off)
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t w = t / G;
+    const uint32_t r = (uint32_t)(t % G);
+    const uint32_t s = (uint32_t)(w / chunks), k = (uint32_t)(w % chunks);
+    const bool valid = s < S;
+    double acc[DB];
+#pragma unroll
+    for (int b = 0; b < DB; ++b) acc[b] = 0.0;
+    float mx = 0.0f;
+    if (valid) {
+        const uint32_t st = segStart[s], n = segN[s];
+        const uint32_t len = (n + chunks - 1) / chunks;
+        const uint32_t b0 = st + k * len;
+        const uint32_t e0 = min(st + n, b0 + len);
+        const float* c = cent ? cent + (size_t)s * D : nullptr;
+        for (uint32_t j = b0 + r; j < e0; j += G) {
+            const float* p = pts + (size_t)P0[j] * D;
+            if (!c) {
+#pragma unroll
+                for (int b = 0; b < DB; ++b)
+                    if (dbase + b < D) acc[b] += (double)p[dbase + b];
+            } else {
+#pragma unroll
+                for (int b = 0; b < DB; ++b)
+                    if (dbase + b < D) {
+                        const double u = (double)p[dbase + b] - (double)c[dbase + b];
+                        acc[b] += u * u;
+                    }
+                if (pmax) {
+                    // calcDistance2(centroid, point), src/BallTree.cuh:81
+                    float tt = c[0] - p[0];
+                    float d2 = tt * tt;
+                    for (uint32_t d = 1; d < D; ++d) {
+                        const float u = c[d] - p[d];
+                        const float sq = u * u;
+                        d2 = d2 + sq;
+                    }
+                    mx = d2 > mx ? d2 : mx;
+                }
+            }
+        }
+    }
+    for (uint32_t o = G >> 1; o >= 1; o >>= 1) {
+#pragma unroll
+        for (int b = 0; b < DB; ++b) acc[b] += __shfl_xor(acc[b], (int)o, 64);
+        const float om = __shfl_xor(mx, (int)o, 64);
+        mx = om > mx ? om : mx;
+    }
+    if (valid && r == 0) {
+#pragma unroll
+        for (int b = 0; b < DB; ++b) partial[w * DB + b] = acc[b];
+        if (pmax) pmax[w] = mx;
+    }
+}
+
+// Combines chunk partials in chunk order.  Mode 0: centroids (FP64 mean ->
+// FP32).  Mode 1: deviation sums, and radii when pmax is given.  When
+// `ballNodes` is given, child s of the level (BFS index S-1+s) is written into
+// its parent's BallNode (left child = odd BFS index).
+__global__ void seg_finalize_kernel(int mode, uint32_t S, uint32_t chunks, uint32_t D, uint32_t dbase,
+                                    const double* __restrict__ partial, const float* __restrict__ pmax,
+                                    const uint32_t* __restrict__ segN, float* __restrict__ cent,
+                                    double* __restrict__ var, float* __restrict__ ballNodes, float inflate) {
+#pragma clang fp contract(off)
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint32_t n = segN[s];
+    double v[DB];
+#pragma unroll
+    for (int b = 0; b < DB; ++b) v[b] = 0.0;
+    for (uint32_t k = 0; k < chunks; ++k) {
+#pragma unroll
+        for (int b = 0; b < DB; ++b) v[b] += partial[((size_t)s * chunks + k) * DB + b];
+    }
+    float* nodeBase = nullptr;
+    if (ballNodes) {
+        const uint32_t g = S - 1 + s;
+        const uint32_t parent = (g - 1) / 2;
+        nodeBase = ballNodes + (size_t)parent * (2 * D + 2) + ((g & 1u) ? 0 : D + 1);
+    }
+    if (mode == 0) {
+#pragma unroll
+        for (int b = 0; b < DB; ++b) {
+            const uint32_t d = dbase + b;
+            if (d < D) {
+                const float c = n ? (float)(v[b] / (double)n) : 0.0f;
+                cent[(size_t)s * D + d] = c;
+                if (nodeBase) nodeBase[d] = c;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int b = 0; b < DB; ++b) {
+            const uint32_t d = dbase + b;
+            if (d < D) var[(size_t)s * D + d] = v[b];
+        }
+        if (pmax && nodeBase) {
+            float m = 0.0f;
+            for (uint32_t k = 0; k < chunks; ++k) {
+                const float x = pmax[(size_t)s * chunks + k];
+                m = x > m ? x : m;
+            }
+            // Certified upper bound of the exact radius: every computed d2 is
+            // within a factor (1 +- 2^-24)^(D+2) of the exact value.
+            nodeBase[D] = n ? sqrtf(m) * inflate + 0x1.0p-100f : -INFINITY;
+        }
+    }
+}
+
+__global__ void leaf_fill_kernel(const float* __restrict__ pts, uint32_t D, const uint32_t* __restrict__ P, uint32_t N,
+                                 uint32_t uniformDim, const uint32_t* __restrict__ nodeDim, uint32_t nodeSize,
+                                 const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
+                                 uint32_t skips, uint32_t indexSize, float* __restrict__ leaves) {
+    const size_t slot = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= indexSize) return;
+    const uint32_t leaf = (uint32_t)(slot >> skips);
+    const uint32_t k = (uint32_t)(slot & ((1u << skips) - 1));
+    float* e = leaves + slot * (1 + D);
+    const uint32_t n = segN ? segN[leaf] : 0u;
+    if (k < n) {
+        uint32_t d = uniformDim;
+        if (nodeDim) d = nodeDim[(nodeSize + leaf - 1) / 2];
+        const uint32_t id = P[(size_t)d * N + segStart[leaf] + k];
+        e[0] = __int_as_float((int32_t)id);
+        for (uint32_t dd = 0; dd < D; ++dd) e[1 + dd] = pts[(size_t)id * D + dd];
+    } else {
+        e[0] = __int_as_float(-1);
+        for (uint32_t dd = 0; dd < D; ++dd) e[1 + dd] = 0.0f;
+    }
+}
+
+__global__ void fill_f32_kernel(float* __restrict__ a, size_t n, float v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LevelBufs {
+    DevBuf<uint32_t> P, P2;              // D (or SD) lists of N ids, double-buffered
+    DevBuf<uint32_t> segOf, segOf2;      // segment per position
+    DevBuf<uint8_t> flag;                // left flag per point id
+    DevBuf<uint32_t> start, n, start2, n2;  // segment tables (cur / next level)
+    DevBuf<uint32_t> Lcount, leftBefore, segDim;
+    DevBuf<uint32_t> cnt, temp;          // partition block counts, scan temp
+    DevBuf<uint32_t> keys, keys2, vals2; // presort scratch
+};
+
+// Sorted lists P_d, d < nl, by (ordered coordinate bits, id).
+void presort(LevelBufs& B, const float* pts, uint32_t N, uint32_t D, uint32_t nl, hipStream_t st) {
+    B.keys.alloc(N);
+    B.keys2.alloc(N);
+    B.vals2.alloc(N);
+    B.temp.ensure(radix_temp_words(N));
+    for (uint32_t d = 0; d < nl; ++d) {
+        uint32_t* vals = B.P.get() + (size_t)d * N;
+        hipLaunchKernelGGL(ordered_keys_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, N, D, d, B.keys.get());
+        launch_iota(vals, N, st);
+        bool alt = radix_sort_pairs(B.keys.get(), vals, B.keys2.get(), B.vals2.get(), N, 0, 32, B.temp.get(), st);
+        if (alt) KNN_HIP(hipMemcpyAsync(vals, B.vals2.get(), sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
+    }
+    KNN_CHECK_LAUNCH();
+    B.keys.release();
+    B.keys2.release();
+    B.vals2.release();
+}
+
+void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg) {
+    B.P.alloc((size_t)nl * N);
+    B.P2.alloc((size_t)nl * N);
+    B.segOf.alloc(N);
+    B.segOf2.alloc(N);
+    B.flag.alloc(N);
+    B.start.alloc(maxSeg);
+    B.n.alloc(maxSeg);
+    B.start2.alloc(maxSeg);
+    B.n2.alloc(maxSeg);
+    B.Lcount.alloc(maxSeg);
+    B.leftBefore.alloc(maxSeg);
+    B.segDim.alloc(maxSeg);
+    const uint32_t nb = nblk(N, PT_TILE);
+    B.cnt.alloc((size_t)nb * nl);
+    size_t tw = scan_temp_words((uint32_t)((size_t)nb * nl));
+    size_t tw2 = scan_temp_words(maxSeg);
+    size_t tw3 = radix_temp_words(N);
+    B.temp.alloc(std::max(tw, std::max(tw2, tw3)));
+}
+
+// Partition every list except `skip` by the point flags.
+void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint32_t S, hipStream_t st) {
+    const uint32_t nlists = (skip < nl) ? nl - 1 : nl;
+    if (nlists == 0) return;
+    const uint32_t nb = nblk(N, PT_TILE);
+    hipLaunchKernelGGL(part_count_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), N, skip,
+                       B.flag.get(), B.cnt.get(), nb);
+    exclusive_scan_u32(B.cnt.get(), B.cnt.get(), nb * nlists, B.temp.get(), st);
+    hipLaunchKernelGGL(part_scatter_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), B.P2.get(), N, skip,
+                       B.flag.get(), B.cnt.get(), nb, B.segOf.get(), B.start.get(), B.n.get(), B.leftBefore.get(),
+                       B.Lcount.get(), S);
+    // The partitioned lists live in P2; the skipped list (already in order)
+    // is carried over so that P2 is complete before the swap.
+    if (skip < nl)
+        KNN_HIP(hipMemcpyAsync(B.P2.get() + (size_t)skip * N, B.P.get() + (size_t)skip * N, sizeof(uint32_t) * N,
+                               hipMemcpyDeviceToDevice, st));
+    std::swap(B.P.p, B.P2.p);
+}
+
+void level_common(LevelBufs& B, uint32_t N, uint32_t S, uint32_t uniformDim, bool perSegDim, hipStream_t st) {
+    exclusive_scan_u32(B.Lcount.get(), B.leftBefore.get(), S, B.temp.get(), st);
+    hipLaunchKernelGGL(flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, B.P.get(), N, uniformDim,
+                       perSegDim ? B.segDim.get() : nullptr, B.segOf.get(), B.start.get(), B.n.get(), B.flag.get(),
+                       B.segOf2.get());
+}
+
+struct Timer {
+    hipEvent_t a{}, b{};
+    hipStream_t st;
+    explicit Timer(hipStream_t s) : st(s) {
+        KNN_HIP(hipEventCreate(&a));
+        KNN_HIP(hipEventCreate(&b));
+        KNN_HIP(hipEventRecord(a, st));
+    }
+    float stop() {
+        KNN_HIP(hipEventRecord(b, st));
+        KNN_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        KNN_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    }
+    ~Timer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+};
+
+void fill_empty_leaves(Handle& h, hipStream_t st) {
+    const knn_tree_info& I = h.info;
+    hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, nullptr, I.dims, nullptr,
+                       0u, 0u, nullptr, I.node_size, nullptr, nullptr, I.depth_skips, I.index_size, h.leaves.get());
+}
+
+// Segmented reduction over the segments (start, n) of a level: mode 0 sums
+// coordinates into centroids, mode 1 sums squared deviations from `cent` and
+// (pmax) the max distance for radii.
+void seg_reduce(const uint32_t* P0, const float* pts, uint32_t N, uint32_t D, const uint32_t* start, const uint32_t* n,
+                uint32_t S, int mode, float* cent, double* var, float* ballNodes, DevBuf<double>& partial,
+                DevBuf<float>& pmax, hipStream_t st) {
+    const uint64_t avg = (N + (uint64_t)S - 1) / S;
+    uint32_t G = 64, chunks = 1;
+    if (avg >= 4096) {
+        chunks = (uint32_t)((avg + 8191) / 8192);
+    } else {
+        uint32_t g = 1;
+        while (g < 64 && (uint64_t)g * 4 < avg) g <<= 1;
+        G = g;
+    }
+    const size_t items = (size_t)S * chunks;
+    partial.ensure(items * DB);
+    pmax.ensure(items);
+    const size_t threads = items * G;
+    const uint32_t grid = (uint32_t)((threads + 255) / 256);
+    const float inflate = 1.0f + (float)(D + 8) * 0x1.0p-23f;
+    for (uint32_t dbase = 0; dbase < D; dbase += DB) {
+        const bool withMax = (mode == 1) && dbase == 0;
+        hipLaunchKernelGGL(seg_reduce_kernel, dim3(grid), dim3(256), 0, st, P0, pts, D, dbase, start, n, S, chunks, G,
+                           mode == 1 ? cent : nullptr, partial.get(), withMax ? pmax.get() : nullptr);
+        hipLaunchKernelGGL(seg_finalize_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, mode, S, chunks, D, dbase,
+                           partial.get(), withMax ? pmax.get() : nullptr, n, cent, var, ballNodes, inflate);
+    }
+    KNN_CHECK_LAUNCH();
+}
+
+}  // namespace
+
+void build_kd(Handle& h, const float* pts, hipStream_t st, uint32_t block) {
+    (void)block;
+    knn_tree_info& I = h.info;
+    const uint32_t N = I.n, D = I.dims, SD = I.split_dims;
+    h.nodes.alloc(I.node_size);
+    h.leaves.alloc((size_t)I.index_size * (1 + D));
+    h.bbox.alloc(2 * (size_t)D);
+    Timer timer(st);
+    if (N == 0) {
+        hipLaunchKernelGGL(fill_f32_kernel, dim3(nblk(I.node_size, 256)), dim3(256), 0, st, h.nodes.get(),
+                           (size_t)I.node_size, 0.0f);
+        fill_empty_leaves(h, st);
+        KNN_CHECK_LAUNCH();
+        h.last_build_ms = timer.stop();
+        compute_bbox(pts, 0, D, h.bbox.get(), st);
+        return;
+    }
+    LevelBufs B;
+    const uint32_t maxSeg = 1u << I.node_depth;
+    alloc_levels(B, N, SD, maxSeg);
+    presort(B, pts, N, D, SD, st);
+    launch_fill_u32(B.segOf.get(), N, 0u, st);
+    launch_fill_u32(B.start.get(), 1, 0u, st);
+    launch_fill_u32(B.n.get(), 1, N, st);
+    for (uint32_t depth = 0; depth < I.node_depth; ++depth) {
+        const uint32_t S = 1u << depth;
+        const uint32_t dim = depth % SD;
+        hipLaunchKernelGGL(kd_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, pts, D, B.P.get() + (size_t)dim * N,
+                           dim, B.start.get(), B.n.get(), S, h.nodes.get(), B.start2.get(), B.n2.get(),
+                           B.Lcount.get());
+        level_common(B, N, S, dim, false, st);
+        partition_lists(B, N, SD, dim, S, st);
+        std::swap(B.segOf.p, B.segOf2.p);
+        std::swap(B.start.p, B.start2.p);
+        std::swap(B.n.p, B.n2.p);
+    }
+    const uint32_t lastDim = (I.node_depth - 1) % SD;
+    hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N,
+                       lastDim, nullptr, I.node_size, B.start.get(), B.n.get(), I.depth_skips, I.index_size,
+                       h.leaves.get());
+    KNN_CHECK_LAUNCH();
+    h.last_build_ms = timer.stop();
+    compute_bbox(pts, N, D, h.bbox.get(), st);
+}
+
+void build_ball(Handle& h, const float* pts, hipStream_t st, uint32_t block) {
+    (void)block;
+    knn_tree_info& I = h.info;
+    const uint32_t N = I.n, D = I.dims;
+    h.nodes.alloc((size_t)I.node_size * (2 * D + 2));
+    h.dims.alloc(I.node_size);
+    h.leaves.alloc((size_t)I.index_size * (1 + D));
+    h.bbox.alloc(2 * (size_t)D);
+    Timer timer(st);
+    if (N == 0) {
+        hipLaunchKernelGGL(fill_f32_kernel, dim3(nblk((size_t)I.node_size * (2 * D + 2), 256)), dim3(256), 0, st,
+                           h.nodes.get(), (size_t)I.node_size * (2 * D + 2), 0.0f);
+        launch_fill_u32(h.dims.get(), I.node_size, 0u, st);
+        fill_empty_leaves(h, st);
+        KNN_CHECK_LAUNCH();
+        h.last_build_ms = timer.stop();
+        compute_bbox(pts, 0, D, h.bbox.get(), st);
+        return;
+    }
+    LevelBufs B;
+    const uint32_t maxSeg = 1u << I.node_depth;
+    alloc_levels(B, N, D, maxSeg);
+    presort(B, pts, N, D, D, st);
+    DevBuf<float> cent, cent2;
+    DevBuf<double> var, var2, partial;
+    DevBuf<float> pmax;
+    cent.alloc((size_t)maxSeg * D);
+    cent2.alloc((size_t)maxSeg * D);
+    var.alloc((size_t)maxSeg * D);
+    var2.alloc((size_t)maxSeg * D);
+    launch_fill_u32(B.segOf.get(), N, 0u, st);
+    launch_fill_u32(B.start.get(), 1, 0u, st);
+    launch_fill_u32(B.n.get(), 1, N, st);
+    // Root centroid and deviation sums (no BallNode entry for the root).
+    seg_reduce(B.P.get(), pts, N, D, B.start.get(), B.n.get(), 1, 0, cent.get(), nullptr, nullptr, partial, pmax, st);
+    seg_reduce(B.P.get(), pts, N, D, B.start.get(), B.n.get(), 1, 1, cent.get(), var.get(), nullptr, partial, pmax, st);
+    for (uint32_t depth = 0; depth < I.node_depth; ++depth) {
+        const uint32_t S = 1u << depth;
+        hipLaunchKernelGGL(ball_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, var.get(), D, B.start.get(),
+                           B.n.get(), S, h.dims.get(), B.segDim.get(), B.start2.get(), B.n2.get(), B.Lcount.get());
+        level_common(B, N, S, 0, true, st);
+        partition_lists(B, N, D, 0xFFFFFFFFu, S, st);
+        std::swap(B.segOf.p, B.segOf2.p);
+        std::swap(B.start.p, B.start2.p);
+        std::swap(B.n.p, B.n2.p);
+        // Children (level depth + 1): centroids, then deviation sums + radii.
+        const uint32_t S2 = 2 * S;
+        seg_reduce(B.P.get(), pts, N, D, B.start.get(), B.n.get(), S2, 0, cent2.get(), nullptr, h.nodes.get(), partial,
+                   pmax, st);
+        seg_reduce(B.P.get(), pts, N, D, B.start.get(), B.n.get(), S2, 1, cent2.get(), var2.get(), h.nodes.get(),
+                   partial, pmax, st);
+        std::swap(cent.p, cent2.p);
+        std::swap(var.p, var2.p);
+    }
+    hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N, 0u,
+                       h.dims.get(), I.node_size, B.start.get(), B.n.get(), I.depth_skips, I.index_size,
+                       h.leaves.get());
+    KNN_CHECK_LAUNCH();
+    h.last_build_ms = timer.stop();
+    compute_bbox(pts, N, D, h.bbox.get(), st);
+}
+
+}  // namespace knn

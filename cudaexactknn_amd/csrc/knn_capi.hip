@@ -1,0 +1,286 @@
+// extern "C" entry points of libknn_mi355x.so (declared in include/knn_mi355x.h).
+// Each converts C++ exceptions into a status plus a thread-local message, the
+// C-ABI counterpart of the reference's Cuda::check -> CudaException
+// (src/KnnTrees.cuh:20-23, 197-202).
+#include <cstring>
+#include <mutex>
+
+#include "knn_internal.hpp"
+
+namespace knn {
+
+static thread_local std::string g_last_error;
+
+static knn_status fail(knn_status s, const std::string& m) {
+    g_last_error = m;
+    return s;
+}
+
+template <class F>
+static knn_status guarded(F&& f) {
+    try {
+        f();
+        return KNN_OK;
+    } catch (const Error& e) {
+        return fail(e.status, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(KNN_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(KNN_ERR_INTERNAL, e.what());
+    }
+}
+
+const DeviceLimits& device_limits() {
+    static DeviceLimits lim;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return;
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) != hipSuccess) return;
+        lim.max_lds_per_block = (uint32_t)p.sharedMemPerBlock;
+        lim.max_threads = (uint32_t)p.maxThreadsPerBlock;
+        lim.cu_count = (uint32_t)p.multiProcessorCount;
+    });
+    return lim;
+}
+
+Handle::~Handle() {
+    if (mlp) destroy_mlp(mlp);
+    magic = 0;
+}
+
+__global__ void fill_results_kernel(int32_t* __restrict__ oi, float* __restrict__ od, size_t n, float maxd2) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        oi[i] = -1;
+        od[i] = maxd2;
+    }
+}
+
+static void check_pts(const float* d_pts, uint32_t n, uint32_t dims) {
+    require(dims >= 1, "dims must be >= 1");
+    require(dims <= 4096, "dims above 4096 are not supported", KNN_ERR_UNSUPPORTED);
+    require(n < 0x80000000u, "point count must be below 2^31 (32-bit indices)");
+    require(n == 0 || d_pts != nullptr, "d_pts is NULL");
+}
+
+static hipStream_t null_stream() { return nullptr; }
+
+// Tuned defaults (BLOCK_SIZE, leaf span) when the caller passes 0.
+static uint32_t default_block(const Handle& h) { return h.kind == KNN_KIND_BRUTE ? 256 : 128; }
+static uint32_t default_span(const Handle& h) {
+    // Scan about 8 points per leaf visit.
+    const uint32_t s = h.info.depth_skips;
+    return s >= 3 ? 0 : 3 - s;
+}
+
+}  // namespace knn
+
+using namespace knn;
+
+extern "C" {
+
+uint32_t knn_version(void) { return 0x000100; }
+
+const char* knn_last_error(void) { return g_last_error.c_str(); }
+
+knn_status knn_bf_create(const float* d_pts, uint32_t n, uint32_t dims, void** out_handle) {
+    return guarded([&] {
+        require(out_handle != nullptr, "out_handle is NULL");
+        *out_handle = nullptr;
+        check_pts(d_pts, n, dims);
+        auto* h = new Handle();
+        try {
+            h->kind = KNN_KIND_BRUTE;
+            h->info.kind = KNN_KIND_BRUTE;
+            h->info.n = n;
+            h->info.dims = dims;
+            h->info.split_dims = dims;
+            h->info.leaf_bytes = 4 + 4 * dims;
+            KNN_HIP(hipGetDevice(&h->device));
+            h->pts.alloc((size_t)n * dims);
+            if (n) KNN_HIP(hipMemcpy(h->pts.get(), d_pts, sizeof(float) * (size_t)n * dims, hipMemcpyDeviceToDevice));
+            KNN_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out_handle = h;
+    });
+}
+
+knn_status knn_kd_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_t split_dims, uint32_t depth_skips,
+                        uint32_t build_block, void** out_handle) {
+    return guarded([&] {
+        require(out_handle != nullptr, "out_handle is NULL");
+        *out_handle = nullptr;
+        check_pts(d_pts, n, dims);
+        require(split_dims >= 1 && split_dims <= dims, "split_dims must be in [1, dims]");
+        require(build_block <= 1024, "build_block must be <= 1024");
+        auto* h = new Handle();
+        try {
+            h->kind = KNN_KIND_KD;
+            tree_sizing(h->info, n, dims, split_dims, depth_skips, false);
+            h->info.kind = KNN_KIND_KD;
+            KNN_HIP(hipGetDevice(&h->device));
+            build_kd(*h, d_pts, null_stream(), build_block);
+            KNN_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out_handle = h;
+    });
+}
+
+knn_status knn_ball_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_t depth_skips, uint32_t build_block,
+                          void** out_handle) {
+    return guarded([&] {
+        require(out_handle != nullptr, "out_handle is NULL");
+        *out_handle = nullptr;
+        check_pts(d_pts, n, dims);
+        require(build_block <= 1024, "build_block must be <= 1024");
+        auto* h = new Handle();
+        try {
+            h->kind = KNN_KIND_BALL;
+            tree_sizing(h->info, n, dims, dims, depth_skips, true);
+            h->info.kind = KNN_KIND_BALL;
+            KNN_HIP(hipGetDevice(&h->device));
+            build_ball(*h, d_pts, null_stream(), build_block);
+            KNN_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out_handle = h;
+    });
+}
+
+knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max_d2, int32_t* d_idx, float* d_dist,
+                     uint32_t k, const knn_query_opts* opts) {
+    return guarded([&] {
+        Handle& h = *as_handle(handle);
+        knn_query_opts o{};
+        if (opts) o = *opts;
+        require(k >= 1, "k must be >= 1");
+        require(k <= 4096, "k above 4096 is not supported", KNN_ERR_UNSUPPORTED);
+        if (q == 0) return;
+        require(d_q && d_idx && d_dist, "query or result pointer is NULL");
+        require(o.form <= KNN_FORM_LEARNED, "unknown form");
+        require(!(o.form == KNN_FORM_BF16 && h.kind != KNN_KIND_BRUTE), "the bf16 form applies to brute force only");
+        require(!(o.form == KNN_FORM_LEARNED && h.kind == KNN_KIND_BRUTE), "the learned form applies to trees only");
+        QueryArgs a{};
+        a.q = d_q;
+        a.nq = q;
+        a.maxd2 = max_d2;
+        a.oidx = d_idx;
+        a.odist = d_dist;
+        a.K = k;
+        a.block = o.block ? o.block : default_block(h);
+        require(a.block <= device_limits().max_threads, "BLOCK_SIZE above the device maximum");
+        a.heap = o.heap != 0;
+        a.track_lds = o.track_lds != 0;
+        a.stack_lds = o.stack_lds != 0;
+        a.sort_queries = (o.flags & KNN_FLAG_NO_QUERY_SORT) == 0;
+        a.leaf_span = o.leaf_span == 0 ? default_span(h) : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
+        a.form = (knn_form)o.form;
+        a.stream = (hipStream_t)o.stream;
+        hipEvent_t e0, e1;
+        KNN_HIP(hipEventCreate(&e0));
+        KNN_HIP(hipEventCreate(&e1));
+        KNN_HIP(hipEventRecord(e0, a.stream));
+        if (h.kind == KNN_KIND_BRUTE) {
+            if (a.form == KNN_FORM_BF16)
+                query_bf_bf16(h, a);
+            else
+                query_bf(h, a);
+        } else if (a.form == KNN_FORM_LEARNED) {
+            query_tree_learned(h, a);
+        } else {
+            uint32_t* perm = nullptr;
+            if (a.sort_queries && q > 1) order_queries(h, a, &perm);
+            query_tree(h, a, perm, nullptr, nullptr);
+        }
+        KNN_HIP(hipEventRecord(e1, a.stream));
+        if (opts == nullptr || opts->stream == nullptr) {
+            KNN_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            KNN_HIP(hipEventElapsedTime(&ms, e0, e1));
+            h.stats[3] = ms;
+            KNN_HIP(hipDeviceSynchronize());
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
+knn_status knn_destroy(void* handle) {
+    return guarded([&] {
+        if (!handle) return;
+        Handle* h = as_handle(handle);
+        delete h;
+    });
+}
+
+knn_status knn_pinned_alloc(size_t bytes, void** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        *out = nullptr;
+        KNN_HIP(hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault));
+    });
+}
+
+knn_status knn_pinned_free(void* p) {
+    return guarded([&] {
+        if (p) KNN_HIP(hipHostFree(p));
+    });
+}
+
+knn_status knn_tree_get_info(const void* handle, knn_tree_info* out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        *out = as_handle(handle)->info;
+    });
+}
+
+knn_status knn_tree_export(const void* handle, void* h_nodes, void* h_leaves, uint32_t* h_dims) {
+    return guarded([&] {
+        Handle& h = *as_handle(handle);
+        require(h.kind != KNN_KIND_BRUTE, "brute-force handles have no tree to export");
+        const knn_tree_info& I = h.info;
+        if (h_nodes)
+            KNN_HIP(hipMemcpy(h_nodes, h.nodes.get(), (size_t)I.node_size * I.node_bytes, hipMemcpyDeviceToHost));
+        if (h_leaves)
+            KNN_HIP(hipMemcpy(h_leaves, h.leaves.get(), (size_t)I.index_size * I.leaf_bytes, hipMemcpyDeviceToHost));
+        if (h_dims && h.kind == KNN_KIND_BALL)
+            KNN_HIP(hipMemcpy(h_dims, h.dims.get(), sizeof(uint32_t) * I.node_size, hipMemcpyDeviceToHost));
+    });
+}
+
+float knn_last_build_ms(const void* handle) {
+    const Handle* h = static_cast<const Handle*>(handle);
+    if (!h || h->magic != kHandleMagic) return -1.0f;
+    return h->last_build_ms;
+}
+
+knn_status knn_learned_train(void* handle, const float* d_sample, uint32_t n_sample, uint32_t k, uint32_t epochs,
+                             float lr, float* stats) {
+    return guarded([&] {
+        Handle& h = *as_handle(handle);
+        require(h.kind != KNN_KIND_BRUTE, "the learned radius applies to trees only");
+        require(n_sample >= 1 && d_sample != nullptr, "empty training sample");
+        require(k >= 1 && k <= 4096, "k out of range");
+        learned_train(h, d_sample, n_sample, k, epochs, lr, stats, nullptr);
+    });
+}
+
+knn_status knn_last_query_stats(const void* handle, float* out4) {
+    return guarded([&] {
+        require(out4 != nullptr, "out4 is NULL");
+        const Handle& h = *as_handle(handle);
+        std::memcpy(out4, h.stats, sizeof(h.stats));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,258 @@
+// Device-side primitives shared by every kernel of libknn_mi355x.
+//
+// Exact arithmetic contract (src/KnnTrees.cuh:65-77, 122-189 of the reference):
+//   d2(q,p) = sequential FP32 sum over dims of (q_d - p_d)^2, every product and
+//   sum rounded separately (no FMA contraction; the library is compiled with
+//   -ffp-contract=off and the functions below also say so locally);
+//   key = (bits(d2) << 32) | uint32(index); a candidate is accepted iff its key
+//   is below the current K-th key, whose initial value is (bits(maxD2), ~0).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace knn {
+
+constexpr uint64_t kEmptyLo = 0xFFFFFFFFull;
+
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+
+// KnnTrees.cuh:122-128
+__device__ __forceinline__ uint64_t make_key(float d2, uint32_t idx) {
+    return ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)idx;
+}
+__device__ __forceinline__ uint32_t key_hi(uint64_t k) { return (uint32_t)(k >> 32); }
+
+// KnnTrees.cuh:112-120: order-preserving bits of a signed float.
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    uint32_t mask = (uint32_t)((int32_t)u >> 31);
+    return (u ^ mask) | ((mask ? 0u : 1u) << 31);
+}
+
+// KnnTrees.cuh:65-77.  0 + t0*t0 == t0*t0 exactly, so the first add is elided.
+template <int D>
+__device__ __forceinline__ float dist2(const float* __restrict__ a, const float* __restrict__ b) {
+#pragma clang fp contract(off)
+    float t = a[0] - b[0];
+    float r = t * t;
+#pragma unroll
+    for (int d = 1; d < D; ++d) {
+        float u = a[d] - b[d];
+        float s = u * u;
+        r = r + s;
+    }
+    return r;
+}
+
+// Runtime-D variant for the generic kernels (q strided in LDS).
+__device__ __forceinline__ float dist2_rt(const float* __restrict__ q, uint32_t qstride,
+                                          const float* __restrict__ p, uint32_t D) {
+#pragma clang fp contract(off)
+    float t = q[0] - p[0];
+    float r = t * t;
+    for (uint32_t d = 1; d < D; ++d) {
+        float u = q[(size_t)d * qstride] - p[d];
+        float s = u * u;
+        r = r + s;
+    }
+    return r;
+}
+
+// Select q[dim] for a runtime dim without dynamic register indexing.
+template <int D>
+__device__ __forceinline__ float select_dim(const float (&q)[D], uint32_t dim) {
+    float v = q[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) v = (dim == (uint32_t)d) ? q[d] : v;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// K-buffers.  Every buffer exposes:
+//   init(empty_key, K), consider(key) -> new K-th key (only called when
+//   key < current K-th key), write(idx_row, dist_row, K).
+// Array buffers emit ascending keys (KnnTrees.cuh:143-149); heap buffers emit
+// descending keys, i.e. a valid max-heap with the K-th neighbour at slot 0.
+// ---------------------------------------------------------------------------
+
+// Sorted ascending array held in VGPRs, KCAP >= K static slots.  The first
+// KCAP-K slots hold key 0 (below every real key except (0.0, index 0), with
+// which it is interchangeable), so the K-th key always sits at slot KCAP-1
+// and every index is static (KnnTrees.cuh:136-152 insertion).
+template <int KCAP>
+struct RegArray {
+    uint64_t s[KCAP];
+    __device__ __forceinline__ uint64_t init(uint64_t empty, uint32_t K) {
+#pragma unroll
+        for (int i = 0; i < KCAP; ++i) s[i] = (i >= KCAP - (int)K) ? empty : 0ull;
+        return empty;
+    }
+    __device__ __forceinline__ uint64_t consider(uint64_t key) {
+#pragma unroll
+        for (int i = 0; i < KCAP; ++i) {
+            uint64_t e = s[i];
+            bool lt = e < key;
+            s[i] = lt ? e : key;
+            key = lt ? key : e;
+        }
+        return s[KCAP - 1];
+    }
+    __device__ __forceinline__ void write(int32_t* __restrict__ oi, float* __restrict__ od, uint32_t K) {
+#pragma unroll
+        for (int i = 0; i < KCAP; ++i) {
+            int j = i - (KCAP - (int)K);
+            if (j >= 0) {
+                oi[j] = (int32_t)(uint32_t)s[i];
+                od[j] = u2f(key_hi(s[i]));
+            }
+        }
+    }
+};
+
+// Strided slot storage for LDS (stride = workgroup size, conflict-free) or
+// private memory (stride 1).
+struct Slots {
+    uint64_t* p;
+    uint32_t stride;
+    __device__ __forceinline__ uint64_t& operator[](uint32_t i) const { return p[(size_t)i * stride]; }
+};
+
+// Sorted ascending array in LDS, runtime K (shift insertion).
+struct LdsArray {
+    Slots s;
+    uint32_t K;
+    __device__ __forceinline__ uint64_t init(uint64_t empty, uint32_t k) {
+        K = k;
+        for (uint32_t i = 0; i < K; ++i) s[i] = empty;
+        return empty;
+    }
+    __device__ __forceinline__ uint64_t consider(uint64_t key) {
+        uint32_t i = K - 1;
+        while (i > 0) {
+            uint64_t prev = s[i - 1];
+            if (prev < key) break;
+            s[i] = prev;
+            --i;
+        }
+        s[i] = key;
+        return s[K - 1];
+    }
+    __device__ __forceinline__ void write(int32_t* __restrict__ oi, float* __restrict__ od, uint32_t) {
+        for (uint32_t i = 0; i < K; ++i) {
+            uint64_t v = s[i];
+            oi[i] = (int32_t)(uint32_t)v;
+            od[i] = u2f(key_hi(v));
+        }
+    }
+};
+
+// Max-heap (KnnTrees.cuh:154-189, with the right-child read guarded).
+struct Heap {
+    Slots s;
+    uint32_t K;
+    __device__ __forceinline__ uint64_t init(uint64_t empty, uint32_t k) {
+        K = k;
+        for (uint32_t i = 0; i < K; ++i) s[i] = empty;
+        return empty;
+    }
+    __device__ __forceinline__ void sift(uint64_t key, uint32_t h, uint32_t n) {
+        for (;;) {
+            uint32_t l = 2 * h + 1;
+            if (l >= n) break;
+            uint32_t next = l;
+            uint64_t big = s[l];
+            if (l + 1 < n) {
+                uint64_t r = s[l + 1];
+                if (r > big) { big = r; next = l + 1; }
+            }
+            if (big <= key) break;
+            s[h] = big;
+            h = next;
+        }
+        s[h] = key;
+    }
+    __device__ __forceinline__ uint64_t consider(uint64_t key) {
+        sift(key, 0, K);
+        return s[0];
+    }
+    // In-place heapsort leaves keys ascending; emit them descending.
+    __device__ __forceinline__ void write(int32_t* __restrict__ oi, float* __restrict__ od, uint32_t) {
+        for (uint32_t n = K; n > 1; --n) {
+            uint64_t top = s[0];
+            uint64_t last = s[n - 1];
+            s[n - 1] = top;
+            sift(last, 0, n - 1);
+        }
+        for (uint32_t i = 0; i < K; ++i) {
+            uint64_t v = s[K - 1 - i];
+            oi[i] = (int32_t)(uint32_t)v;
+            od[i] = u2f(key_hi(v));
+        }
+    }
+};
+
+// Heap in per-lane private memory (scratch), runtime K <= KCAP.
+template <int KCAP>
+struct PrivHeap {
+    uint64_t mem[KCAP];
+    Heap h;
+    __device__ __forceinline__ uint64_t init(uint64_t empty, uint32_t k) {
+        h.s.p = mem;
+        h.s.stride = 1;
+        return h.init(empty, k);
+    }
+    __device__ __forceinline__ uint64_t consider(uint64_t key) { return h.consider(key); }
+    __device__ __forceinline__ void write(int32_t* oi, float* od, uint32_t K) { h.write(oi, od, K); }
+};
+
+// Buffer kinds selected by the host (USE_HEAP_TRACKING x USE_TRACKING_SHMEM).
+enum BufKind : int { BUF_REG8 = 0, BUF_REG16 = 1, BUF_REG32 = 2, BUF_LDS_ARRAY = 3, BUF_LDS_HEAP = 4, BUF_PRIV_HEAP = 5 };
+
+template <int KIND> struct BufOf;
+template <> struct BufOf<BUF_REG8> { using T = RegArray<8>; static constexpr bool lds = false; };
+template <> struct BufOf<BUF_REG16> { using T = RegArray<16>; static constexpr bool lds = false; };
+template <> struct BufOf<BUF_REG32> { using T = RegArray<32>; static constexpr bool lds = false; };
+template <> struct BufOf<BUF_LDS_ARRAY> { using T = LdsArray; static constexpr bool lds = true; };
+template <> struct BufOf<BUF_LDS_HEAP> { using T = Heap; static constexpr bool lds = true; };
+template <> struct BufOf<BUF_PRIV_HEAP> { using T = PrivHeap<32>; static constexpr bool lds = false; };
+
+template <class B>
+__device__ __forceinline__ void bind_lds(B&, uint64_t*, uint32_t) {}
+__device__ __forceinline__ void bind_lds(LdsArray& b, uint64_t* base, uint32_t stride) { b.s.p = base; b.s.stride = stride; }
+__device__ __forceinline__ void bind_lds(Heap& b, uint64_t* base, uint32_t stride) { b.s.p = base; b.s.stride = stride; }
+
+// Inclusive scan across a wave64.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread across the workgroup; returns the
+// thread's exclusive prefix and the workgroup total.  lds_w holds
+// (waves + 1) words.  Contains barriers: every thread must call it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds_w, uint32_t* total) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63 || threadIdx.x == blockDim.x - 1) lds_w[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < nw; ++w) { uint32_t t = lds_w[w]; lds_w[w] = run; run += t; }
+        lds_w[nw] = run;
+    }
+    __syncthreads();
+    uint32_t r = lds_w[wave] + inc - v;
+    *total = lds_w[nw];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace knn

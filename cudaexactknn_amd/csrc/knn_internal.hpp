@@ -1,0 +1,167 @@
+// Host-side internals of libknn_mi355x: handle layout, errors, device buffers
+// and the launch-facing declarations shared by the .hip translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/knn_mi355x.h"
+
+namespace knn {
+
+struct Error : std::runtime_error {
+    knn_status status;
+    Error(knn_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define KNN_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::knn::Error(e_ == hipErrorOutOfMemory ? KNN_ERR_OOM : KNN_ERR_HIP,              \
+                               std::string(#call) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+#define KNN_CHECK_LAUNCH() KNN_HIP(hipGetLastError())
+
+inline void require(bool ok, const std::string& msg, knn_status s = KNN_ERR_INVALID_ARG) {
+    if (!ok) throw Error(s, msg);
+}
+
+// Owning device buffer (grow-only when used as scratch).
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        void* q = nullptr;
+        KNN_HIP(hipMalloc(&q, count * sizeof(T)));
+        p = static_cast<T*>(q);
+        n = count;
+    }
+    void ensure(size_t count) {
+        if (count > n) alloc(count);
+    }
+    T* get() const { return p; }
+};
+
+constexpr uint32_t kHandleMagic = 0x4B4E4E35u;  // "KNN5"
+
+struct Mlp;  // learned radius predictor (knn_learned.hip)
+
+struct Handle {
+    uint32_t magic = kHandleMagic;
+    knn_kind kind = KNN_KIND_BRUTE;
+    knn_tree_info info{};
+    int device = 0;
+    DevBuf<float> pts;        // brute force: owned copy of the points
+    DevBuf<float> nodes;      // kd: float split values; ball: BallNode<D> array
+    DevBuf<uint32_t> dims;    // ball: split dimension per node
+    DevBuf<float> leaves;     // IndexPoint<D> array (int32 index + D floats)
+    DevBuf<float> bbox;       // 2 x D (min, max) of the points, for query ordering
+    DevBuf<float> norms;      // brute force: ||p||^2 (bf16 form)
+    DevBuf<uint16_t> pts_bf16;  // brute force: points rounded to bf16 (bf16 form)
+    // Query scratch (grow-only, reused across calls on this handle).
+    DevBuf<uint32_t> q_keys, q_vals, q_keys_alt, q_vals_alt, q_temp;
+    DevBuf<uint8_t> q_scratch;
+    float last_build_ms = -1.0f;
+    float stats[4] = {0, 0, 0, 0};
+    Mlp* mlp = nullptr;
+    ~Handle();
+};
+
+inline Handle* as_handle(const void* h) {
+    Handle* p = const_cast<Handle*>(static_cast<const Handle*>(h));
+    require(p != nullptr && p->magic == kHandleMagic, "invalid knn handle");
+    return p;
+}
+
+// Device properties cached per process.
+struct DeviceLimits {
+    uint32_t max_lds_per_block = 65536;
+    uint32_t max_threads = 1024;
+    uint32_t cu_count = 256;
+};
+const DeviceLimits& device_limits();
+
+// knn_prims.hip
+size_t scan_temp_words(uint32_t n);
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* temp, hipStream_t st);
+size_t radix_temp_words(uint32_t n);
+bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
+                      int begin_bit, int end_bit, uint32_t* temp, hipStream_t st);
+void launch_iota(uint32_t* a, uint32_t n, hipStream_t st);
+void launch_fill_u32(uint32_t* a, uint32_t n, uint32_t v, hipStream_t st);
+
+// Tree sizing shared by both trees (src/KdTree.cuh:56-62, src/BallTree.cuh:123-128).
+inline uint32_t log2_ceil_host(uint32_t x) {
+    uint32_t c = 2, r = 1;
+    while (c < x) { c *= 2; r += 1; }
+    return r;
+}
+inline void tree_sizing(knn_tree_info& t, uint32_t n, uint32_t D, uint32_t SD, uint32_t skips, bool ball) {
+    uint32_t L = log2_ceil_host(n);
+    uint32_t s = skips;
+    if (ball && s < 1) s = 1;
+    if (s > L - 1) s = L - 1;
+    t.n = n;
+    t.dims = D;
+    t.split_dims = SD;
+    t.depth_skips = s;
+    t.node_depth = L - s;
+    t.max_children = 1u << s;
+    t.index_size = 1u << (t.node_depth + s);
+    t.node_size = (1u << t.node_depth) - 1u;
+    t.leaf_bytes = 4 + 4 * D;
+    t.node_bytes = ball ? 8 * (D + 1) : 4;
+}
+
+// Query parameters after option resolution.
+struct QueryArgs {
+    const float* q;
+    uint32_t nq;
+    float maxd2;
+    int32_t* oidx;
+    float* odist;
+    uint32_t K;
+    uint32_t block;
+    bool heap, track_lds, stack_lds, sort_queries;
+    uint32_t leaf_span;  // extra levels folded into one leaf scan
+    knn_form form;
+    hipStream_t stream;
+};
+
+// Builders (knn_build.hip)
+void build_kd(Handle& h, const float* d_pts, hipStream_t st, uint32_t block);
+void build_ball(Handle& h, const float* d_pts, hipStream_t st, uint32_t block);
+void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hipStream_t st);
+
+// Queries
+void query_bf(Handle& h, const QueryArgs& a);               // knn_bf.hip
+void query_bf_bf16(Handle& h, const QueryArgs& a);          // knn_bf16.hip
+void prepare_bf16(Handle& h, hipStream_t st);               // knn_bf16.hip
+void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
+                uint32_t* d_short_flags);                    // knn_tree_query.hip
+void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm);  // knn_tree_query.hip
+
+// Learned radius (knn_learned.hip)
+void learned_train(Handle& h, const float* d_sample, uint32_t n_sample, uint32_t k, uint32_t epochs, float lr,
+                   float* stats, hipStream_t st);
+void query_tree_learned(Handle& h, const QueryArgs& a);
+void destroy_mlp(Mlp* m);
+
+}  // namespace knn

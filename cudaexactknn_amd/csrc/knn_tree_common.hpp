@@ -1,0 +1,159 @@
+// Shared pieces of the kd / ball traversal kernels.
+//
+// Both trees use the reference's implicit BFS node layout (children of node i
+// at 2i+1, 2i+2; src/KnnTrees.cuh:95-103) and its leaf layout: IndexPoint<D>
+// buckets of 2^depthSkips slots padded with index -1 (src/KnnTrees.cuh:25-29,
+// src/KdTreeGpu.cuh:354-363).
+//
+// Pruning is certified and non-strict: a subtree is skipped only when
+// bits(bound) > high word of the current K-th key, where `bound` is a lower
+// bound on every FP32 distance the subtree can produce.  That makes every tree
+// return exactly the brute-force result (the reference prunes with
+// `bound >= maxDistance2`, src/KdTreeGpu.cuh:55,71, and can drop exact ties).
+#pragma once
+#include "knn_device.hpp"
+
+namespace knn {
+
+struct TreeArgs {
+    const float* nodes;
+    const float* leaves;   // IndexPoint<D>: int32 index followed by D floats
+    uint32_t node_depth;
+    uint32_t node_size;
+    uint32_t max_children;
+    uint32_t split_dims;
+    uint32_t sd_inv;       // ceil(65536 / split_dims): level % SD without a divide
+    uint32_t stop_level;   // descend to this level, then scan every leaf slot below
+    uint32_t stack_depth;  // LDS stack entries per lane
+    uint32_t dims;         // runtime D (generic kernels)
+    const float* qs;
+    const uint32_t* perm;  // optional processing order
+    uint32_t nq;
+    float maxd2;
+    uint32_t K;
+    int32_t* oidx;
+    float* odist;
+    const float* init_r2;     // optional per-query starting radius^2 (learned form)
+    uint32_t* short_flags;    // optional: set to 1 for a query whose tightened radius left a slot empty
+};
+
+// Query coordinates: compile-time D in VGPRs, or runtime D lane-interleaved in LDS.
+template <int D>
+struct QPoint {
+    float v[D];
+    __device__ __forceinline__ void load(const float* __restrict__ qs, uint32_t qi, uint32_t, float*, uint32_t,
+                                         uint32_t) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) v[d] = qs[(size_t)qi * D + d];
+    }
+    __device__ __forceinline__ float at(uint32_t dim) const { return select_dim<D>(v, dim); }
+    __device__ __forceinline__ float dist_to(const float* __restrict__ p) const { return dist2<D>(v, p); }
+    __device__ __forceinline__ uint32_t dims() const { return D; }
+};
+
+template <>
+struct QPoint<0> {
+    float* base;
+    uint32_t stride, D;
+    __device__ __forceinline__ void load(const float* __restrict__ qs, uint32_t qi, uint32_t Drt, float* lds,
+                                         uint32_t BS, uint32_t tid) {
+        base = lds + tid;
+        stride = BS;
+        D = Drt;
+        for (uint32_t d = 0; d < D; ++d) base[(size_t)d * BS] = qs[(size_t)qi * D + d];
+    }
+    __device__ __forceinline__ float at(uint32_t dim) const { return base[(size_t)dim * stride]; }
+    __device__ __forceinline__ float dist_to(const float* __restrict__ p) const { return dist2_rt(base, stride, p, D); }
+    __device__ __forceinline__ uint32_t dims() const { return D; }
+};
+
+// Scan leaf slots [slot0, slot0 + count): each slot is an IndexPoint<D>.
+// `stop_at_empty`: a single real bucket, whose padding is a suffix.
+template <int D, class Q, class B>
+__device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uint32_t stride, uint32_t slot0,
+                                           uint32_t count, bool stop_at_empty, const Q& q, B& buf,
+                                           uint64_t& worst, uint32_t& whi) {
+    for (uint32_t o = 0; o < count; ++o) {
+        const float* e = leaves + (size_t)(slot0 + o) * stride;
+        float d2;
+        int32_t pi;
+        if constexpr (D == 3) {
+            const float4 v = *reinterpret_cast<const float4*>(e);
+            pi = __float_as_int(v.x);
+            const float p[3] = {v.y, v.z, v.w};
+            d2 = q.dist_to(p);
+        } else {
+            pi = __float_as_int(e[0]);
+            d2 = q.dist_to(e + 1);
+        }
+        if (pi == -1) {
+            if (stop_at_empty) break;
+            continue;
+        }
+        if (f2u(d2) <= whi) {
+            const uint64_t key = make_key(d2, (uint32_t)pi);
+            if (key < worst) {
+                worst = buf.consider(key);
+                whi = key_hi(worst);
+            }
+        }
+    }
+}
+
+// Leaf-slot range under `node` (at `level`), scanning the full subtree when
+// level < node_depth.
+__device__ __forceinline__ void leaf_range(const TreeArgs& A, uint32_t node, uint32_t level, uint32_t& slot0,
+                                           uint32_t& count) {
+    const uint32_t down = A.node_depth - level;
+    const uint32_t leftmost = ((node + 1) << down) - 1;
+    slot0 = (leftmost - A.node_size) * A.max_children;
+    count = A.max_children << down;
+}
+
+__device__ __forceinline__ uint32_t level_dim(uint32_t level, uint32_t sd, uint32_t sd_inv) {
+    return level - sd * ((level * sd_inv) >> 16);
+}
+
+// Learned-form bookkeeping and the canonical write.
+template <class B>
+__device__ __forceinline__ void finish_query(const TreeArgs& A, uint32_t qi, B& buf, uint64_t worst, bool tightened) {
+    buf.write(A.oidx + (size_t)qi * A.K, A.odist + (size_t)qi * A.K, A.K);
+    if (tightened && A.short_flags) {
+        // An empty slot remains iff the K-th key is still an empty key (index ~0).
+        if ((uint32_t)worst == 0xFFFFFFFFu) A.short_flags[qi] = 1u;
+    }
+}
+
+// Starting K-th key: (bits(maxD2), ~0), tightened by the learned radius when given.
+__device__ __forceinline__ uint64_t start_key(const TreeArgs& A, uint32_t qi, bool& tightened) {
+    const uint64_t base = ((uint64_t)f2u(A.maxd2) << 32) | kEmptyLo;
+    tightened = false;
+    if (A.init_r2) {
+        const float r2 = A.init_r2[qi];
+        const uint64_t k2 = ((uint64_t)f2u(r2) << 32) | kEmptyLo;
+        if (r2 >= 0.0f && k2 < base) {
+            tightened = true;
+            return k2;
+        }
+    }
+    return base;
+}
+
+// Certified lower bound on the computed FP32 distance from q to any point in
+// the ball (c, r); r is stored as an upper bound of the exact radius.  The
+// float sequence is identical to the oracle's.  `border` (= A - r, unclamped)
+// orders the children like the reference (src/BallTreeGpu.cuh:70).
+template <class Q>
+__device__ __forceinline__ float ball_bound(const Q& q, const float* __restrict__ c, float r, float shrink_a,
+                                            float shrink_b, float& border) {
+#pragma clang fp contract(off)
+    const float d2 = q.dist_to(c);
+    const float a = sqrtf(d2) * shrink_a;
+    const float b = a - r;
+    border = b;
+    if (!(b > 0.0f)) return 0.0f;
+    const float l = (b * b) * shrink_b;
+    return l < 0x1.0p-100f ? 0.0f : l;
+}
+
+}  // namespace knn

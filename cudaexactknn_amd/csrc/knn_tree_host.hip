@@ -1,0 +1,116 @@
+// Host dispatch of the tree traversal kernels and query ordering.
+//
+// Query ordering: queries are visited in Morton order of their coordinates
+// inside the points' bounding box (at most 30 code bits over the first
+// min(D, 30) dims), so the 64 lanes of a wave traverse neighbouring paths:
+// their node and leaf loads hit the same cache lines and their stacks diverge
+// less.  Results are still written to each query's own row.
+#include "knn_tree_common.hpp"
+#include "knn_internal.hpp"
+
+namespace knn {
+
+using TreeFn = void (*)(TreeArgs);
+using BallFn = void (*)(TreeArgs, float, float);
+TreeFn kd_kernel_for(uint32_t D, int kind, bool stack_lds, bool& generic);
+BallFn ball_kernel_for(uint32_t D, int kind, bool stack_lds, bool& generic);
+int pick_buf_kind(bool heap, bool lds, uint32_t K);
+
+__global__ void morton_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D, const float* __restrict__ bbox,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const uint32_t Dm = D < 30 ? D : 30;
+    uint32_t bpd = 30 / Dm;
+    if (bpd > 10) bpd = 10;
+    const float scale = (float)(1u << bpd);
+    uint32_t code = 0;
+    for (uint32_t d = 0; d < Dm; ++d) {
+        const float lo = bbox[d], hi = bbox[D + d];
+        const float x = qs[(size_t)i * D + d];
+        const float w = hi - lo;
+        float u = w > 0.0f ? (x - lo) / w * scale : 0.0f;
+        u = u < 0.0f ? 0.0f : u;  // also maps NaN to 0
+        uint32_t c = u >= scale ? (1u << bpd) - 1u : (uint32_t)u;
+        for (uint32_t b = 0; b < bpd; ++b) code |= ((c >> b) & 1u) << (b * Dm + (Dm - 1 - d));
+    }
+    keys[i] = code;
+    vals[i] = i;
+}
+
+void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
+    const uint32_t n = a.nq, D = h.info.dims;
+    h.q_keys.ensure(n);
+    h.q_vals.ensure(n);
+    h.q_keys_alt.ensure(n);
+    h.q_vals_alt.ensure(n);
+    h.q_temp.ensure(radix_temp_words(n));
+    hipLaunchKernelGGL(morton_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D, h.bbox.get(),
+                       h.q_keys.get(), h.q_vals.get());
+    const uint32_t Dm = D < 30 ? D : 30;
+    uint32_t bpd = 30 / Dm;
+    if (bpd > 10) bpd = 10;
+    const int bits = (int)(bpd * Dm);
+    const int end = (bits + 7) / 8 * 8;
+    bool alt = radix_sort_pairs(h.q_keys.get(), h.q_vals.get(), h.q_keys_alt.get(), h.q_vals_alt.get(), n, 0, end,
+                                h.q_temp.get(), a.stream);
+    KNN_CHECK_LAUNCH();
+    *d_perm = alt ? h.q_vals_alt.get() : h.q_vals.get();
+}
+
+void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
+                uint32_t* d_short_flags) {
+    const knn_tree_info& I = h.info;
+    TreeArgs A{};
+    A.nodes = h.nodes.get();
+    A.leaves = h.leaves.get();
+    A.node_depth = I.node_depth;
+    A.node_size = I.node_size;
+    A.max_children = I.max_children;
+    A.split_dims = I.split_dims;
+    A.sd_inv = (65536u + I.split_dims - 1) / I.split_dims;
+    const uint32_t span = a.leaf_span < I.node_depth ? a.leaf_span : I.node_depth;
+    A.stop_level = I.node_depth - span;
+    A.stack_depth = A.stop_level + 1;
+    A.dims = I.dims;
+    A.qs = a.q;
+    A.perm = d_perm;
+    A.nq = a.nq;
+    A.maxd2 = a.maxd2;
+    A.K = a.K;
+    A.oidx = a.oidx;
+    A.odist = a.odist;
+    A.init_r2 = d_init_r2;
+    A.short_flags = d_short_flags;
+
+    const int kind = pick_buf_kind(a.heap, a.track_lds, a.K);
+    const bool lds_buf = kind == BUF_LDS_ARRAY || kind == BUF_LDS_HEAP;
+    bool generic = false;
+    TreeFn kfn = nullptr;
+    BallFn bfn = nullptr;
+    if (h.kind == KNN_KIND_KD)
+        kfn = kd_kernel_for(I.dims, kind, a.stack_lds, generic);
+    else
+        bfn = ball_kernel_for(I.dims, kind, a.stack_lds, generic);
+    const uint32_t BS = a.block;
+    size_t lds = 0;
+    if (a.stack_lds) lds += (size_t)BS * A.stack_depth * 8;
+    if (lds_buf) lds += (size_t)BS * a.K * 8;
+    if (generic) lds += (size_t)BS * I.dims * 4;
+    require(lds <= device_limits().max_lds_per_block,
+            "LDS request of " + std::to_string(lds) + " bytes exceeds the per-workgroup limit of " +
+                std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",
+            KNN_ERR_INVALID_ARG);
+    const uint32_t grid = (a.nq + BS - 1) / BS;
+    if (kfn) {
+        hipLaunchKernelGGL(kfn, dim3(grid), dim3(BS), lds, a.stream, A);
+    } else {
+        const uint32_t D = I.dims;
+        const float shrink_a = 1.0f - (float)(D + 8) * 0x1.0p-23f;
+        const float shrink_b = 1.0f - (float)(2 * D + 16) * 0x1.0p-23f;
+        hipLaunchKernelGGL(bfn, dim3(grid), dim3(BS), lds, a.stream, A, shrink_a, shrink_b);
+    }
+    KNN_CHECK_LAUNCH();
+}
+
+}  // namespace knn

@@ -1,0 +1,156 @@
+/*
+ * knn_mi355x.h -- C ABI of libknn_mi355x.so, the MI355X (gfx950) exact-kNN
+ * library.  Plain pointers and sizes only; no HIP or torch types.
+ *
+ * The reference (rezvoj/CudaExactKnn) is a header-only C++ template library
+ * with no C ABI: its drop-in surface is the KnnTrees class templates.  Each
+ * entry point below names the reference interface it replaces; the C++
+ * header KnnTrees.hpp maps the templates onto these calls one to one.
+ *
+ * Conventions
+ *  - Device pointers (d_*) are HIP device allocations on the current device.
+ *  - Points and queries are row-major float32 [count][dims]
+ *    (KnnTrees::Array<float, D>, src/KnnTrees.cuh:7-18).
+ *  - Results are row-major [q][k] int32 indices and float32 squared
+ *    distances (Query::rIndexes / rDistances, src/KnnTrees.cuh:224-231).
+ *  - Every call is synchronous unless knn_query_opts.stream is non-NULL, in
+ *    which case work is enqueued on that hipStream_t and the call returns
+ *    after enqueueing (the caller synchronises).
+ *  - A non-zero status leaves a thread-local message in knn_last_error().
+ *
+ * Result semantics (identical for every structure and option):
+ *  - the K smallest keys (bits(d2) << 32 | index) among points whose key is
+ *    below (bits(max_d2) << 32 | 0xFFFFFFFF); d2 is the sequential FP32 sum of
+ *    squared differences (src/KnnTrees.cuh:65-77), bit-exact to the
+ *    reference's CPU BruteForce (src/BruteForce.cuh:27-44);
+ *  - array mode (heap = 0) writes keys ascending; heap mode writes them
+ *    descending (a valid max-heap, K-th neighbour first);
+ *  - unfilled slots hold index -1 and distance max_d2 (bits preserved);
+ *  - trees prune only with certified lower bounds, so they return exactly
+ *    the brute-force result, ties and negative max_d2 included.
+ */
+#ifndef KNN_MI355X_H
+#define KNN_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum knn_status {
+    KNN_OK = 0,
+    KNN_ERR_INVALID_ARG = 1,  /* bad size / pointer / option */
+    KNN_ERR_HIP = 2,          /* HIP runtime error (message in knn_last_error) */
+    KNN_ERR_OOM = 3,          /* device allocation failed */
+    KNN_ERR_UNSUPPORTED = 4,  /* e.g. dims or K beyond the supported maximum */
+    KNN_ERR_INTERNAL = 5
+} knn_status;
+
+typedef enum knn_kind { KNN_KIND_BRUTE = 0, KNN_KIND_KD = 1, KNN_KIND_BALL = 2 } knn_kind;
+
+/* Query form.  PLAIN is the exact FP32 path; BF16 (brute force only) filters
+ * with bf16 MFMA dot products and a certified error bound, then reranks the
+ * survivors exactly; LEARNED (trees only) starts from a predicted radius and
+ * re-runs the plain traversal for any query the prediction left short.  All
+ * forms return identical results. */
+typedef enum knn_form { KNN_FORM_PLAIN = 0, KNN_FORM_BF16 = 1, KNN_FORM_LEARNED = 2 } knn_form;
+
+/* Flags for knn_query_opts.flags. */
+#define KNN_FLAG_NO_QUERY_SORT 0x1u  /* trees: keep caller order (no Morton reordering) */
+
+typedef struct knn_query_opts {
+    uint32_t block;      /* BLOCK_SIZE: threads per workgroup, 1..1024 (0 = tuned default) */
+    uint32_t heap;       /* USE_HEAP_TRACKING */
+    uint32_t track_lds;  /* USE_TRACKING_SHMEM: K-buffer in LDS */
+    uint32_t stack_lds;  /* USE_STACK_SHMEM (trees): per-lane stack in LDS; 0 = register bit-stack */
+    uint32_t form;       /* knn_form */
+    uint32_t flags;      /* KNN_FLAG_* */
+    uint32_t leaf_span;  /* trees: scan 2^leaf_span leaf buckets per visit (0 = tuned default,
+                            0xFFFFFFFF = the tree's own buckets only) */
+    uint32_t reserved;
+    void* stream;        /* hipStream_t, NULL = synchronous on the null stream */
+} knn_query_opts;
+
+typedef struct knn_tree_info {
+    uint32_t kind;          /* knn_kind */
+    uint32_t n;             /* points */
+    uint32_t dims;          /* D */
+    uint32_t split_dims;    /* kd: SPLIT_DIMENSIONS; ball/brute: D */
+    uint32_t depth_skips;   /* effective depthSkips after clamping */
+    uint32_t node_depth;    /* levels of internal nodes */
+    uint32_t max_children;  /* leaf bucket capacity = 2^depth_skips */
+    uint32_t node_size;     /* internal nodes = 2^node_depth - 1 */
+    uint32_t index_size;    /* leaf slots = 2^(node_depth + depth_skips) */
+    uint32_t node_bytes;    /* bytes per node: 4 (kd float) or 8(D+1) (BallNode<D>) */
+    uint32_t leaf_bytes;    /* bytes per leaf slot: 4 + 4D (IndexPoint<D>) */
+    uint32_t reserved;
+} knn_tree_info;
+
+/* Library version as 0xMMmmpp. */
+uint32_t knn_version(void);
+
+/* Replaces BruteForceGpu<D>::BruteForceGpu(const Array<float,D>* points, uint size)
+ * (src/BruteForceGpu.cuh:64-68): copies d_pts (device, n x dims) into the handle. */
+knn_status knn_bf_create(const float* d_pts, uint32_t n, uint32_t dims, void** out_handle);
+
+/* Replaces KdTreeGpu<D,SPLIT_DIMENSIONS,BLOCK_SIZE>::KdTreeGpu(points, size, depthSkips)
+ * (src/KdTreeGpu.cuh:375-385).  build_block is BLOCK_SIZE (0 = default). */
+knn_status knn_kd_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_t split_dims,
+                        uint32_t depth_skips, uint32_t build_block, void** out_handle);
+
+/* Replaces BallTreeGpu<D,BLOCK_SIZE>::BallTreeGpu(points, size, depthSkips = 1)
+ * (src/BallTreeGpu.cuh:614-624). */
+knn_status knn_ball_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_t depth_skips,
+                          uint32_t build_block, void** out_handle);
+
+/* Replaces batchKnn<K,BLOCK_SIZE,USE_HEAP_TRACKING,USE_TRACKING_SHMEM[,USE_STACK_SHMEM]>(Query&)
+ * of all three GPU classes (src/BruteForceGpu.cuh:74-83, src/KdTreeGpu.cuh:393-410,
+ * src/BallTreeGpu.cuh:631-648).  d_q: q x dims; d_idx: q x k; d_dist: q x k.
+ * opts may be NULL (defaults). */
+knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max_d2,
+                     int32_t* d_idx, float* d_dist, uint32_t k, const knn_query_opts* opts);
+
+/* Frees a handle (the reference destructors, e.g. src/KdTreeGpu.cuh:387-390). */
+knn_status knn_destroy(void* handle);
+
+/* Thread-local description of the last failing call on this thread
+ * (replaces CudaException's what(), src/KnnTrees.cuh:20-23). */
+const char* knn_last_error(void);
+
+/* Replace pinnedMalloc / pinnedFree (src/KnnTrees.cuh:233-243). */
+knn_status knn_pinned_alloc(size_t bytes, void** out);
+knn_status knn_pinned_free(void* p);
+
+/* Structure introspection and export (no reference counterpart: the
+ * reference keeps its buffers private).  Export copies the node array
+ * (node_size * node_bytes) and the leaf array (index_size * leaf_bytes, the
+ * IndexPoint<D> layout) to host memory; ball trees also write each node's
+ * split dimension to h_dims (may be NULL). */
+knn_status knn_tree_get_info(const void* handle, knn_tree_info* out);
+knn_status knn_tree_export(const void* handle, void* h_nodes, void* h_leaves, uint32_t* h_dims);
+
+/* Milliseconds the last build on this handle spent on the device
+ * (hipEvent-timed), or -1. */
+float knn_last_build_ms(const void* handle);
+
+/* Learned radius (KNN_FORM_LEARNED): train the per-tree MLP radius
+ * predictor on the GPU from exact radii of `n_sample` sample queries
+ * (d_sample: n_sample x dims), for neighbour count k.  epochs/lr as given
+ * (0 = defaults).  Stats (may be NULL): [0] final training loss,
+ * [1] training ms. */
+knn_status knn_learned_train(void* handle, const float* d_sample, uint32_t n_sample, uint32_t k,
+                             uint32_t epochs, float lr, float* stats);
+
+/* Statistics of the last knn_query on this handle (may be used after the
+ * call returns):  [0] queries re-run by the plain traversal (learned form),
+ * [1] BF16 survivor fraction (bf16 form), [2] mean predicted / exact radius
+ * ratio (learned form), [3] kernel ms of the query (hipEvent-timed). */
+knn_status knn_last_query_stats(const void* handle, float* out4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KNN_MI355X_H */

@@ -1,0 +1,239 @@
+"""GPU parity tests: the HIP library through its C ABI against the CPU oracle.
+
+Bar: indices and distance bits identical to the oracle's BruteForce (which
+restates src/BruteForce.cuh:27-44) for every structure, option and edge case;
+tree structures identical to the oracle's (kd: split values and leaf buckets
+bit for bit; ball: dims and leaves, centroids within one FP32 ulp).
+"""
+import numpy as np
+import pytest
+
+import cudaexactknn_amd as knn
+from cudaexactknn_amd import datasets
+from tests import pins
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x, cuda):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(cuda)
+
+
+def run(struct, qs, k, cuda, md=np.inf, **opts):
+    import torch
+    q = dev(qs, cuda)
+    ri = torch.empty((len(qs), k), dtype=torch.int32, device=cuda)
+    rd = torch.empty((len(qs), k), dtype=torch.float32, device=cuda)
+    struct.batchKnn(knn.Query(q, len(qs), md, ri, rd), k, **opts)
+    torch.cuda.synchronize()
+    return ri.cpu().numpy(), rd.cpu().numpy()
+
+
+def same(a, b, what=""):
+    (ai, ad), (bi, bd) = a, b
+    bad = np.nonzero(~(np.all(ai == bi, axis=1) & np.all(ad.view(np.uint32) == bd.view(np.uint32), axis=1)))[0]
+    assert len(bad) == 0, f"{what}: {len(bad)} rows differ, first {bad[:5]}: got {ai[bad[0]]} want {bi[bad[0]]}"
+
+
+BF_VARIANTS = [dict(USE_HEAP_TRACKING=h, USE_TRACKING_SHMEM=s) for h in (False, True) for s in (False, True)]
+TREE_VARIANTS = [dict(USE_HEAP_TRACKING=h, USE_TRACKING_SHMEM=s, USE_STACK_SHMEM=t)
+                 for h in (False, True) for s in (False, True) for t in (False, True)]
+
+
+@pytest.mark.parametrize("d", [2, 3, 4, 5, 8, 16, 33, 128])
+def test_bf_dims_and_variants(oracle, cuda, d):
+    g = np.random.Generator(np.random.Philox(d))
+    pts = (g.standard_normal((1500, d)) * np.exp(g.uniform(-1, 1, d))).astype(np.float32)
+    qs = (g.standard_normal((300, d)) * 1.5).astype(np.float32)
+    bf = knn.BruteForceGpu(dev(pts, cuda))
+    for k in (1, 8, 10, 32, 40):
+        for v in BF_VARIANTS:
+            want = oracle.bf_knn(pts, qs, k, heap=v["USE_HEAP_TRACKING"])
+            same(run(bf, qs, k, cuda, BLOCK_SIZE=64, **v), want, f"d={d} k={k} {v}")
+
+
+def test_bf_lattice_and_radius_edges(oracle, cuda):
+    pts, qs = pins.lattice_case(33, 500, 90, 3)
+    bf = knn.BruteForceGpu(dev(pts, cuda))
+    for md in (np.inf, 0.0, -0.0, -2.0, 1.0, 4.0, float("nan"), 1e-40):
+        for v in BF_VARIANTS:
+            for k in (3, 16):
+                ei, ed = pins.exact_lattice_knn(pts, qs, k, md, v["USE_HEAP_TRACKING"])
+                same(run(bf, qs, k, cuda, md=md, BLOCK_SIZE=64, **v), (ei, ed), f"md={md} {v}")
+
+
+def test_golden_fixture_all_structures(cuda):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "lattice_d3.npz"))
+    p = dev(g["points"], cuda)
+    structs = [knn.BruteForceGpu(p), knn.KdTreeGpu(p, depth_skips=0), knn.KdTreeGpu(p, depth_skips=2),
+               knn.BallTreeGpu(p, depth_skips=1), knn.BallTreeGpu(p, depth_skips=3)]
+    for case in range(int(g["ncases"])):
+        md, heap, k = float(g[f"maxd2_{case}"]), bool(g[f"heap_{case}"]), int(g[f"k_{case}"])
+        for s in structs:
+            got = run(s, g["queries"], k, cuda, md=md, BLOCK_SIZE=32 if k > 32 else 0, USE_HEAP_TRACKING=heap)
+            same(got, (g[f"idx_{case}"], g[f"dist_{case}"]), f"case {case} {type(s).__name__}")
+
+
+def test_bf_tiny_sizes(oracle, cuda):
+    for n in (0, 1, 2, 3, 7):
+        pts = np.arange(n * 3, dtype=np.float32).reshape(n, 3) * 0.25
+        qs = np.random.Generator(np.random.Philox(n)).random((5, 3), dtype=np.float32)
+        if n == 0:
+            import torch
+            bf = knn.BruteForceGpu(torch.empty((0, 3), dtype=torch.float32, device=cuda))
+        else:
+            bf = knn.BruteForceGpu(dev(pts, cuda))
+        for k in (1, 4, 9):
+            same(run(bf, qs, k, cuda, md=3.0), oracle.bf_knn(pts.reshape(n, 3), qs, k, max_d2=3.0), f"n={n} k={k}")
+
+
+def test_bf_special_values(oracle, cuda):
+    g = np.random.Generator(np.random.Philox(99))
+    pts = g.standard_normal((400, 4)).astype(np.float32)
+    pts[0] = [-0.0, 0.0, -0.0, 0.0]
+    pts[1] = [1e-41, -1e-42, 3e-39, 0.0]      # denormals
+    pts[2] = [3e38, -3e38, 1e30, 0.0]         # distances overflow to +inf
+    pts[3] = pts[4]
+    qs = g.standard_normal((64, 4)).astype(np.float32)
+    qs[0] = pts[1]
+    qs[1] = [-3e38, 3e38, 0.0, 0.0]
+    bf = knn.BruteForceGpu(dev(pts, cuda))
+    for v in BF_VARIANTS:
+        same(run(bf, qs, 12, cuda, **v), oracle.bf_knn(pts, qs, 12, heap=v["USE_HEAP_TRACKING"]), str(v))
+        same(run(bf, qs, 450, cuda, BLOCK_SIZE=16, **v), oracle.bf_knn(pts, qs, 450, heap=v["USE_HEAP_TRACKING"]),
+             "k>n " + str(v))
+
+
+@pytest.mark.parametrize("d,sd,skips", [(3, 3, 0), (3, 3, 3), (4, 4, 0), (4, 3, 3), (8, 8, 2), (2, 1, 1),
+                                        (5, 5, 1), (6, 2, 0)])
+def test_kd_structure_matches_oracle(oracle, cuda, d, sd, skips):
+    g = np.random.Generator(np.random.Philox(d * 100 + skips))
+    pts = g.random((3001, d), dtype=np.float32)
+    pts[:100, 0] = 0.25
+    pts[200] = pts[201]
+    t = knn.KdTreeGpu(dev(pts, cuda), depth_skips=skips, split_dimensions=sd)
+    o = oracle.KdTree(pts, skips, sd)
+    info = t.info()
+    for key in ("node_depth", "max_children", "node_size", "index_size"):
+        assert info[key] == o.info()[key]
+    nodes, leaf_idx, leaf_pts, _ = t.export()
+    onodes, oleaf = o.export()
+    assert np.array_equal(nodes.view(np.uint32), onodes.view(np.uint32))
+    assert np.array_equal(leaf_idx, oleaf)
+    m = leaf_idx >= 0
+    assert np.array_equal(leaf_pts[m], pts[leaf_idx[m]])
+
+
+@pytest.mark.parametrize("d,skips", [(3, 1), (3, 0), (4, 2), (8, 1), (5, 3)])
+def test_ball_structure_matches_oracle(oracle, cuda, d, skips):
+    g = np.random.Generator(np.random.Philox(500 + d * 10 + skips))
+    pts = (g.standard_normal((2500, d)) * np.exp(g.uniform(-1, 1, d))).astype(np.float32)
+    t = knn.BallTreeGpu(dev(pts, cuda), depth_skips=skips)
+    o = oracle.BallTree(pts, skips)
+    nodes, leaf_idx, _, dims = t.export()
+    onodes, odims, oleaf = o.export()
+    assert np.array_equal(dims, odims)
+    assert np.array_equal(leaf_idx, oleaf)
+    a, b = nodes.reshape(onodes.shape), onodes
+    ulp = np.spacing(np.abs(b)).astype(np.float64)
+    assert np.all(np.abs(a.astype(np.float64) - b) <= 2 * ulp)
+    ok, msg = pins.check_ball_structure(pts, t.info(), nodes, dims, leaf_idx)
+    assert ok, msg
+
+
+@pytest.mark.parametrize("kind", ["kd", "ball"])
+@pytest.mark.parametrize("d", [3, 4, 8, 6])
+def test_tree_queries_all_variants(oracle, cuda, kind, d):
+    g = np.random.Generator(np.random.Philox(1000 + d))
+    pts = (g.standard_normal((4000, d)) * np.exp(g.uniform(-1, 1, d))).astype(np.float32)
+    pts[7] = pts[8]
+    qs = (g.standard_normal((700, d)) * 1.2).astype(np.float32)
+    qs[0] = pts[8]
+    p = dev(pts, cuda)
+    trees = ([knn.KdTreeGpu(p, depth_skips=s) for s in (0, 3)] if kind == "kd"
+             else [knn.BallTreeGpu(p, depth_skips=s) for s in (1, 4)])
+    for k in (1, 10, 16, 33):
+        for md in (np.inf, 0.3):
+            want = {h: oracle.bf_knn(pts, qs, k, max_d2=md, heap=h) for h in (False, True)}
+            for t in trees:
+                for v in TREE_VARIANTS:
+                    for span in (0, 0xFFFFFFFF, 2):
+                        got = run(t, qs, k, cuda, md=md, BLOCK_SIZE=64, leaf_span=span, **v)
+                        same(got, want[v["USE_HEAP_TRACKING"]], f"{kind} d={d} k={k} md={md} span={span} {v}")
+
+
+@pytest.mark.parametrize("kind", ["kd", "ball"])
+def test_tree_radius_edges_and_ties(oracle, cuda, kind):
+    pts, qs = pins.lattice_case(77, 900, 120, 3, side=7)
+    p = dev(pts, cuda)
+    t = knn.KdTreeGpu(p) if kind == "kd" else knn.BallTreeGpu(p)
+    for md in (np.inf, 0.0, -0.0, -5.0, 1.0, 2.0, 3.0, float("nan")):
+        for heap in (False, True):
+            ei, ed = pins.exact_lattice_knn(pts, qs, 14, md, heap)
+            same(run(t, qs, 14, cuda, md=md, USE_HEAP_TRACKING=heap), (ei, ed), f"{kind} md={md} heap={heap}")
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 17])
+def test_tree_tiny_sizes(oracle, cuda, n):
+    import torch
+    g = np.random.Generator(np.random.Philox(n + 3))
+    pts = g.random((n, 3), dtype=np.float32)
+    qs = g.random((9, 3), dtype=np.float32)
+    p = dev(pts, cuda) if n else torch.empty((0, 3), dtype=torch.float32, device=cuda)
+    for t in (knn.KdTreeGpu(p), knn.KdTreeGpu(p, depth_skips=2), knn.BallTreeGpu(p), knn.BallTreeGpu(p, depth_skips=0)):
+        for k in (1, 4):
+            same(run(t, qs, k, cuda, md=2.0), oracle.bf_knn(pts, qs, k, max_d2=2.0), f"n={n} k={k}")
+
+
+def test_zero_queries_is_a_noop(cuda):
+    import torch
+    pts = np.random.Generator(np.random.Philox(1)).random((50, 3), dtype=np.float32)
+    t = knn.KdTreeGpu(dev(pts, cuda))
+    e = torch.empty((0, 3), dtype=torch.float32, device=cuda)
+    ri = torch.empty((0, 4), dtype=torch.int32, device=cuda)
+    rd = torch.empty((0, 4), dtype=torch.float32, device=cuda)
+    t.batchKnn(knn.Query(e, 0, 1.0, ri, rd), 4)
+
+
+def test_reference_functionality_matrix(oracle, cuda):
+    """tests/FunctionalityTests.cu of the reference: D=4, 100,000 points,
+    50,000 queries, K=10, maxDistance2=5.9, every CPU and GPU variant; here
+    seeded, and compared bit-exact (the reference set-compares)."""
+    g = np.random.Generator(np.random.Philox(4))
+    pts = g.random((100000, 4), dtype=np.float32)
+    qs = g.random((50000, 4), dtype=np.float32)
+    md = 5.9
+    want = oracle.bf_knn(pts, qs, 10, max_d2=md)
+    want_h = oracle.bf_knn(pts, qs, 10, max_d2=md, heap=True)
+    p = dev(pts, cuda)
+    bf = knn.BruteForceGpu(p)
+    same(run(bf, qs, 10, cuda, md=md, BLOCK_SIZE=16), want, "bf array")
+    same(run(bf, qs, 10, cuda, md=md, BLOCK_SIZE=16, USE_HEAP_TRACKING=True), want_h, "bf heap")
+    same(run(bf, qs, 10, cuda, md=md, BLOCK_SIZE=16, USE_TRACKING_SHMEM=True), want, "bf shmem")
+    kd = knn.KdTreeGpu(p, depth_skips=0, split_dimensions=4, block_size=16)
+    same(run(kd, qs, 10, cuda, md=md, BLOCK_SIZE=16, USE_HEAP_TRACKING=True, USE_STACK_SHMEM=True), want_h, "kd 4/4")
+    kd2 = knn.KdTreeGpu(p, depth_skips=3, split_dimensions=3, block_size=16)
+    same(run(kd2, qs, 10, cuda, md=md, BLOCK_SIZE=16, USE_TRACKING_SHMEM=True), want, "kd 4/3")
+    ball = knn.BallTreeGpu(p, depth_skips=0, block_size=16)
+    same(run(ball, qs, 10, cuda, md=md, BLOCK_SIZE=16, USE_HEAP_TRACKING=True, USE_TRACKING_SHMEM=True,
+             USE_STACK_SHMEM=True), want_h, "ball sh/sh")
+    ball2 = knn.BallTreeGpu(p, depth_skips=2, block_size=16)
+    same(run(ball2, qs, 10, cuda, md=md, BLOCK_SIZE=16), want, "ball reg/reg")
+    # CPU oracle trees (the reference's KdTree/BallTree rows)
+    same(oracle.KdTree(pts, 3).knn(qs[:5000], 10, max_d2=md), tuple(a[:5000] for a in want), "cpu kd")
+    same(oracle.BallTree(pts, 3).knn(qs[:5000], 10, max_d2=md), tuple(a[:5000] for a in want), "cpu ball")
+
+
+def test_uniform_small_config_bit_exact(oracle, cuda):
+    """65,536 uniform points, 16,384 queries, D=3, K=8, heap tracking,
+    maxRadius=+inf, seed 0: BruteForce and KdTree (depthSkips=0)."""
+    pts, qs = datasets.uniform_cube(65536, 16384, 3, 0)
+    want = oracle.bf_knn(pts, qs, 8, heap=True)
+    p = dev(pts, cuda)
+    same(run(knn.BruteForceGpu(p), qs, 8, cuda, USE_HEAP_TRACKING=True), want, "bf")
+    kd = knn.KdTreeGpu(p, depth_skips=0)
+    for v in TREE_VARIANTS:
+        if v["USE_HEAP_TRACKING"]:
+            same(run(kd, qs, 8, cuda, **v), want, f"kd {v}")
