@@ -1,0 +1,251 @@
+"""Benchmark: exact kNN queries/sec on one MI355X.
+
+Workload (the real-size tree configuration): 16,777,216 float32 points and
+4,194,304 queries in R^3 from a mixture of 32 isotropic Gaussians (centres
+uniform in [0,100)^3, sigma log-uniform in [0.5, 5], seed 1),
+KdTreeGpu<3,3,256> (depthSkips 0) and BallTreeGpu<3,256> (depthSkips 1),
+batchKnn<K=16>, maxRadius = +inf.
+
+A step is one batchKnn over all 4,194,304 queries on the kd tree (inputs
+resident in HBM).  `value` is queries/sec over the timed steps.  The ball
+tree, both build times and the plain brute force at the small uniform
+configuration are reported alongside.
+
+Roofline: the dominant kernel is the kd traversal.  Its algorithmic bytes
+per launch are Q * (4D query + 8K results + 4 order index) plus the node
+bytes and leaf slots (4 + 4D bytes each) the traversal itself reads, counted
+by the library's visit counters in an untimed run; divided by the kernel's
+hipEvent-measured average duration.  Peak: 8 TB/s HBM.
+
+CPU baseline: the CPU oracle's kd tree (a port of the reference's KdTree,
+src/KdTree.cuh) on this host, on a bounded query sample, with its results
+checked against the GPU's rows for the same queries.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+K = 16
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    (scaled by a copy-kernel calibration), if the summary was committed."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+        return data.get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(pts, qs, gpu_idx, gpu_dist, sample: int):
+    """Time the CPU oracle kd tree on a bounded sample (test infrastructure)."""
+    from oracle import oracle as O
+    O.build()
+    t0 = time.perf_counter()
+    tree = O.KdTree(pts, 0)
+    build_s = time.perf_counter() - t0
+    q1 = qs[: sample // 16]
+    t0 = time.perf_counter()
+    i1, d1 = tree.knn(q1, K, threads=1)
+    one_core = len(q1) / (time.perf_counter() - t0)
+    qa = qs[:sample]
+    ncores = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    ia, da = tree.knn(qa, K, threads=ncores)
+    all_core = len(qa) / (time.perf_counter() - t0)
+    exact = bool(np.array_equal(ia, gpu_idx[:sample]) and
+                 np.array_equal(da.view(np.uint32), gpu_dist[:sample].view(np.uint32)))
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": round(all_core, 1), "unit": "queries/s", "cores": ncores, "kind": "port",
+        "sample": f"oracle KdTree (depthSkips 0) built on all 16,777,216 points in {build_s:.1f} s; "
+                  f"{len(qa)} queries timed on {ncores} threads, {len(q1)} on 1 thread",
+        "value_1core": round(one_core, 1), "build_s": round(build_s, 2), "cpu_model": model,
+        "sample_bit_exact_vs_gpu": exact,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--points", type=int, default=1 << 24)
+    ap.add_argument("--queries", type=int, default=1 << 22)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--leaf-span", type=int, default=0)
+    ap.add_argument("--stack-lds", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
+              "described in DESIGN.md and not measured here", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import cudaexactknn_amd as knn
+    from cudaexactknn_amd import datasets
+
+    dev = torch.device("cuda:0")
+    log("generating the 3-D Gaussian-mixture workload")
+    pts, qs = datasets.gaussian_mixture_3d(args.points, args.queries, 32, seed=1)
+    p = torch.from_numpy(pts).to(dev)
+    q = torch.from_numpy(qs).to(dev)
+    nq = qs.shape[0]
+    ri = torch.empty((nq, K), dtype=torch.int32, device=dev)
+    rd = torch.empty((nq, K), dtype=torch.float32, device=dev)
+    query = knn.Query(q, nq, float("inf"), ri, rd)
+    torch.cuda.synchronize()
+
+    log("building kd tree")
+    kd = knn.KdTreeGpu(p, depth_skips=0, split_dimensions=3, block_size=256)
+    kd_build = [kd.build_ms()]
+    for _ in range(2):
+        kd.close()
+        kd = knn.KdTreeGpu(p, depth_skips=0, split_dimensions=3, block_size=256)
+        kd_build.append(kd.build_ms())
+    log(f"kd build ms {kd_build}")
+
+    opts = dict(BLOCK_SIZE=args.block, leaf_span=args.leaf_span, USE_STACK_SHMEM=args.stack_lds)
+    for _ in range(args.warmup):
+        kd.batchKnn(query, K, **opts)
+    torch.cuda.synchronize()
+    kernel_ms, order_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kd.batchKnn(query, K, **opts)
+        st = kd.last_query_stats()
+        kernel_ms.append(st["kernel_ms"])
+        order_ms.append(st["order_ms"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1000.0 / args.steps
+    qps = nq / (elapsed / args.steps)
+    log(f"kd: {ms_per_step:.3f} ms/step, {qps / 1e6:.2f} M q/s, traversal kernel {np.mean(kernel_ms):.3f} ms, "
+        f"ordering {np.mean(order_ms):.3f} ms")
+    kd_idx = ri.cpu().numpy()
+    kd_dist = rd.cpu().numpy()
+
+    # Untimed counting run: algorithmic bytes of the traversal kernel.
+    kd.batchKnn(query, K, count_visits=True, **opts)
+    cs = kd.last_query_stats()
+    alg_bytes = cs["node_bytes"] + cs["leaf_slots"] * (4 + 4 * 3) + nq * (4 * 3 + 8 * K + 4)
+    kavg = float(np.mean(kernel_ms))
+    achieved = alg_bytes / (kavg * 1e-3) / 1e9
+    log(f"kd visits/query: node bytes {cs['node_bytes'] / nq:.1f}, leaf slots {cs['leaf_slots'] / nq:.1f}; "
+        f"algorithmic {alg_bytes / 1e9:.3f} GB per launch -> {achieved:.1f} GB/s")
+
+    log("building ball tree")
+    ball = knn.BallTreeGpu(p, depth_skips=1, block_size=256)
+    ball_build = ball.build_ms()
+    ri2 = torch.empty_like(ri)
+    rd2 = torch.empty_like(rd)
+    bq = knn.Query(q, nq, float("inf"), ri2, rd2)
+    ball.batchKnn(bq, K, BLOCK_SIZE=args.block, leaf_span=args.leaf_span)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        ball.batchKnn(bq, K, BLOCK_SIZE=args.block, leaf_span=args.leaf_span)
+    torch.cuda.synchronize()
+    ball_qps = nq / ((time.perf_counter() - t0) / max(1, args.steps // 2))
+    ball_same = bool(torch.equal(ri, ri2) and torch.equal(rd.view(torch.int32), rd2.view(torch.int32)))
+    log(f"ball: {ball_qps / 1e6:.2f} M q/s, build {ball_build:.1f} ms, identical to kd: {ball_same}")
+    ball.close()
+
+    # Plain brute force on the small uniform configuration (65,536 x 16,384, D=3, K=8, heap).
+    up, uq = datasets.uniform_cube(65536, 16384, 3, 0)
+    bf = knn.BruteForceGpu(torch.from_numpy(up).to(dev))
+    bri = torch.empty((16384, 8), dtype=torch.int32, device=dev)
+    brd = torch.empty((16384, 8), dtype=torch.float32, device=dev)
+    bfq = knn.Query(torch.from_numpy(uq).to(dev), 16384, float("inf"), bri, brd)
+    bf.batchKnn(bfq, 8, USE_HEAP_TRACKING=True)
+    bms = []
+    for _ in range(5):
+        bf.batchKnn(bfq, 8, USE_HEAP_TRACKING=True)
+        bms.append(bf.last_query_stats()["kernel_ms"])
+    bf_kernel_ms = float(np.mean(bms))
+    bf_ops = 3 * 3 * 65536 * 16384
+    log(f"bf 65,536 x 16,384: kernel {bf_kernel_ms:.3f} ms, {bf_ops / bf_kernel_ms / 1e9:.1f} TFLOP/s (3D ops/pair)")
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle kd tree on this host)")
+        cpu = cpu_baseline(pts, qs, kd_idx, kd_dist, args.cpu_sample)
+        log(f"CPU: {cpu['value']} q/s on {cpu['cores']} threads, {cpu['value_1core']} q/s on 1; "
+            f"sample bit-exact: {cpu['sample_bit_exact_vs_gpu']}")
+
+    out = {
+        "metric": "exact kNN queries/sec (kd tree, K=16, idx bit-exact)",
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "KdTreeGpu<3,3,256>/BallTreeGpu<3,256> batchKnn<K=16>, 16,777,216 points, "
+                        "4,194,304 queries, 32-Gaussian mixture in R^3, seed 1",
+            "points": args.points, "queries": nq, "dims": 3, "k": K, "max_radius": "inf",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles("kd_query_kernel"),
+            "kernel": "kd_query_kernel<3, REG16, register bit-stack>",
+            "kernel_ms": round(kavg, 4),
+            "alg_bytes_per_launch": int(alg_bytes),
+        },
+        "cpu_baseline": cpu,
+        "extra": {
+            "kd_build_ms": [round(x, 2) for x in kd_build],
+            "kd_order_ms": round(float(np.mean(order_ms)), 4),
+            "kd_node_bytes_per_query": round(cs["node_bytes"] / nq, 2),
+            "kd_leaf_slots_per_query": round(cs["leaf_slots"] / nq, 2),
+            "ball_qps": round(ball_qps, 1),
+            "ball_build_ms": round(ball_build, 2),
+            "ball_identical_to_kd": ball_same,
+            "bf_uniform_65536x16384_k8_kernel_ms": round(bf_kernel_ms, 4),
+            "bf_uniform_tflops_3D_ops": round(bf_ops / bf_kernel_ms / 1e9, 2),
+        },
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -109,10 +109,11 @@ class _Structure:
                                          ctypes.byref(opts)))
 
     def last_query_stats(self) -> dict:
-        buf = (ctypes.c_float * 4)()
+        buf = (ctypes.c_float * 8)()
         _lib.check(_lib.load().knn_last_query_stats(self._h, buf))
         return {"rerun_queries": buf[0], "bf16_survivor_fraction": buf[1], "radius_ratio": buf[2],
-                "query_ms": buf[3]}
+                "query_ms": buf[3], "kernel_ms": buf[4], "node_bytes": buf[5], "leaf_slots": buf[6],
+                "order_ms": buf[7]}
 
     def close(self) -> None:
         if self._h and self._h.value:
@@ -132,14 +133,15 @@ class _Structure:
         self.close()
 
 
-def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, leaf_span=0, stream=None):
+def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, leaf_span=0, stream=None,
+          count_visits=False):
     o = QueryOpts()
     o.block = int(BLOCK_SIZE or 0)
     o.heap = int(bool(heap))
     o.track_lds = int(bool(track))
     o.stack_lds = int(bool(stack))
     o.form = int(form)
-    o.flags = 0 if sort_queries else _lib.KNN_FLAG_NO_QUERY_SORT
+    o.flags = (0 if sort_queries else _lib.KNN_FLAG_NO_QUERY_SORT) | (_lib.KNN_FLAG_COUNT_VISITS if count_visits else 0)
     o.leaf_span = int(leaf_span)
     o.stream = stream
     return o
@@ -167,11 +169,11 @@ class BruteForceGpu(_Structure):
 class _Tree(_Structure):
     def batchKnn(self, query: Query, K: int, BLOCK_SIZE: int = 0, USE_HEAP_TRACKING: bool = False,
                  USE_TRACKING_SHMEM: bool = False, USE_STACK_SHMEM: bool = False, form: int = FORM_PLAIN,
-                 sort_queries: bool = True, leaf_span: int = 0, stream=None) -> None:
+                 sort_queries: bool = True, leaf_span: int = 0, stream=None, count_visits: bool = False) -> None:
         """batchKnn<K, BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM>(Query&)
         (src/KdTreeGpu.cuh:393-410, src/BallTreeGpu.cuh:631-648)."""
         self._query(query, K, _opts(BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM, form,
-                                    sort_queries, leaf_span, stream))
+                                    sort_queries, leaf_span, stream, count_visits))
 
     def build_ms(self) -> float:
         return float(_lib.load().knn_last_build_ms(self._h))

@@ -36,6 +36,7 @@ __global__ void __launch_bounds__(1024) ball_query_kernel(TreeArgs A, float shri
     bool tightened;
     uint64_t worst = buf.init(start_key(A, qi, tightened), A.K);
     uint32_t whi = key_hi(worst);
+    uint32_t nbytes = 0, nslots = 0;
     const uint32_t Dn = q.dims();
     const uint32_t nstride = 2 * Dn + 2;
     const uint32_t stride = 1 + Dn;
@@ -43,7 +44,7 @@ __global__ void __launch_bounds__(1024) ball_query_kernel(TreeArgs A, float shri
     const bool single_bucket = stop == A.node_depth;
 
     if (A.node_size == 0 || A.leaves == nullptr) {
-        finish_query(A, qi, buf, worst, tightened);
+        finish_query(A, qi, buf, worst, tightened, nbytes, nslots);
         return;
     }
 
@@ -60,6 +61,7 @@ __global__ void __launch_bounds__(1024) ball_query_kernel(TreeArgs A, float shri
             bool skip = false;
             while (level < stop) {
                 const float* nd = A.nodes + (size_t)node * nstride;
+                nbytes += 8u * (Dn + 1);
                 float lbd, rbd;
                 const float lb = ball_bound(q, nd, nd[Dn], shrink_a, shrink_b, lbd);
                 const float rb = ball_bound(q, nd + Dn + 1, nd[2 * Dn + 1], shrink_a, shrink_b, rbd);
@@ -79,7 +81,7 @@ __global__ void __launch_bounds__(1024) ball_query_kernel(TreeArgs A, float shri
             if (skip) continue;
             uint32_t slot0, count;
             leaf_range(A, node, level, slot0, count);
-            scan_slots<D>(A.leaves, stride, slot0, count, single_bucket, q, buf, worst, whi);
+            scan_slots<D>(A.leaves, stride, slot0, count, single_bucket, q, buf, worst, whi, nslots);
         }
     } else {
         uint32_t node = 0, level = 0, pend = 0, side = 0;
@@ -87,6 +89,7 @@ __global__ void __launch_bounds__(1024) ball_query_kernel(TreeArgs A, float shri
             bool skip = false;
             while (level < stop) {
                 const float* nd = A.nodes + (size_t)node * nstride;
+                nbytes += 8u * (Dn + 1);
                 float lbd, rbd;
                 const float lb = ball_bound(q, nd, nd[Dn], shrink_a, shrink_b, lbd);
                 const float rb = ball_bound(q, nd + Dn + 1, nd[2 * Dn + 1], shrink_a, shrink_b, rbd);
@@ -105,7 +108,7 @@ __global__ void __launch_bounds__(1024) ball_query_kernel(TreeArgs A, float shri
             if (!skip) {
                 uint32_t slot0, count;
                 leaf_range(A, node, level, slot0, count);
-                scan_slots<D>(A.leaves, stride, slot0, count, single_bucket, q, buf, worst, whi);
+                scan_slots<D>(A.leaves, stride, slot0, count, single_bucket, q, buf, worst, whi, nslots);
             }
             bool more = false;
             while (pend) {
@@ -114,6 +117,7 @@ __global__ void __launch_bounds__(1024) ball_query_kernel(TreeArgs A, float shri
                 const uint32_t anc = ((node + 1) >> (level - l)) - 1;
                 const uint32_t right = (side >> l) & 1u;  // far child is the right one
                 const float* nd = A.nodes + (size_t)anc * nstride + (right ? Dn + 1 : 0);
+                nbytes += 4u * (Dn + 1);
                 float bd;
                 const float fb = ball_bound(q, nd, nd[Dn], shrink_a, shrink_b, bd);
                 if (f2u(fb) > whi) continue;
@@ -125,7 +129,7 @@ __global__ void __launch_bounds__(1024) ball_query_kernel(TreeArgs A, float shri
             if (!more) break;
         }
     }
-    finish_query(A, qi, buf, worst, tightened);
+    finish_query(A, qi, buf, worst, tightened, nbytes, nslots);
 }
 
 using BallFn = void (*)(TreeArgs, float, float);

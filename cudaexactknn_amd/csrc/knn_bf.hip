@@ -177,6 +177,7 @@ void query_bf(Handle& h, const QueryArgs& a) {
                 std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",
             KNN_ERR_INVALID_ARG);
     const uint32_t grid = (a.nq + BS - 1) / BS;
+    EventTimer timer(a.timed, a.stream);
     if (fn) {
         hipLaunchKernelGGL(fn, dim3(grid), dim3(BS), lds, a.stream, h.pts.get(), h.info.n, a.q, a.nq, a.maxd2, a.K,
                            a.oidx, a.odist, tile_pts);
@@ -186,6 +187,7 @@ void query_bf(Handle& h, const QueryArgs& a) {
                            a.K, a.oidx, a.odist, tile_pts);
     }
     KNN_CHECK_LAUNCH();
+    h.stats[4] = timer.stop();
 }
 
 }  // namespace knn

@@ -183,6 +183,9 @@ knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max
         a.track_lds = o.track_lds != 0;
         a.stack_lds = o.stack_lds != 0;
         a.sort_queries = (o.flags & KNN_FLAG_NO_QUERY_SORT) == 0;
+        a.count_visits = (o.flags & KNN_FLAG_COUNT_VISITS) != 0;
+        a.timed = (opts == nullptr || opts->stream == nullptr);
+        for (float& v : h.stats) v = 0.0f;
         a.leaf_span = o.leaf_span == 0 ? default_span(h) : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
         a.form = (knn_form)o.form;
         a.stream = (hipStream_t)o.stream;
@@ -275,11 +278,11 @@ knn_status knn_learned_train(void* handle, const float* d_sample, uint32_t n_sam
     });
 }
 
-knn_status knn_last_query_stats(const void* handle, float* out4) {
+knn_status knn_last_query_stats(const void* handle, float* out8) {
     return guarded([&] {
-        require(out4 != nullptr, "out4 is NULL");
+        require(out8 != nullptr, "out8 is NULL");
         const Handle& h = *as_handle(handle);
-        std::memcpy(out4, h.stats, sizeof(h.stats));
+        std::memcpy(out8, h.stats, sizeof(h.stats));
     });
 }
 

@@ -79,7 +79,8 @@ struct Handle {
     DevBuf<uint32_t> q_keys, q_vals, q_keys_alt, q_vals_alt, q_temp;
     DevBuf<uint8_t> q_scratch;
     float last_build_ms = -1.0f;
-    float stats[4] = {0, 0, 0, 0};
+    float stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    DevBuf<unsigned long long> counters;  // visit counters (KNN_FLAG_COUNT_VISITS)
     Mlp* mlp = nullptr;
     ~Handle();
 };
@@ -139,10 +140,39 @@ struct QueryArgs {
     float* odist;
     uint32_t K;
     uint32_t block;
-    bool heap, track_lds, stack_lds, sort_queries;
+    bool heap, track_lds, stack_lds, sort_queries, count_visits;
     uint32_t leaf_span;  // extra levels folded into one leaf scan
     knn_form form;
     hipStream_t stream;
+    bool timed;  // synchronous call: time kernels with events into Handle::stats
+};
+
+// Event pair timing a region on a stream when `on`.
+struct EventTimer {
+    hipEvent_t a{}, b{};
+    bool on;
+    hipStream_t st;
+    EventTimer(bool enable, hipStream_t s) : on(enable), st(s) {
+        if (on) {
+            KNN_HIP(hipEventCreate(&a));
+            KNN_HIP(hipEventCreate(&b));
+            KNN_HIP(hipEventRecord(a, st));
+        }
+    }
+    float stop() {
+        if (!on) return 0.0f;
+        KNN_HIP(hipEventRecord(b, st));
+        KNN_HIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        KNN_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    }
+    ~EventTimer() {
+        if (on) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    }
 };
 
 // Builders (knn_build.hip)

@@ -38,12 +38,13 @@ __global__ void __launch_bounds__(1024) kd_query_kernel(TreeArgs A) {
     bool tightened;
     uint64_t worst = buf.init(start_key(A, qi, tightened), A.K);
     uint32_t whi = key_hi(worst);
+    uint32_t nbytes = 0, nslots = 0;
     const uint32_t SD = A.split_dims, stop = A.stop_level;
     const uint32_t stride = 1 + q.dims();
     const bool single_bucket = stop == A.node_depth;
 
     if (A.node_size == 0 || A.leaves == nullptr) {
-        finish_query(A, qi, buf, worst, tightened);
+        finish_query(A, qi, buf, worst, tightened, nbytes, nslots);
         return;
     }
 
@@ -60,6 +61,7 @@ __global__ void __launch_bounds__(1024) kd_query_kernel(TreeArgs A) {
             uint32_t dim = level_dim(level, SD, A.sd_inv);
             while (level < stop) {
                 const float diff = q.at(dim) - A.nodes[node];
+                nbytes += 4u;
                 const uint32_t l = 2 * node + 1;
                 const float b = diff * diff;
                 const uint32_t near_c = diff < 0.0f ? l : l + 1;
@@ -74,13 +76,14 @@ __global__ void __launch_bounds__(1024) kd_query_kernel(TreeArgs A) {
             }
             uint32_t slot0, count;
             leaf_range(A, node, level, slot0, count);
-            scan_slots<D>(A.leaves, stride, slot0, count, single_bucket, q, buf, worst, whi);
+            scan_slots<D>(A.leaves, stride, slot0, count, single_bucket, q, buf, worst, whi, nslots);
         }
     } else {
         uint32_t node = 0, level = 0, dim = 0, pend = 0;
         for (;;) {
             while (level < stop) {
                 const float diff = q.at(dim) - A.nodes[node];
+                nbytes += 4u;
                 const uint32_t l = 2 * node + 1;
                 const float b = diff * diff;
                 if (f2u(b) <= whi) pend |= 1u << level;
@@ -90,7 +93,7 @@ __global__ void __launch_bounds__(1024) kd_query_kernel(TreeArgs A) {
             }
             uint32_t slot0, count;
             leaf_range(A, node, level, slot0, count);
-            scan_slots<D>(A.leaves, stride, slot0, count, single_bucket, q, buf, worst, whi);
+            scan_slots<D>(A.leaves, stride, slot0, count, single_bucket, q, buf, worst, whi, nslots);
             bool more = false;
             while (pend) {
                 const uint32_t l = 31u - __clz(pend);
@@ -98,6 +101,7 @@ __global__ void __launch_bounds__(1024) kd_query_kernel(TreeArgs A) {
                 const uint32_t anc = ((node + 1) >> (level - l)) - 1;
                 const uint32_t adim = level_dim(l, SD, A.sd_inv);
                 const float diff = q.at(adim) - A.nodes[anc];
+                nbytes += 4u;
                 const float b = diff * diff;
                 if (f2u(b) > whi) continue;
                 const uint32_t lc = 2 * anc + 1;
@@ -110,7 +114,7 @@ __global__ void __launch_bounds__(1024) kd_query_kernel(TreeArgs A) {
             if (!more) break;
         }
     }
-    finish_query(A, qi, buf, worst, tightened);
+    finish_query(A, qi, buf, worst, tightened, nbytes, nslots);
 }
 
 using TreeFn = void (*)(TreeArgs);

@@ -35,6 +35,7 @@ struct TreeArgs {
     float* odist;
     const float* init_r2;     // optional per-query starting radius^2 (learned form)
     uint32_t* short_flags;    // optional: set to 1 for a query whose tightened radius left a slot empty
+    unsigned long long* counters;  // optional: [0] node bytes read, [1] leaf slots read
 };
 
 // Query coordinates: compile-time D in VGPRs, or runtime D lane-interleaved in LDS.
@@ -72,8 +73,9 @@ struct QPoint<0> {
 template <int D, class Q, class B>
 __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uint32_t stride, uint32_t slot0,
                                            uint32_t count, bool stop_at_empty, const Q& q, B& buf,
-                                           uint64_t& worst, uint32_t& whi) {
+                                           uint64_t& worst, uint32_t& whi, uint32_t& nslots) {
     for (uint32_t o = 0; o < count; ++o) {
+        ++nslots;
         const float* e = leaves + (size_t)(slot0 + o) * stride;
         float d2;
         int32_t pi;
@@ -116,8 +118,13 @@ __device__ __forceinline__ uint32_t level_dim(uint32_t level, uint32_t sd, uint3
 
 // Learned-form bookkeeping and the canonical write.
 template <class B>
-__device__ __forceinline__ void finish_query(const TreeArgs& A, uint32_t qi, B& buf, uint64_t worst, bool tightened) {
+__device__ __forceinline__ void finish_query(const TreeArgs& A, uint32_t qi, B& buf, uint64_t worst, bool tightened,
+                                             uint32_t nbytes, uint32_t nslots) {
     buf.write(A.oidx + (size_t)qi * A.K, A.odist + (size_t)qi * A.K, A.K);
+    if (A.counters) {
+        atomicAdd(&A.counters[0], (unsigned long long)nbytes);
+        atomicAdd(&A.counters[1], (unsigned long long)nslots);
+    }
     if (tightened && A.short_flags) {
         // An empty slot remains iff the K-th key is still an empty key (index ~0).
         if ((uint32_t)worst == 0xFFFFFFFFu) A.short_flags[qi] = 1u;

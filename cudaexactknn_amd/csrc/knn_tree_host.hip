@@ -40,6 +40,7 @@ __global__ void morton_kernel(const float* __restrict__ qs, uint32_t nq, uint32_
 
 void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
     const uint32_t n = a.nq, D = h.info.dims;
+    EventTimer timer(a.timed, a.stream);
     h.q_keys.ensure(n);
     h.q_vals.ensure(n);
     h.q_keys_alt.ensure(n);
@@ -56,6 +57,7 @@ void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
                                 h.q_temp.get(), a.stream);
     KNN_CHECK_LAUNCH();
     *d_perm = alt ? h.q_vals_alt.get() : h.q_vals.get();
+    h.stats[7] = timer.stop();
 }
 
 void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
@@ -82,6 +84,11 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     A.odist = a.odist;
     A.init_r2 = d_init_r2;
     A.short_flags = d_short_flags;
+    if (a.count_visits) {
+        h.counters.ensure(2);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(h.counters.get()), 4, 0u, a.stream);
+        A.counters = h.counters.get();
+    }
 
     const int kind = pick_buf_kind(a.heap, a.track_lds, a.K);
     const bool lds_buf = kind == BUF_LDS_ARRAY || kind == BUF_LDS_HEAP;
@@ -102,6 +109,7 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
                 std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",
             KNN_ERR_INVALID_ARG);
     const uint32_t grid = (a.nq + BS - 1) / BS;
+    EventTimer timer(a.timed, a.stream);
     if (kfn) {
         hipLaunchKernelGGL(kfn, dim3(grid), dim3(BS), lds, a.stream, A);
     } else {
@@ -111,6 +119,14 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
         hipLaunchKernelGGL(bfn, dim3(grid), dim3(BS), lds, a.stream, A, shrink_a, shrink_b);
     }
     KNN_CHECK_LAUNCH();
+    h.stats[4] = timer.stop();
+    if (a.count_visits) {
+        unsigned long long c[2] = {0, 0};
+        KNN_HIP(hipMemcpyAsync(c, h.counters.get(), sizeof(c), hipMemcpyDeviceToHost, a.stream));
+        KNN_HIP(hipStreamSynchronize(a.stream));
+        h.stats[5] = (float)c[0];
+        h.stats[6] = (float)c[1];
+    }
 }
 
 }  // namespace knn

@@ -59,6 +59,7 @@ typedef enum knn_form { KNN_FORM_PLAIN = 0, KNN_FORM_BF16 = 1, KNN_FORM_LEARNED 
 
 /* Flags for knn_query_opts.flags. */
 #define KNN_FLAG_NO_QUERY_SORT 0x1u  /* trees: keep caller order (no Morton reordering) */
+#define KNN_FLAG_COUNT_VISITS 0x2u   /* trees: count node bytes and leaf slots read (stats[5], [6]) */
 
 typedef struct knn_query_opts {
     uint32_t block;      /* BLOCK_SIZE: threads per workgroup, 1..1024 (0 = tuned default) */
@@ -143,11 +144,14 @@ float knn_last_build_ms(const void* handle);
 knn_status knn_learned_train(void* handle, const float* d_sample, uint32_t n_sample, uint32_t k,
                              uint32_t epochs, float lr, float* stats);
 
-/* Statistics of the last knn_query on this handle (may be used after the
- * call returns):  [0] queries re-run by the plain traversal (learned form),
+/* Statistics of the last synchronous knn_query on this handle:
+ * [0] queries re-run by the plain traversal (learned form),
  * [1] BF16 survivor fraction (bf16 form), [2] mean predicted / exact radius
- * ratio (learned form), [3] kernel ms of the query (hipEvent-timed). */
-knn_status knn_last_query_stats(const void* handle, float* out4);
+ * ratio (learned form), [3] ms of the whole query, [4] ms of the main
+ * kernel (traversal or scan), [5] node bytes read and [6] leaf slots read
+ * (with KNN_FLAG_COUNT_VISITS), [7] ms spent ordering queries.
+ * All timings are hipEvent-measured on the query's stream. */
+knn_status knn_last_query_stats(const void* handle, float* out8);
 
 #ifdef __cplusplus
 }
