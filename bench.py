@@ -226,7 +226,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles("kd_query_kernel"),
-            "kernel": "kd_query_kernel<3, REG16, register bit-stack>",
+            "kernel": "kd_packet_kernel<3, REG16> (wave-cooperative traversal)",
             "kernel_ms": round(kavg, 4),
             "alg_bytes_per_launch": int(alg_bytes),
         },

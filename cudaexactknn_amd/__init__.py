@@ -134,14 +134,15 @@ class _Structure:
 
 
 def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, leaf_span=0, stream=None,
-          count_visits=False):
+          count_visits=False, per_lane=False, morton=False):
     o = QueryOpts()
     o.block = int(BLOCK_SIZE or 0)
     o.heap = int(bool(heap))
     o.track_lds = int(bool(track))
     o.stack_lds = int(bool(stack))
     o.form = int(form)
-    o.flags = (0 if sort_queries else _lib.KNN_FLAG_NO_QUERY_SORT) | (_lib.KNN_FLAG_COUNT_VISITS if count_visits else 0)
+    o.flags = ((0 if sort_queries else _lib.KNN_FLAG_NO_QUERY_SORT) | (_lib.KNN_FLAG_COUNT_VISITS if count_visits else 0)
+               | (_lib.KNN_FLAG_PER_LANE if per_lane else 0) | (_lib.KNN_FLAG_MORTON_ORDER if morton else 0))
     o.leaf_span = int(leaf_span)
     o.stream = stream
     return o
@@ -169,11 +170,12 @@ class BruteForceGpu(_Structure):
 class _Tree(_Structure):
     def batchKnn(self, query: Query, K: int, BLOCK_SIZE: int = 0, USE_HEAP_TRACKING: bool = False,
                  USE_TRACKING_SHMEM: bool = False, USE_STACK_SHMEM: bool = False, form: int = FORM_PLAIN,
-                 sort_queries: bool = True, leaf_span: int = 0, stream=None, count_visits: bool = False) -> None:
+                 sort_queries: bool = True, leaf_span: int = 0, stream=None, count_visits: bool = False,
+                 per_lane: bool = False, morton: bool = False) -> None:
         """batchKnn<K, BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM>(Query&)
         (src/KdTreeGpu.cuh:393-410, src/BallTreeGpu.cuh:631-648)."""
         self._query(query, K, _opts(BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM, form,
-                                    sort_queries, leaf_span, stream, count_visits))
+                                    sort_queries, leaf_span, stream, count_visits, per_lane, morton))
 
     def build_ms(self) -> float:
         return float(_lib.load().knn_last_build_ms(self._h))

@@ -22,6 +22,8 @@ KNN_KIND_BRUTE, KNN_KIND_KD, KNN_KIND_BALL = 0, 1, 2
 KNN_FORM_PLAIN, KNN_FORM_BF16, KNN_FORM_LEARNED = 0, 1, 2
 KNN_FLAG_NO_QUERY_SORT = 0x1
 KNN_FLAG_COUNT_VISITS = 0x2
+KNN_FLAG_PER_LANE = 0x4
+KNN_FLAG_MORTON_ORDER = 0x8
 
 # Every symbol include/knn_mi355x.h declares.
 EXPORTED = (

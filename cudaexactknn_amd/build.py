@@ -30,7 +30,7 @@ COMMON = [
 
 SOURCES = [
     "knn_prims.hip", "knn_build.hip", "knn_bf.hip", "knn_bf16.hip", "knn_kd_query.hip",
-    "knn_ball_query.hip", "knn_tree_host.hip", "knn_learned.hip", "knn_capi.hip",
+    "knn_ball_query.hip", "knn_kd_packet.hip", "knn_ball_packet.hip", "knn_tree_host.hip", "knn_learned.hip", "knn_capi.hip",
 ]
 
 
@@ -41,6 +41,21 @@ def _compile(src: str) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
     return obj
+
+
+ROOT = os.path.dirname(HERE)
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "functionality_tests.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "functionality_tests")
+
+
+def build_cpp_test() -> str:
+    """Compiles the C++ drop-in test against include/KnnTrees.hpp and the library."""
+    cmd = [HIPCC, *COMMON, "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC, "-L", HERE, "-lknn_mi355x",
+           "-Wl,-rpath,$ORIGIN/../../cudaexactknn_amd", "-o", CPP_TEST_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed:\n{r.stdout}\n{r.stderr}")
+    return CPP_TEST_BIN
 
 
 def build(jobs: int | None = None, verbose: bool = True) -> str:
@@ -54,6 +69,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     shutil.move(tmp, LIB)
+    build_cpp_test()
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
     return LIB

@@ -147,7 +147,9 @@ static BallFn ball_pick1(int kind, bool stack_lds) {
         case BUF_REG32: return ball_pick2<D, BUF_REG32>(stack_lds);
         case BUF_LDS_ARRAY: return ball_pick2<D, BUF_LDS_ARRAY>(stack_lds);
         case BUF_LDS_HEAP: return ball_pick2<D, BUF_LDS_HEAP>(stack_lds);
-        default: return ball_pick2<D, BUF_PRIV_HEAP>(stack_lds);
+        case BUF_REGH8: return ball_pick2<D, BUF_REGH8>(stack_lds);
+        case BUF_REGH16: return ball_pick2<D, BUF_REGH16>(stack_lds);
+        default: return ball_pick2<D, BUF_REGH32>(stack_lds);
     }
 }
 

@@ -14,11 +14,22 @@
 
 namespace knn {
 
+// Split scans: blockIdx.y selects the point chunk [y*chunk, (y+1)*chunk); with
+// `partial` set, each (query, chunk) writes its K smallest keys ascending to
+// partial[(q * gridDim.y + y) * K ...] for bf_merge_kernel.
+// Large D keeps the whole query in VGPRs: allow up to 256 lanes per
+// workgroup so the compiler may use more than 128 VGPRs instead of spilling.
+constexpr int bf_max_threads(int D) { return D >= 32 ? 256 : 1024; }
+
 template <int D, int KIND>
-__global__ void __launch_bounds__(1024) bf_kernel(const float* __restrict__ pts, uint32_t n,
+__global__ void __launch_bounds__(bf_max_threads(D)) bf_kernel(const float* __restrict__ pts_all, uint32_t n_all,
                                                   const float* __restrict__ qs, uint32_t nq, float maxd2,
                                                   uint32_t K, int32_t* __restrict__ oidx,
-                                                  float* __restrict__ odist, uint32_t tile_pts) {
+                                                  float* __restrict__ odist, uint32_t tile_pts, uint32_t chunk,
+                                                  uint64_t* __restrict__ partial) {
+    const uint32_t c0 = blockIdx.y * chunk;
+    const uint32_t n = c0 < n_all ? min(chunk, n_all - c0) : 0u;
+    const float* pts = pts_all + (size_t)c0 * D;
     constexpr int DP = (D + 3) & ~3;
     extern __shared__ __align__(16) uint8_t lds_raw[];
     float* tile = reinterpret_cast<float*>(lds_raw);
@@ -49,7 +60,7 @@ __global__ void __launch_bounds__(1024) bf_kernel(const float* __restrict__ pts,
             for (uint32_t j = 0; j < cnt; ++j) {
                 const float d2 = dist2<D>(q, tile + j * DP);
                 if (f2u(d2) <= whi) {
-                    const uint64_t key = make_key(d2, base + j);
+                    const uint64_t key = make_key(d2, c0 + base + j);
                     if (key < worst) {
                         worst = buf.consider(key);
                         whi = key_hi(worst);
@@ -58,15 +69,24 @@ __global__ void __launch_bounds__(1024) bf_kernel(const float* __restrict__ pts,
             }
         }
     }
-    if (active) buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
+    if (active) {
+        if (partial)
+            buf.write_keys(partial + ((size_t)qi * gridDim.y + blockIdx.y) * K, K);
+        else
+            buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
+    }
 }
 
 // Runtime-D fallback: the query lives in LDS (lane-interleaved).
 template <int KIND>
-__global__ void __launch_bounds__(1024) bf_kernel_rt(const float* __restrict__ pts, uint32_t n, uint32_t D,
+__global__ void __launch_bounds__(1024) bf_kernel_rt(const float* __restrict__ pts_all, uint32_t n_all, uint32_t D,
                                                      const float* __restrict__ qs, uint32_t nq, float maxd2,
                                                      uint32_t K, int32_t* __restrict__ oidx,
-                                                     float* __restrict__ odist, uint32_t tile_pts) {
+                                                     float* __restrict__ odist, uint32_t tile_pts, uint32_t chunk,
+                                                     uint64_t* __restrict__ partial) {
+    const uint32_t c0 = blockIdx.y * chunk;
+    const uint32_t n = c0 < n_all ? min(chunk, n_all - c0) : 0u;
+    const float* pts = pts_all + (size_t)c0 * D;
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const uint32_t DP = (D + 3) & ~3u;
     const uint32_t BS = blockDim.x, tid = threadIdx.x;
@@ -95,7 +115,7 @@ __global__ void __launch_bounds__(1024) bf_kernel_rt(const float* __restrict__ p
             for (uint32_t j = 0; j < cnt; ++j) {
                 const float d2 = dist2_rt(qlds + tid, BS, tile + (size_t)j * DP, D);
                 if (f2u(d2) <= whi) {
-                    const uint64_t key = make_key(d2, base + j);
+                    const uint64_t key = make_key(d2, c0 + base + j);
                     if (key < worst) {
                         worst = buf.consider(key);
                         whi = key_hi(worst);
@@ -104,12 +124,48 @@ __global__ void __launch_bounds__(1024) bf_kernel_rt(const float* __restrict__ p
             }
         }
     }
-    if (active) buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
+    if (active) {
+        if (partial)
+            buf.write_keys(partial + ((size_t)qi * gridDim.y + blockIdx.y) * K, K);
+        else
+            buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
+    }
 }
 
-using BfFn = void (*)(const float*, uint32_t, const float*, uint32_t, float, uint32_t, int32_t*, float*, uint32_t);
+// K-way merge of the S ascending partial lists of each query: repeatedly take
+// the smallest head.  Heads live in LDS (lane-interleaved).  Writes ascending
+// (array mode) or descending (heap mode) rows.
+__global__ void __launch_bounds__(256) bf_merge_kernel(const uint64_t* __restrict__ partial, uint32_t nq, uint32_t S,
+                                                       uint32_t K, bool descending, int32_t* __restrict__ oidx,
+                                                       float* __restrict__ odist) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    uint16_t* head = reinterpret_cast<uint16_t*>(lds_raw);
+    const uint32_t BS = blockDim.x, tid = threadIdx.x;
+    const uint32_t qi = blockIdx.x * BS + tid;
+    if (qi >= nq) return;
+    const uint64_t* L = partial + (size_t)qi * S * K;
+    for (uint32_t s = 0; s < S; ++s) head[(size_t)s * BS + tid] = 0;
+    for (uint32_t k = 0; k < K; ++k) {
+        uint64_t best = ~0ull;
+        uint32_t bs = 0;
+        for (uint32_t s = 0; s < S; ++s) {
+            const uint32_t h = head[(size_t)s * BS + tid];
+            if (h < K) {
+                const uint64_t v = L[(size_t)s * K + h];
+                if (v < best) { best = v; bs = s; }
+            }
+        }
+        head[(size_t)bs * BS + tid] += 1;
+        const uint32_t o = descending ? K - 1 - k : k;
+        oidx[(size_t)qi * K + o] = (int32_t)(uint32_t)best;
+        odist[(size_t)qi * K + o] = u2f(key_hi(best));
+    }
+}
+
+using BfFn = void (*)(const float*, uint32_t, const float*, uint32_t, float, uint32_t, int32_t*, float*, uint32_t,
+                     uint32_t, uint64_t*);
 using BfRtFn = void (*)(const float*, uint32_t, uint32_t, const float*, uint32_t, float, uint32_t, int32_t*, float*,
-                        uint32_t);
+                        uint32_t, uint32_t, uint64_t*);
 
 template <int D>
 static BfFn bf_pick(int kind) {
@@ -119,7 +175,9 @@ static BfFn bf_pick(int kind) {
         case BUF_REG32: return bf_kernel<D, BUF_REG32>;
         case BUF_LDS_ARRAY: return bf_kernel<D, BUF_LDS_ARRAY>;
         case BUF_LDS_HEAP: return bf_kernel<D, BUF_LDS_HEAP>;
-        default: return bf_kernel<D, BUF_PRIV_HEAP>;
+        case BUF_REGH8: return bf_kernel<D, BUF_REGH8>;
+        case BUF_REGH16: return bf_kernel<D, BUF_REGH16>;
+        default: return bf_kernel<D, BUF_REGH32>;
     }
 }
 
@@ -130,19 +188,21 @@ static BfRtFn bf_pick_rt(int kind) {
         case BUF_REG32: return bf_kernel_rt<BUF_REG32>;
         case BUF_LDS_ARRAY: return bf_kernel_rt<BUF_LDS_ARRAY>;
         case BUF_LDS_HEAP: return bf_kernel_rt<BUF_LDS_HEAP>;
-        default: return bf_kernel_rt<BUF_PRIV_HEAP>;
+        case BUF_REGH8: return bf_kernel_rt<BUF_REGH8>;
+        case BUF_REGH16: return bf_kernel_rt<BUF_REGH16>;
+        default: return bf_kernel_rt<BUF_REGH32>;
     }
 }
 
 // Buffer kind from (USE_HEAP_TRACKING, USE_TRACKING_SHMEM, K).  Register
 // buffers hold at most 32 slots; larger K moves the buffer to LDS (same
-// results).
+// results).  Heap tracking in LDS is a true max-heap; in registers it is the
+// static sorted network emitting max-heap order.
 int pick_buf_kind(bool heap, bool lds, uint32_t K) {
-    if (heap) return (lds || K > 32) ? BUF_LDS_HEAP : BUF_PRIV_HEAP;
-    if (lds || K > 32) return BUF_LDS_ARRAY;
-    if (K <= 8) return BUF_REG8;
-    if (K <= 16) return BUF_REG16;
-    return BUF_REG32;
+    if (lds || K > 32) return heap ? BUF_LDS_HEAP : BUF_LDS_ARRAY;
+    if (K <= 8) return heap ? BUF_REGH8 : BUF_REG8;
+    if (K <= 16) return heap ? BUF_REGH16 : BUF_REG16;
+    return heap ? BUF_REGH32 : BUF_REG32;
 }
 
 static bool buf_in_lds(int kind) { return kind == BUF_LDS_ARRAY || kind == BUF_LDS_HEAP; }
@@ -150,7 +210,10 @@ static bool buf_in_lds(int kind) { return kind == BUF_LDS_ARRAY || kind == BUF_L
 void query_bf(Handle& h, const QueryArgs& a) {
     const uint32_t D = h.info.dims;
     const int kind = pick_buf_kind(a.heap, a.track_lds, a.K);
-    const uint32_t BS = a.block;
+    // BLOCK_SIZE only changes the schedule, never the result: kernels built
+    // for at most 256 lanes run larger requests as 256-lane workgroups.
+    const bool big_d = D >= 32 && (D == 32 || D == 64 || D == 128);
+    const uint32_t BS = (big_d && a.block > 256) ? 256 : a.block;
     const uint32_t DP = (D + 3) & ~3u;
     uint32_t tile_pts = 8192 / DP;
     if (tile_pts > 1024) tile_pts = 1024;
@@ -177,14 +240,39 @@ void query_bf(Handle& h, const QueryArgs& a) {
                 std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",
             KNN_ERR_INVALID_ARG);
     const uint32_t grid = (a.nq + BS - 1) / BS;
+    // Too few query blocks to fill the CUs: split the points into S chunks
+    // (grid.y), each writing partial top-K keys, then merge.
+    const uint32_t n = h.info.n;
+    const uint32_t want_blocks = 4 * device_limits().cu_count;
+    uint32_t S = 1;
+    if (grid < want_blocks && n >= 2048) {
+        S = (want_blocks + grid - 1) / grid;
+        const uint32_t max_by_size = n / 1024;
+        if (S > max_by_size) S = max_by_size;
+        if (S > 64) S = 64;
+        if (S < 1) S = 1;
+        while (S > 1 && (size_t)a.nq * S * a.K * 8 > ((size_t)1 << 30)) S /= 2;
+    }
+    const uint32_t chunk = S > 1 ? (n + S - 1) / S : (n ? n : 1);
+    uint64_t* partial = nullptr;
+    if (S > 1) {
+        const size_t words = (size_t)a.nq * S * a.K;
+        h.q_scratch.ensure(words * 8);
+        partial = reinterpret_cast<uint64_t*>(h.q_scratch.get());
+    }
     EventTimer timer(a.timed, a.stream);
     if (fn) {
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(BS), lds, a.stream, h.pts.get(), h.info.n, a.q, a.nq, a.maxd2, a.K,
-                           a.oidx, a.odist, tile_pts);
+        hipLaunchKernelGGL(fn, dim3(grid, S), dim3(BS), lds, a.stream, h.pts.get(), n, a.q, a.nq, a.maxd2, a.K,
+                           a.oidx, a.odist, tile_pts, chunk, partial);
     } else {
         BfRtFn rf = bf_pick_rt(kind);
-        hipLaunchKernelGGL(rf, dim3(grid), dim3(BS), lds, a.stream, h.pts.get(), h.info.n, D, a.q, a.nq, a.maxd2,
-                           a.K, a.oidx, a.odist, tile_pts);
+        hipLaunchKernelGGL(rf, dim3(grid, S), dim3(BS), lds, a.stream, h.pts.get(), n, D, a.q, a.nq, a.maxd2, a.K,
+                           a.oidx, a.odist, tile_pts, chunk, partial);
+    }
+    if (S > 1) {
+        const uint32_t mb = 128;
+        hipLaunchKernelGGL(bf_merge_kernel, dim3((a.nq + mb - 1) / mb), dim3(mb), (size_t)mb * S * 2, a.stream,
+                           partial, a.nq, S, a.K, a.heap, a.oidx, a.odist);
     }
     KNN_CHECK_LAUNCH();
     h.stats[4] = timer.stop();

@@ -67,12 +67,26 @@ static void check_pts(const float* d_pts, uint32_t n, uint32_t dims) {
 
 static hipStream_t null_stream() { return nullptr; }
 
-// Tuned defaults (BLOCK_SIZE, leaf span) when the caller passes 0.
-static uint32_t default_block(const Handle& h) { return h.kind == KNN_KIND_BRUTE ? 256 : 128; }
-static uint32_t default_span(const Handle& h) {
-    // Scan about 8 points per leaf visit.
-    const uint32_t s = h.info.depth_skips;
-    return s >= 3 ? 0 : 3 - s;
+// Tuned defaults (BLOCK_SIZE, leaf span, query order) when the caller passes 0,
+// measured on the 16.7M-point / 4.2M-query 3-D mixture with K = 16:
+//  - trees: one wave per workgroup (the wave-cooperative traversal keeps its
+//    stack per wave; 64 measured fastest);
+//  - kd: leaf groups of ~2K points, queries in tree-cell order;
+//  - ball: leaf groups of ~K points, queries in Morton order;
+//  - per-lane traversal: groups of ~K points.
+static uint32_t default_block(const Handle& h) { return h.kind == KNN_KIND_BRUTE ? 256 : 64; }
+static uint32_t ceil_log2(uint64_t x) {
+    uint32_t r = 0;
+    while ((1ull << r) < x) ++r;
+    return r;
+}
+static uint32_t default_span(const Handle& h, uint32_t K, bool per_lane) {
+    if (h.kind == KNN_KIND_BRUTE || h.info.max_children == 0) return 0;
+    const uint64_t target = (h.kind == KNN_KIND_KD && !per_lane) ? 2ull * K : (uint64_t)K;
+    const uint64_t mc = h.info.max_children;
+    uint32_t s = target > mc ? ceil_log2((target + mc - 1) / mc) : 0;
+    if (s > h.info.node_depth) s = h.info.node_depth;
+    return s;
 }
 
 }  // namespace knn
@@ -184,9 +198,12 @@ knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max
         a.stack_lds = o.stack_lds != 0;
         a.sort_queries = (o.flags & KNN_FLAG_NO_QUERY_SORT) == 0;
         a.count_visits = (o.flags & KNN_FLAG_COUNT_VISITS) != 0;
+        a.per_lane = (o.flags & KNN_FLAG_PER_LANE) != 0;
+        a.order_morton = (o.flags & KNN_FLAG_MORTON_ORDER) != 0 || h.kind == KNN_KIND_BALL;
         a.timed = (opts == nullptr || opts->stream == nullptr);
         for (float& v : h.stats) v = 0.0f;
-        a.leaf_span = o.leaf_span == 0 ? default_span(h) : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
+        a.leaf_span = o.leaf_span == 0 ? default_span(h, k, a.per_lane)
+                                       : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
         a.form = (knn_form)o.form;
         a.stream = (hipStream_t)o.stream;
         hipEvent_t e0, e1;

@@ -80,7 +80,11 @@ __device__ __forceinline__ float select_dim(const float (&q)[D], uint32_t dim) {
 // KCAP-K slots hold key 0 (below every real key except (0.0, index 0), with
 // which it is interchangeable), so the K-th key always sits at slot KCAP-1
 // and every index is static (KnnTrees.cuh:136-152 insertion).
-template <int KCAP>
+// DESC = heap tracking kept in registers: a per-lane heap needs dynamic
+// indexing, which VGPRs do not have (it would live in scratch), so the
+// register form keeps the same sorted network and emits the keys descending,
+// the canonical max-heap layout.
+template <int KCAP, bool DESC = false>
 struct RegArray {
     uint64_t s[KCAP];
     __device__ __forceinline__ uint64_t init(uint64_t empty, uint32_t K) {
@@ -98,13 +102,22 @@ struct RegArray {
         }
         return s[KCAP - 1];
     }
+    // Raw keys, ascending (partial results of a split scan).
+    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t K) {
+#pragma unroll
+        for (int i = 0; i < KCAP; ++i) {
+            int j = i - (KCAP - (int)K);
+            if (j >= 0) ok[j] = s[i];
+        }
+    }
     __device__ __forceinline__ void write(int32_t* __restrict__ oi, float* __restrict__ od, uint32_t K) {
 #pragma unroll
         for (int i = 0; i < KCAP; ++i) {
             int j = i - (KCAP - (int)K);
             if (j >= 0) {
-                oi[j] = (int32_t)(uint32_t)s[i];
-                od[j] = u2f(key_hi(s[i]));
+                const int o = DESC ? (int)K - 1 - j : j;
+                oi[o] = (int32_t)(uint32_t)s[i];
+                od[o] = u2f(key_hi(s[i]));
             }
         }
     }
@@ -145,6 +158,9 @@ struct LdsArray {
             od[i] = u2f(key_hi(v));
         }
     }
+    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t) {
+        for (uint32_t i = 0; i < K; ++i) ok[i] = s[i];
+    }
 };
 
 // Max-heap (KnnTrees.cuh:154-189, with the right-child read guarded).
@@ -176,14 +192,22 @@ struct Heap {
         sift(key, 0, K);
         return s[0];
     }
-    // In-place heapsort leaves keys ascending; emit them descending.
-    __device__ __forceinline__ void write(int32_t* __restrict__ oi, float* __restrict__ od, uint32_t) {
+    // In-place heapsort leaves keys ascending.
+    __device__ __forceinline__ void sort_ascending() {
         for (uint32_t n = K; n > 1; --n) {
             uint64_t top = s[0];
             uint64_t last = s[n - 1];
             s[n - 1] = top;
             sift(last, 0, n - 1);
         }
+    }
+    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t) {
+        sort_ascending();
+        for (uint32_t i = 0; i < K; ++i) ok[i] = s[i];
+    }
+    // Emit descending (canonical max-heap order).
+    __device__ __forceinline__ void write(int32_t* __restrict__ oi, float* __restrict__ od, uint32_t) {
+        sort_ascending();
         for (uint32_t i = 0; i < K; ++i) {
             uint64_t v = s[K - 1 - i];
             oi[i] = (int32_t)(uint32_t)v;
@@ -192,22 +216,11 @@ struct Heap {
     }
 };
 
-// Heap in per-lane private memory (scratch), runtime K <= KCAP.
-template <int KCAP>
-struct PrivHeap {
-    uint64_t mem[KCAP];
-    Heap h;
-    __device__ __forceinline__ uint64_t init(uint64_t empty, uint32_t k) {
-        h.s.p = mem;
-        h.s.stride = 1;
-        return h.init(empty, k);
-    }
-    __device__ __forceinline__ uint64_t consider(uint64_t key) { return h.consider(key); }
-    __device__ __forceinline__ void write(int32_t* oi, float* od, uint32_t K) { h.write(oi, od, K); }
-};
-
 // Buffer kinds selected by the host (USE_HEAP_TRACKING x USE_TRACKING_SHMEM).
-enum BufKind : int { BUF_REG8 = 0, BUF_REG16 = 1, BUF_REG32 = 2, BUF_LDS_ARRAY = 3, BUF_LDS_HEAP = 4, BUF_PRIV_HEAP = 5 };
+enum BufKind : int {
+    BUF_REG8 = 0, BUF_REG16 = 1, BUF_REG32 = 2, BUF_LDS_ARRAY = 3, BUF_LDS_HEAP = 4,
+    BUF_REGH8 = 5, BUF_REGH16 = 6, BUF_REGH32 = 7
+};
 
 template <int KIND> struct BufOf;
 template <> struct BufOf<BUF_REG8> { using T = RegArray<8>; static constexpr bool lds = false; };
@@ -215,7 +228,9 @@ template <> struct BufOf<BUF_REG16> { using T = RegArray<16>; static constexpr b
 template <> struct BufOf<BUF_REG32> { using T = RegArray<32>; static constexpr bool lds = false; };
 template <> struct BufOf<BUF_LDS_ARRAY> { using T = LdsArray; static constexpr bool lds = true; };
 template <> struct BufOf<BUF_LDS_HEAP> { using T = Heap; static constexpr bool lds = true; };
-template <> struct BufOf<BUF_PRIV_HEAP> { using T = PrivHeap<32>; static constexpr bool lds = false; };
+template <> struct BufOf<BUF_REGH8> { using T = RegArray<8, true>; static constexpr bool lds = false; };
+template <> struct BufOf<BUF_REGH16> { using T = RegArray<16, true>; static constexpr bool lds = false; };
+template <> struct BufOf<BUF_REGH32> { using T = RegArray<32, true>; static constexpr bool lds = false; };
 
 template <class B>
 __device__ __forceinline__ void bind_lds(B&, uint64_t*, uint32_t) {}

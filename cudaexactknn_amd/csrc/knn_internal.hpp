@@ -140,7 +140,7 @@ struct QueryArgs {
     float* odist;
     uint32_t K;
     uint32_t block;
-    bool heap, track_lds, stack_lds, sort_queries, count_visits;
+    bool heap, track_lds, stack_lds, sort_queries, count_visits, per_lane, order_morton;
     uint32_t leaf_span;  // extra levels folded into one leaf scan
     knn_form form;
     hipStream_t stream;

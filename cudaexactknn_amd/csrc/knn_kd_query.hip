@@ -132,7 +132,9 @@ static TreeFn kd_pick1(int kind, bool stack_lds) {
         case BUF_REG32: return kd_pick2<D, BUF_REG32>(stack_lds);
         case BUF_LDS_ARRAY: return kd_pick2<D, BUF_LDS_ARRAY>(stack_lds);
         case BUF_LDS_HEAP: return kd_pick2<D, BUF_LDS_HEAP>(stack_lds);
-        default: return kd_pick2<D, BUF_PRIV_HEAP>(stack_lds);
+        case BUF_REGH8: return kd_pick2<D, BUF_REGH8>(stack_lds);
+        case BUF_REGH16: return kd_pick2<D, BUF_REGH16>(stack_lds);
+        default: return kd_pick2<D, BUF_REGH32>(stack_lds);
     }
 }
 

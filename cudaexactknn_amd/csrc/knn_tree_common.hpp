@@ -25,6 +25,7 @@ struct TreeArgs {
     uint32_t sd_inv;       // ceil(65536 / split_dims): level % SD without a divide
     uint32_t stop_level;   // descend to this level, then scan every leaf slot below
     uint32_t stack_depth;  // LDS stack entries per lane
+    uint32_t prime_level;  // wave-cooperative kernels: level whose subtree each lane scans first
     uint32_t dims;         // runtime D (generic kernels)
     const float* qs;
     const uint32_t* perm;  // optional processing order
@@ -102,6 +103,52 @@ __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uin
     }
 }
 
+// Wave-cooperative scan of leaf slots [slot0, slot0 + count) (count, slot0
+// wave-uniform).  Every lane loads the same slot address (one cache-line
+// request per wave), four slots are in flight per step, and readfirstlane
+// moves each slot into SGPRs; every lane then tests the slot against its own
+// query.  (Scalar loads through the constant address space were measured
+// slower here: ~20 waves per CU stream different leaves through a small
+// scalar cache.)  Slots inside the lane's primed range [land0, land0 + landN)
+// were already considered and are skipped.
+__device__ __forceinline__ float bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+template <int D, class Q, class B>
+__device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
+                                            bool stop_at_empty, const Q& q, bool active, uint32_t land0,
+                                            uint32_t landN, B& buf, uint64_t& worst, uint32_t& whi,
+                                            uint32_t& nslots) {
+    constexpr uint32_t stride = 1 + D;
+    // A group lies wholly inside or wholly outside the lane's primed range
+    // when it is no larger than that range; otherwise test slot by slot.
+    const bool group_primed = slot0 - land0 < landN && count <= landN;
+    const bool per_slot = count > landN;
+    for (uint32_t o = 0; o < count; ++o) {
+        const uint32_t sl = slot0 + o;
+        const float* e = leaves + (size_t)sl * stride;
+        const int32_t pi = __float_as_int(bcast(e[0]));
+        ++nslots;
+        if (pi == -1) {
+            if (stop_at_empty) break;
+            continue;
+        }
+        float p[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) p[d] = bcast(e[1 + d]);
+        const float d2 = q.dist_to(p);
+        const bool mine = active && (per_slot ? (sl - land0 >= landN) : !group_primed);
+        if (mine && f2u(d2) <= whi) {
+            const uint64_t key = make_key(d2, (uint32_t)pi);
+            if (key < worst) {
+                worst = buf.consider(key);
+                whi = key_hi(worst);
+            }
+        }
+    }
+}
+
 // Leaf-slot range under `node` (at `level`), scanning the full subtree when
 // level < node_depth.
 __device__ __forceinline__ void leaf_range(const TreeArgs& A, uint32_t node, uint32_t level, uint32_t& slot0,
@@ -110,6 +157,12 @@ __device__ __forceinline__ void leaf_range(const TreeArgs& A, uint32_t node, uin
     const uint32_t leftmost = ((node + 1) << down) - 1;
     slot0 = (leftmost - A.node_size) * A.max_children;
     count = A.max_children << down;
+}
+
+// Bytes of the per-wave traversal stacks of the wave-cooperative kernels,
+// rounded up so the K-buffer that follows is 16-byte aligned.
+__host__ __device__ __forceinline__ size_t packet_stack_bytes(uint32_t nwaves, uint32_t depth, uint32_t words) {
+    return ((size_t)nwaves * depth * words * 4 + 15) & ~(size_t)15;
 }
 
 __device__ __forceinline__ uint32_t level_dim(uint32_t level, uint32_t sd, uint32_t sd_inv) {

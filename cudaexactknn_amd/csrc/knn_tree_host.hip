@@ -14,6 +14,8 @@ using TreeFn = void (*)(TreeArgs);
 using BallFn = void (*)(TreeArgs, float, float);
 TreeFn kd_kernel_for(uint32_t D, int kind, bool stack_lds, bool& generic);
 BallFn ball_kernel_for(uint32_t D, int kind, bool stack_lds, bool& generic);
+TreeFn kd_packet_kernel_for(uint32_t D, int kind);
+BallFn ball_packet_kernel_for(uint32_t D, int kind);
 int pick_buf_kind(bool heap, bool lds, uint32_t K);
 
 __global__ void morton_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D, const float* __restrict__ bbox,
@@ -38,6 +40,45 @@ __global__ void morton_kernel(const float* __restrict__ qs, uint32_t nq, uint32_
     vals[i] = i;
 }
 
+// Tree order: each query descends the top `levels` levels of the tree (kd:
+// split planes; ball: nearer centroid) and its landing cell, numbered left to
+// right, is its sort key.  Cells follow the data density, unlike a fixed grid.
+__global__ void kd_order_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D, const float* __restrict__ nodes,
+                                uint32_t levels, uint32_t SD, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    uint32_t node = 0, dim = 0;
+    for (uint32_t l = 0; l < levels; ++l) {
+        const float x = qs[(size_t)i * D + dim];
+        node = 2 * node + 1 + (x < nodes[node] ? 0u : 1u);
+        dim = (dim + 1 == SD) ? 0 : dim + 1;
+    }
+    keys[i] = node - ((1u << levels) - 1u);
+    vals[i] = i;
+}
+
+__global__ void ball_order_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D,
+                                  const float* __restrict__ nodes, uint32_t levels, uint32_t* __restrict__ keys,
+                                  uint32_t* __restrict__ vals) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float* q = qs + (size_t)i * D;
+    uint32_t node = 0;
+    for (uint32_t l = 0; l < levels; ++l) {
+        const float* nd = nodes + (size_t)node * (2 * D + 2);
+        float dl = 0.0f, dr = 0.0f;
+        for (uint32_t d = 0; d < D; ++d) {
+            const float a = q[d] - nd[d], b = q[d] - nd[D + 1 + d];
+            dl += a * a;
+            dr += b * b;
+        }
+        node = 2 * node + 1 + (dl <= dr ? 0u : 1u);
+    }
+    keys[i] = node - ((1u << levels) - 1u);
+    vals[i] = i;
+}
+
 void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
     const uint32_t n = a.nq, D = h.info.dims;
     EventTimer timer(a.timed, a.stream);
@@ -46,12 +87,24 @@ void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
     h.q_keys_alt.ensure(n);
     h.q_vals_alt.ensure(n);
     h.q_temp.ensure(radix_temp_words(n));
-    hipLaunchKernelGGL(morton_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D, h.bbox.get(),
-                       h.q_keys.get(), h.q_vals.get());
-    const uint32_t Dm = D < 30 ? D : 30;
-    uint32_t bpd = 30 / Dm;
-    if (bpd > 10) bpd = 10;
-    const int bits = (int)(bpd * Dm);
+    int bits;
+    const uint32_t tree_levels = h.info.node_depth < 24 ? h.info.node_depth : 24;
+    if (a.order_morton || h.info.n == 0) {
+        hipLaunchKernelGGL(morton_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D, h.bbox.get(),
+                           h.q_keys.get(), h.q_vals.get());
+        const uint32_t Dm = D < 30 ? D : 30;
+        uint32_t bpd = 30 / Dm;
+        if (bpd > 10) bpd = 10;
+        bits = (int)(bpd * Dm);
+    } else if (h.kind == KNN_KIND_KD) {
+        hipLaunchKernelGGL(kd_order_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D, h.nodes.get(),
+                           tree_levels, h.info.split_dims, h.q_keys.get(), h.q_vals.get());
+        bits = (int)tree_levels;
+    } else {
+        hipLaunchKernelGGL(ball_order_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D,
+                           h.nodes.get(), tree_levels, h.q_keys.get(), h.q_vals.get());
+        bits = (int)tree_levels;
+    }
     const int end = (bits + 7) / 8 * 8;
     bool alt = radix_sort_pairs(h.q_keys.get(), h.q_vals.get(), h.q_keys_alt.get(), h.q_vals_alt.get(), n, 0, end,
                                 h.q_temp.get(), a.stream);
@@ -74,6 +127,13 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     const uint32_t span = a.leaf_span < I.node_depth ? a.leaf_span : I.node_depth;
     A.stop_level = I.node_depth - span;
     A.stack_depth = A.stop_level + 1;
+    {
+        // Deepest level whose subtrees all hold >= 2K points (each holds
+        // floor or ceil of N / 2^level), capped at the scan level.
+        uint32_t pl = 0;
+        while (pl < A.stop_level && ((uint64_t)I.n >> (pl + 1)) >= 2ull * a.K) ++pl;
+        A.prime_level = pl;
+    }
     A.dims = I.dims;
     A.qs = a.q;
     A.perm = d_perm;
@@ -95,15 +155,29 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     bool generic = false;
     TreeFn kfn = nullptr;
     BallFn bfn = nullptr;
-    if (h.kind == KNN_KIND_KD)
-        kfn = kd_kernel_for(I.dims, kind, a.stack_lds, generic);
-    else
-        bfn = ball_kernel_for(I.dims, kind, a.stack_lds, generic);
-    const uint32_t BS = a.block;
+    uint32_t BS = a.block;
     size_t lds = 0;
-    if (a.stack_lds) lds += (size_t)BS * A.stack_depth * 8;
+    // Wave-cooperative traversal for ordered batches (needs whole waves).
+    const bool packet = !a.per_lane && d_perm != nullptr && BS % 64 == 0;
+    if (packet) {
+        if (BS > 256) BS = 256;
+        if (h.kind == KNN_KIND_KD)
+            kfn = kd_packet_kernel_for(I.dims, kind);
+        else
+            bfn = ball_packet_kernel_for(I.dims, kind);
+    }
+    if (kfn || bfn) {
+        const uint32_t words = h.kind == KNN_KIND_KD ? 1 + 2 * I.dims : 1;
+        lds = packet_stack_bytes((BS + 63) / 64, A.stack_depth, words);
+    } else {
+        if (h.kind == KNN_KIND_KD)
+            kfn = kd_kernel_for(I.dims, kind, a.stack_lds, generic);
+        else
+            bfn = ball_kernel_for(I.dims, kind, a.stack_lds, generic);
+        if (a.stack_lds) lds += (size_t)BS * A.stack_depth * 8;
+        if (generic) lds += (size_t)BS * I.dims * 4;
+    }
     if (lds_buf) lds += (size_t)BS * a.K * 8;
-    if (generic) lds += (size_t)BS * I.dims * 4;
     require(lds <= device_limits().max_lds_per_block,
             "LDS request of " + std::to_string(lds) + " bytes exceeds the per-workgroup limit of " +
                 std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",

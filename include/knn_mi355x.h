@@ -58,8 +58,12 @@ typedef enum knn_kind { KNN_KIND_BRUTE = 0, KNN_KIND_KD = 1, KNN_KIND_BALL = 2 }
 typedef enum knn_form { KNN_FORM_PLAIN = 0, KNN_FORM_BF16 = 1, KNN_FORM_LEARNED = 2 } knn_form;
 
 /* Flags for knn_query_opts.flags. */
-#define KNN_FLAG_NO_QUERY_SORT 0x1u  /* trees: keep caller order (no Morton reordering) */
+#define KNN_FLAG_NO_QUERY_SORT 0x1u  /* trees: keep caller order (no spatial reordering) */
 #define KNN_FLAG_COUNT_VISITS 0x2u   /* trees: count node bytes and leaf slots read (stats[5], [6]) */
+#define KNN_FLAG_MORTON_ORDER 0x8u   /* kd: order queries by Morton code instead of tree cell
+                                        (ball trees always use Morton order) */
+#define KNN_FLAG_PER_LANE 0x4u       /* trees: one independent traversal per lane (the reference's
+                                        shape) instead of the wave-cooperative traversal */
 
 typedef struct knn_query_opts {
     uint32_t block;      /* BLOCK_SIZE: threads per workgroup, 1..1024 (0 = tuned default) */

@@ -53,6 +53,23 @@ def test_bf_dims_and_variants(oracle, cuda, d):
             same(run(bf, qs, k, cuda, BLOCK_SIZE=64, **v), want, f"d={d} k={k} {v}")
 
 
+@pytest.mark.parametrize("d", [3, 8, 32, 64, 128, 7])
+def test_bf_split_scan_matches_oracle(oracle, cuda, d):
+    """Few queries and many points: the scan is split across workgroups
+    (partial top-K per point chunk, then a K-way merge)."""
+    g = np.random.Generator(np.random.Philox(70 + d))
+    pts = (g.standard_normal((30000, d)) * np.exp(g.uniform(-1, 1, d))).astype(np.float32)
+    pts[100] = pts[20000]
+    qs = (g.standard_normal((200, d)) * 1.3).astype(np.float32)
+    qs[0] = pts[20000]
+    bf = knn.BruteForceGpu(dev(pts, cuda))
+    for k in (1, 16, 33):
+        for v in BF_VARIANTS:
+            for md in (np.inf, float(np.float32(0.5 * d))):
+                want = oracle.bf_knn(pts, qs, k, max_d2=md, heap=v["USE_HEAP_TRACKING"])
+                same(run(bf, qs, k, cuda, md=md, BLOCK_SIZE=128, **v), want, f"split d={d} k={k} md={md} {v}")
+
+
 def test_bf_lattice_and_radius_edges(oracle, cuda):
     pts, qs = pins.lattice_case(33, 500, 90, 3)
     bf = knn.BruteForceGpu(dev(pts, cuda))
@@ -160,8 +177,11 @@ def test_tree_queries_all_variants(oracle, cuda, kind, d):
             for t in trees:
                 for v in TREE_VARIANTS:
                     for span in (0, 0xFFFFFFFF, 2):
-                        got = run(t, qs, k, cuda, md=md, BLOCK_SIZE=64, leaf_span=span, **v)
-                        same(got, want[v["USE_HEAP_TRACKING"]], f"{kind} d={d} k={k} md={md} span={span} {v}")
+                        for per_lane in (False, True):
+                            for bs in ((64, 192) if not per_lane else (64,)):
+                                got = run(t, qs, k, cuda, md=md, BLOCK_SIZE=bs, leaf_span=span, per_lane=per_lane, **v)
+                                same(got, want[v["USE_HEAP_TRACKING"]],
+                                     f"{kind} d={d} k={k} md={md} span={span} lane={per_lane} bs={bs} {v}")
 
 
 @pytest.mark.parametrize("kind", ["kd", "ball"])
@@ -172,7 +192,9 @@ def test_tree_radius_edges_and_ties(oracle, cuda, kind):
     for md in (np.inf, 0.0, -0.0, -5.0, 1.0, 2.0, 3.0, float("nan")):
         for heap in (False, True):
             ei, ed = pins.exact_lattice_knn(pts, qs, 14, md, heap)
-            same(run(t, qs, 14, cuda, md=md, USE_HEAP_TRACKING=heap), (ei, ed), f"{kind} md={md} heap={heap}")
+            for per_lane in (False, True):
+                same(run(t, qs, 14, cuda, md=md, USE_HEAP_TRACKING=heap, per_lane=per_lane), (ei, ed),
+                     f"{kind} md={md} heap={heap} lane={per_lane}")
 
 
 @pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 17])
