@@ -259,3 +259,54 @@ def test_uniform_small_config_bit_exact(oracle, cuda):
     for v in TREE_VARIANTS:
         if v["USE_HEAP_TRACKING"]:
             same(run(kd, qs, 8, cuda, **v), want, f"kd {v}")
+
+
+@pytest.mark.parametrize("d", [3, 16, 32, 64, 100, 128])
+def test_bf16_form_equals_plain(oracle, cuda, d):
+    """BF16-MFMA filter + exact rerank returns the plain FP32 result bit for
+    bit, on per-dimension-scaled Gaussian data, with ties and near-ties."""
+    g = np.random.Generator(np.random.Philox(900 + d))
+    scale = np.exp(g.uniform(np.log(0.1), np.log(10.0), d))
+    pts = (g.standard_normal((20000, d)) * scale).astype(np.float32)
+    qs = (g.standard_normal((700, d)) * scale).astype(np.float32)
+    pts[5] = pts[6]                        # duplicate point: exact tie
+    qs[0] = pts[6]                         # query on a point: distance 0
+    qs[1] = pts[7] + np.float32(1e-6)      # near-coincident
+    qs[2] = pts[8] * np.float32(1 + 2.0 ** -20)
+    bf = knn.BruteForceGpu(dev(pts, cuda))
+    for k in (1, 8, 32):
+        for heap in (False, True):
+            for md in (np.inf, float(np.float32(np.sum(scale ** 2)))):
+                want = oracle.bf_knn(pts, qs, k, max_d2=md, heap=heap)
+                got = run(bf, qs, k, cuda, md=md, USE_HEAP_TRACKING=heap, form=knn.FORM_BF16)
+                same(got, want, f"bf16 d={d} k={k} heap={heap} md={md}")
+    st = bf.last_query_stats()
+    assert 0.0 < st["bf16_survivor_fraction"] < 0.5
+
+
+def test_bf16_form_overflow_falls_back(oracle, cuda):
+    """Every point equidistant from the query: all are candidates, the
+    candidate buffer overflows and the plain kernel recomputes the query."""
+    g = np.random.Generator(np.random.Philox(5))
+    d = 32
+    v = g.standard_normal((6000, d))
+    pts = (v / np.linalg.norm(v, axis=1, keepdims=True) * 4.0).astype(np.float32)
+    qs = np.zeros((3, d), np.float32)
+    qs[1] = pts[0]
+    bf = knn.BruteForceGpu(dev(pts, cuda))
+    for k in (4, 32):
+        want = oracle.bf_knn(pts, qs, k)
+        same(run(bf, qs, k, cuda, form=knn.FORM_BF16), want, f"overflow k={k}")
+    assert bf.last_query_stats()["rerun_queries"] >= 1
+
+
+def test_bf16_form_radius_and_small_sets(oracle, cuda):
+    g = np.random.Generator(np.random.Philox(17))
+    for n in (1, 31, 33, 100):
+        pts = g.standard_normal((n, 40)).astype(np.float32)
+        qs = g.standard_normal((50, 40)).astype(np.float32)
+        bf = knn.BruteForceGpu(dev(pts, cuda))
+        for md in (np.inf, 60.0, 0.0, -1.0):
+            for k in (1, 5, 40):
+                want = oracle.bf_knn(pts, qs, k, max_d2=md)
+                same(run(bf, qs, k, cuda, md=md, form=knn.FORM_BF16), want, f"n={n} md={md} k={k}")
