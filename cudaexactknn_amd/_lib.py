@@ -29,7 +29,8 @@ KNN_FLAG_MORTON_ORDER = 0x8
 EXPORTED = (
     "knn_version", "knn_bf_create", "knn_kd_build", "knn_ball_build", "knn_query", "knn_destroy",
     "knn_last_error", "knn_pinned_alloc", "knn_pinned_free", "knn_tree_get_info", "knn_tree_export",
-    "knn_last_build_ms", "knn_learned_train", "knn_last_query_stats",
+    "knn_last_build_ms", "knn_learned_train", "knn_last_query_stats", "knn_mlp_param_count", "knn_mlp_forward",
+    "knn_mlp_gradient",
 )
 
 
@@ -89,9 +90,13 @@ def load() -> ctypes.CDLL:
     L.knn_last_build_ms.argtypes = [P]
     L.knn_learned_train.argtypes = [P, P, u32, u32, u32, f32, P]
     L.knn_last_query_stats.argtypes = [P, P]
+    L.knn_mlp_param_count.argtypes = [ctypes.POINTER(u32)]
+    L.knn_mlp_forward.argtypes = [P, P, u32, u32, P]
+    L.knn_mlp_gradient.argtypes = [P, P, P, u32, P, ctypes.POINTER(f32)]
     for name in ("knn_bf_create", "knn_kd_build", "knn_ball_build", "knn_query", "knn_destroy",
                  "knn_pinned_alloc", "knn_pinned_free", "knn_tree_get_info", "knn_tree_export",
-                 "knn_learned_train", "knn_last_query_stats"):
+                 "knn_learned_train", "knn_last_query_stats", "knn_mlp_param_count", "knn_mlp_forward",
+                 "knn_mlp_gradient"):
         getattr(L, name).restype = st
     _LIB = L
     return L

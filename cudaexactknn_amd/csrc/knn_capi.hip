@@ -295,6 +295,32 @@ knn_status knn_learned_train(void* handle, const float* d_sample, uint32_t n_sam
     });
 }
 
+knn_status knn_mlp_param_count(uint32_t* out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        *out = kMlpParams;
+    });
+}
+
+knn_status knn_mlp_forward(const float* d_params, const float* d_x, uint32_t n, uint32_t bf16, float* d_out) {
+    return guarded([&] {
+        require(d_params && d_x && d_out, "NULL pointer");
+        if (n == 0) return;
+        mlp_forward(d_params, d_x, n, bf16 != 0, d_out, nullptr);
+        KNN_HIP(hipDeviceSynchronize());
+    });
+}
+
+knn_status knn_mlp_gradient(const float* d_params, const float* d_x, const float* d_t, uint32_t n, float* d_grad,
+                            float* h_loss) {
+    return guarded([&] {
+        require(d_params && d_x && d_t && d_grad, "NULL pointer");
+        require(n >= 1, "empty batch");
+        mlp_gradient(d_params, d_x, d_t, n, d_grad, h_loss, nullptr);
+        KNN_HIP(hipDeviceSynchronize());
+    });
+}
+
 knn_status knn_last_query_stats(const void* handle, float* out8) {
     return guarded([&] {
         require(out8 != nullptr, "out8 is NULL");

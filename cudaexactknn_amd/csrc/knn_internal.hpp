@@ -193,5 +193,9 @@ void learned_train(Handle& h, const float* d_sample, uint32_t n_sample, uint32_t
                    float* stats, hipStream_t st);
 void query_tree_learned(Handle& h, const QueryArgs& a);
 void destroy_mlp(Mlp* m);
+void mlp_gradient(const float* d_params, const float* d_x, const float* d_t, uint32_t n, float* d_grad, float* loss,
+                  hipStream_t st);
+void mlp_forward(const float* d_params, const float* d_x, uint32_t n, bool bf16, float* d_out, hipStream_t st);
+constexpr uint32_t kMlpParams = 12 * 64 + 64 + 64 * 64 + 64 + 64 + 1;
 
 }  // namespace knn

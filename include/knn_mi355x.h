@@ -157,6 +157,19 @@ knn_status knn_learned_train(void* handle, const float* d_sample, uint32_t n_sam
  * All timings are hipEvent-measured on the query's stream. */
 knn_status knn_last_query_stats(const void* handle, float* out8);
 
+/* Radius-predictor MLP kernels used by the learned form (no reference
+ * counterpart), exposed for direct use and for gradient checks.  The network
+ * is 12 -> 64 -> 64 -> 1 with ReLU; parameters are one float array laid out
+ * W1[64][12], b1[64], W2[64][64], b2[64], W3[64], b3 (knn_mlp_param_count).
+ * forward: d_out[i] = f(d_x[i*12 ...]) in FP32, or with bf16 weights and
+ * activations and FP32 accumulation when bf16 != 0.
+ * gradient: d_grad = d/dparams of mean_i (f(x_i) - t_i)^2 (FP32), *h_loss =
+ * that mean (h_loss may be NULL). */
+knn_status knn_mlp_param_count(uint32_t* out);
+knn_status knn_mlp_forward(const float* d_params, const float* d_x, uint32_t n, uint32_t bf16, float* d_out);
+knn_status knn_mlp_gradient(const float* d_params, const float* d_x, const float* d_t, uint32_t n, float* d_grad,
+                            float* h_loss);
+
 #ifdef __cplusplus
 }
 #endif

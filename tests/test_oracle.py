@@ -233,3 +233,21 @@ def test_planted_mistakes_are_caught(oracle, mutation):
 def test_faithful_oracle_passes_all_pins(oracle):
     fails, *_ = _pins_all(oracle)
     assert fails == []
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The FP64 MLP oracle's analytic gradient equals central differences."""
+    from oracle import mlp
+    g = np.random.Generator(np.random.Philox(3))
+    p = g.standard_normal(mlp.N_PARAMS) * 0.2
+    X = g.standard_normal((40, mlp.F_IN))
+    T = g.standard_normal(40)
+    grad, _ = mlp.gradient(p, X, T)
+    idx = g.choice(mlp.N_PARAMS, 200, replace=False)
+    idx = np.concatenate([idx, [mlp.N_PARAMS - 1, 0, 12 * 64, 12 * 64 + 64]])
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        fd = (mlp.loss(p + e, X, T) - mlp.loss(p - e, X, T)) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, grad[i])
