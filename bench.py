@@ -13,8 +13,9 @@ configuration are reported alongside.
 
 Roofline: the dominant kernel is the kd traversal.  Its algorithmic bytes
 per launch are Q * (4D query + 8K results + 4 order index) plus the node
-bytes and leaf slots (4 + 4D bytes each) the traversal itself reads, counted
-by the library's visit counters in an untimed run; divided by the kernel's
+bytes (4 per node) and leaf slots (4 + 4D bytes each) that each query's own
+depth-first search reads, counted by the library's visit counters in an
+untimed per-lane run; divided by the wave-cooperative kernel's
 hipEvent-measured average duration.  Peak: 8 TB/s HBM.
 
 CPU baseline: the CPU oracle's kd tree (a port of the reference's KdTree,
@@ -153,14 +154,23 @@ def main() -> int:
     kd_idx = ri.cpu().numpy()
     kd_dist = rd.cpu().numpy()
 
-    # Untimed counting run: algorithmic bytes of the traversal kernel.
-    kd.batchKnn(query, K, count_visits=True, **opts)
+    # Untimed counting runs.  Algorithmic bytes follow the reference's
+    # per-query traversal (per-lane DFS, src/KdTreeGpu.cuh:37-90): the nodes
+    # and leaf slots each query's own search reads, plus query, order index
+    # and result bytes.  The wave-cooperative kernel's issued bytes (each
+    # wave-shared load counted once per wave) are reported beside them.
+    kd.batchKnn(query, K, count_visits=True, per_lane=True, **opts)
     cs = kd.last_query_stats()
-    alg_bytes = cs["node_bytes"] + cs["leaf_slots"] * (4 + 4 * 3) + nq * (4 * 3 + 8 * K + 4)
+    kd.batchKnn(query, K, count_visits=True, **opts)
+    ws = kd.last_query_stats()
+    io_bytes = nq * (4 * 3 + 8 * K + 4)
+    alg_bytes = cs["node_bytes"] + cs["leaf_slots"] * (4 + 4 * 3) + io_bytes
+    issued_bytes = ws["node_bytes"] + ws["leaf_slots"] * (4 + 4 * 3) + io_bytes
     kavg = float(np.mean(kernel_ms))
     achieved = alg_bytes / (kavg * 1e-3) / 1e9
-    log(f"kd visits/query: node bytes {cs['node_bytes'] / nq:.1f}, leaf slots {cs['leaf_slots'] / nq:.1f}; "
-        f"algorithmic {alg_bytes / 1e9:.3f} GB per launch -> {achieved:.1f} GB/s")
+    log(f"kd per-query traversal: node bytes {cs['node_bytes'] / nq:.1f}, leaf slots {cs['leaf_slots'] / nq:.1f}; "
+        f"algorithmic {alg_bytes / 1e9:.3f} GB per launch -> {achieved:.1f} GB/s; wave-issued "
+        f"{issued_bytes / 1e9:.3f} GB")
 
     log("building ball tree")
     ball = knn.BallTreeGpu(p, depth_skips=1, block_size=256)
@@ -236,6 +246,8 @@ def main() -> int:
             "kd_order_ms": round(float(np.mean(order_ms)), 4),
             "kd_node_bytes_per_query": round(cs["node_bytes"] / nq, 2),
             "kd_leaf_slots_per_query": round(cs["leaf_slots"] / nq, 2),
+            "kd_wave_issued_bytes_per_launch": int(issued_bytes),
+            "kd_wave_issued_gbs": round(issued_bytes / (kavg * 1e-3) / 1e9, 1),
             "ball_qps": round(ball_qps, 1),
             "ball_build_ms": round(ball_build, 2),
             "ball_identical_to_kd": ball_same,

@@ -17,22 +17,16 @@ __device__ __forceinline__ float unif_b(float v) {
 }
 
 template <int D, int KIND>
-__global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shrink_a, float shrink_b) {
-    extern __shared__ __align__(16) uint8_t lds_raw[];
-    const uint32_t BS = blockDim.x, tid = threadIdx.x;
-    const uint32_t wave = tid >> 6;
-    const uint32_t nwaves = (BS + 63) >> 6;
-    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth;
-    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, 1));
-
-    const uint32_t i = blockIdx.x * BS + tid;
+__device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_a, float shrink_b, uint32_t i,
+                                                uint32_t* stack, uint64_t* kbuf, uint32_t BS) {
+    const uint32_t tid = threadIdx.x;
     const bool active = i < A.nq;
     const uint32_t qi = active ? (A.perm ? A.perm[i] : i) : 0u;
     QPoint<D> q;
 #pragma unroll
     for (int d = 0; d < D; ++d) q.v[d] = active ? A.qs[(size_t)qi * D + d] : 0.0f;
     typename BufOf<KIND>::T buf;
-    bind_lds(buf, kbuf + tid, BS);
+    bind_lds(buf, kbuf, BS);
     bool tightened = false;
     uint64_t worst = 0;
     if (active) worst = buf.init(start_key(A, qi, tightened), A.K);
@@ -57,6 +51,11 @@ __global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shri
         leaf_range(A, n, pl, land0, landN);
         scan_slots<D>(A.leaves, stride, land0, landN, pl == A.node_depth, q, buf, worst, whi, nslots);
     }
+    // Visit counters: the priming descent is per lane; the shared traversal
+    // issues each node and slot load once per wave, so it is counted once,
+    // on the wave's first active lane.
+    const uint32_t prime_bytes = nbytes, prime_slots = nslots;
+    nbytes = nslots = 0;
     uint32_t node = 0, level = 0, sp = 0;
 
     if (__ballot(active) != 0ull) {
@@ -121,7 +120,41 @@ __global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shri
             if (!found) break;
         }
     }
-    if (active) finish_query(A, qi, buf, worst, tightened, nbytes, nslots);
+    const uint64_t act = __ballot(active);
+    const bool lead = act != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)act) - 1);
+    if (active)
+        finish_query(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
+                     prime_slots + (lead ? nslots : 0u));
+}
+
+template <int D, int KIND>
+__global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shrink_a, float shrink_b) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const uint32_t BS = blockDim.x, tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const uint32_t nwaves = (BS + 63) >> 6;
+    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth;
+    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, 1));
+    ball_packet_run<D, KIND>(A, shrink_a, shrink_b, blockIdx.x * BS + tid, stack, kbuf + tid, BS);
+}
+
+template <int D, int KIND>
+__global__ void __launch_bounds__(256) ball_packet_persistent(const TreeArgs* __restrict__ sets,
+                                                              const uint2* __restrict__ items, uint32_t nitems,
+                                                              uint32_t* __restrict__ counter, uint32_t stack_depth,
+                                                              float shrink_a, float shrink_b) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * stack_depth;
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
+        if (it >= nitems) break;
+        const uint2 w = items[it];
+        const TreeArgs A = sets[w.x];
+        ball_packet_run<D, KIND>(A, shrink_a, shrink_b, w.y + lane, stack, nullptr, 64);
+    }
 }
 
 using BallFn = void (*)(TreeArgs, float, float);

@@ -67,18 +67,24 @@ __global__ void bbox_finish_kernel(const uint32_t* __restrict__ omin, const uint
     }
 }
 
-void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hipStream_t st) {
-    DevBuf<uint32_t> ext;
-    ext.alloc(2 * (size_t)D);
-    launch_fill_u32(ext.get(), D, 0xFFFFFFFFu, st);
-    launch_fill_u32(ext.get() + D, D, 0u, st);
+// ext: 2*D scratch words.  No allocation and no host synchronisation, so it
+// can be captured into a graph.
+void compute_bbox_ws(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, uint32_t* ext, hipStream_t st) {
+    launch_fill_u32(ext, D, 0xFFFFFFFFu, st);
+    launch_fill_u32(ext + D, D, 0u, st);
     if (n) {
         uint32_t gx = nblk(n, 256);
         if (gx > 1024) gx = 1024;
-        hipLaunchKernelGGL(bbox_kernel, dim3(gx, D), dim3(256), 0, st, d_pts, n, D, ext.get(), ext.get() + D);
+        hipLaunchKernelGGL(bbox_kernel, dim3(gx, D), dim3(256), 0, st, d_pts, n, D, ext, ext + D);
     }
-    hipLaunchKernelGGL(bbox_finish_kernel, dim3(nblk(D, 64)), dim3(64), 0, st, ext.get(), ext.get() + D, D, d_bbox);
+    hipLaunchKernelGGL(bbox_finish_kernel, dim3(nblk(D, 64)), dim3(64), 0, st, ext, ext + D, D, d_bbox);
     KNN_CHECK_LAUNCH();
+}
+
+void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hipStream_t st) {
+    DevBuf<uint32_t> ext;
+    ext.alloc(2 * (size_t)D);
+    compute_bbox_ws(d_pts, n, D, d_bbox, ext.get(), st);
     KNN_HIP(hipStreamSynchronize(st));
 }
 
@@ -370,6 +376,10 @@ __global__ void fill_f32_kernel(float* __restrict__ a, size_t n, float v) {
 // ---------------------------------------------------------------------------
 namespace {
 
+}  // namespace
+
+// Build workspace, sized by alloc_levels for the largest tree it serves;
+// builds reuse it in stream order.
 struct LevelBufs {
     DevBuf<uint32_t> P, P2;              // D (or SD) lists of N ids, double-buffered
     DevBuf<uint32_t> segOf, segOf2;      // segment per position
@@ -378,14 +388,16 @@ struct LevelBufs {
     DevBuf<uint32_t> Lcount, leftBefore, segDim;
     DevBuf<uint32_t> cnt, temp;          // partition block counts, scan temp
     DevBuf<uint32_t> keys, keys2, vals2; // presort scratch
+    DevBuf<uint32_t> ext;                // bounding-box scratch
+    DevBuf<float> cent, cent2;           // ball: centroids per segment
+    DevBuf<double> var, var2, partial;   // ball: deviation sums, chunk partials
+    DevBuf<float> pmax;                  // ball: chunk radius maxima
 };
+
+namespace {
 
 // Sorted lists P_d, d < nl, by (ordered coordinate bits, id).
 void presort(LevelBufs& B, const float* pts, uint32_t N, uint32_t D, uint32_t nl, hipStream_t st) {
-    B.keys.alloc(N);
-    B.keys2.alloc(N);
-    B.vals2.alloc(N);
-    B.temp.ensure(radix_temp_words(N));
     for (uint32_t d = 0; d < nl; ++d) {
         uint32_t* vals = B.P.get() + (size_t)d * N;
         hipLaunchKernelGGL(ordered_keys_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, N, D, d, B.keys.get());
@@ -394,34 +406,54 @@ void presort(LevelBufs& B, const float* pts, uint32_t N, uint32_t D, uint32_t nl
         if (alt) KNN_HIP(hipMemcpyAsync(vals, B.vals2.get(), sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
     }
     KNN_CHECK_LAUNCH();
-    B.keys.release();
-    B.keys2.release();
-    B.vals2.release();
 }
 
-void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg) {
-    B.P.alloc((size_t)nl * N);
-    B.P2.alloc((size_t)nl * N);
-    B.segOf.alloc(N);
-    B.segOf2.alloc(N);
-    B.flag.alloc(N);
-    B.start.alloc(maxSeg);
-    B.n.alloc(maxSeg);
-    B.start2.alloc(maxSeg);
-    B.n2.alloc(maxSeg);
-    B.Lcount.alloc(maxSeg);
-    B.leftBefore.alloc(maxSeg);
-    B.segDim.alloc(maxSeg);
+}  // namespace
+
+// Sizes the workspace for trees of up to N points, nl lists (split dims),
+// maxSeg leaf segments and dims D (ball trees also need centroid storage).
+void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32_t D, bool ball) {
+    if (N == 0) N = 1;
+    B.P.ensure((size_t)nl * N);
+    B.P2.ensure((size_t)nl * N);
+    B.segOf.ensure(N);
+    B.segOf2.ensure(N);
+    B.flag.ensure(N);
+    B.start.ensure(maxSeg);
+    B.n.ensure(maxSeg);
+    B.start2.ensure(maxSeg);
+    B.n2.ensure(maxSeg);
+    B.Lcount.ensure(maxSeg);
+    B.leftBefore.ensure(maxSeg);
+    B.segDim.ensure(maxSeg);
     const uint32_t nb = nblk(N, PT_TILE);
-    B.cnt.alloc((size_t)nb * nl);
+    B.cnt.ensure((size_t)nb * nl);
     size_t tw = scan_temp_words((uint32_t)((size_t)nb * nl));
     size_t tw2 = scan_temp_words(maxSeg);
     size_t tw3 = radix_temp_words(N);
-    B.temp.alloc(std::max(tw, std::max(tw2, tw3)));
+    B.temp.ensure(std::max(tw, std::max(tw2, tw3)));
+    B.keys.ensure(N);
+    B.keys2.ensure(N);
+    B.vals2.ensure(N);
+    B.ext.ensure(2 * (size_t)D);
+    if (ball) {
+        B.cent.ensure((size_t)maxSeg * D);
+        B.cent2.ensure((size_t)maxSeg * D);
+        B.var.ensure((size_t)maxSeg * D);
+        B.var2.ensure((size_t)maxSeg * D);
+        // seg_reduce partials: at most N / 4096 + S chunks per level
+        const size_t items = (size_t)maxSeg + N / 4096 + 2;
+        B.partial.ensure(items * 4);
+        B.pmax.ensure(items);
+    }
 }
 
-// Partition every list except `skip` by the point flags.
-void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint32_t S, hipStream_t st) {
+namespace {
+
+// Partition every list except `skip` by the point flags (segment tables of
+// the current level: start, n, segOf).
+void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint32_t S, const uint32_t* start,
+                     const uint32_t* n, const uint32_t* segOf, hipStream_t st) {
     const uint32_t nlists = (skip < nl) ? nl - 1 : nl;
     if (nlists == 0) return;
     const uint32_t nb = nblk(N, PT_TILE);
@@ -429,8 +461,7 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
                        B.flag.get(), B.cnt.get(), nb);
     exclusive_scan_u32(B.cnt.get(), B.cnt.get(), nb * nlists, B.temp.get(), st);
     hipLaunchKernelGGL(part_scatter_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), B.P2.get(), N, skip,
-                       B.flag.get(), B.cnt.get(), nb, B.segOf.get(), B.start.get(), B.n.get(), B.leftBefore.get(),
-                       B.Lcount.get(), S);
+                       B.flag.get(), B.cnt.get(), nb, segOf, start, n, B.leftBefore.get(), B.Lcount.get(), S);
     // The partitioned lists live in P2; the skipped list (already in order)
     // is carried over so that P2 is complete before the swap.
     if (skip < nl)
@@ -439,11 +470,11 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
     std::swap(B.P.p, B.P2.p);
 }
 
-void level_common(LevelBufs& B, uint32_t N, uint32_t S, uint32_t uniformDim, bool perSegDim, hipStream_t st) {
+void level_common(LevelBufs& B, uint32_t N, uint32_t S, uint32_t uniformDim, bool perSegDim, const uint32_t* start,
+                  const uint32_t* n, const uint32_t* segOf, uint32_t* segOf2, hipStream_t st) {
     exclusive_scan_u32(B.Lcount.get(), B.leftBefore.get(), S, B.temp.get(), st);
     hipLaunchKernelGGL(flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, B.P.get(), N, uniformDim,
-                       perSegDim ? B.segDim.get() : nullptr, B.segOf.get(), B.start.get(), B.n.get(), B.flag.get(),
-                       B.segOf2.get());
+                       perSegDim ? B.segDim.get() : nullptr, segOf, start, n, B.flag.get(), segOf2);
 }
 
 struct Timer {
@@ -506,111 +537,120 @@ void seg_reduce(const uint32_t* P0, const float* pts, uint32_t N, uint32_t D, co
 
 }  // namespace
 
-void build_kd(Handle& h, const float* pts, hipStream_t st, uint32_t block) {
-    (void)block;
-    knn_tree_info& I = h.info;
-    const uint32_t N = I.n, D = I.dims, SD = I.split_dims;
-    h.nodes.alloc(I.node_size);
+LevelBufs* new_level_bufs() { return new LevelBufs(); }
+void delete_level_bufs(LevelBufs* b) { delete b; }
+
+// Allocates a tree's output buffers (outside any graph capture).
+void alloc_tree(Handle& h) {
+    const knn_tree_info& I = h.info;
+    const uint32_t D = I.dims;
+    if (h.kind == KNN_KIND_KD) {
+        h.nodes.alloc(I.node_size);
+    } else {
+        h.nodes.alloc((size_t)I.node_size * (2 * D + 2));
+        h.dims.alloc(I.node_size);
+    }
     h.leaves.alloc((size_t)I.index_size * (1 + D));
     h.bbox.alloc(2 * (size_t)D);
-    Timer timer(st);
+}
+
+// kd build on a prepared workspace: kernels and device copies only.
+void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
+    knn_tree_info& I = h.info;
+    const uint32_t N = I.n, D = I.dims, SD = I.split_dims;
     if (N == 0) {
         hipLaunchKernelGGL(fill_f32_kernel, dim3(nblk(I.node_size, 256)), dim3(256), 0, st, h.nodes.get(),
                            (size_t)I.node_size, 0.0f);
         fill_empty_leaves(h, st);
-        KNN_CHECK_LAUNCH();
-        h.last_build_ms = timer.stop();
-        compute_bbox(pts, 0, D, h.bbox.get(), st);
+        compute_bbox_ws(pts, 0, D, h.bbox.get(), B.ext.get(), st);
         return;
     }
-    LevelBufs B;
-    const uint32_t maxSeg = 1u << I.node_depth;
-    alloc_levels(B, N, SD, maxSeg);
     presort(B, pts, N, D, SD, st);
     launch_fill_u32(B.segOf.get(), N, 0u, st);
     launch_fill_u32(B.start.get(), 1, 0u, st);
     launch_fill_u32(B.n.get(), 1, N, st);
+    uint32_t *start = B.start.get(), *n = B.n.get(), *start2 = B.start2.get(), *n2 = B.n2.get();
+    uint32_t *segOf = B.segOf.get(), *segOf2 = B.segOf2.get();
     for (uint32_t depth = 0; depth < I.node_depth; ++depth) {
         const uint32_t S = 1u << depth;
         const uint32_t dim = depth % SD;
         hipLaunchKernelGGL(kd_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, pts, D, B.P.get() + (size_t)dim * N,
-                           dim, B.start.get(), B.n.get(), S, h.nodes.get(), B.start2.get(), B.n2.get(),
-                           B.Lcount.get());
-        level_common(B, N, S, dim, false, st);
-        partition_lists(B, N, SD, dim, S, st);
-        std::swap(B.segOf.p, B.segOf2.p);
-        std::swap(B.start.p, B.start2.p);
-        std::swap(B.n.p, B.n2.p);
+                           dim, start, n, S, h.nodes.get(), start2, n2, B.Lcount.get());
+        level_common(B, N, S, dim, false, start, n, segOf, segOf2, st);
+        partition_lists(B, N, SD, dim, S, start, n, segOf, st);
+        std::swap(segOf, segOf2);
+        std::swap(start, start2);
+        std::swap(n, n2);
     }
     const uint32_t lastDim = (I.node_depth - 1) % SD;
     hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N,
-                       lastDim, nullptr, I.node_size, B.start.get(), B.n.get(), I.depth_skips, I.index_size,
-                       h.leaves.get());
+                       lastDim, nullptr, I.node_size, start, n, I.depth_skips, I.index_size, h.leaves.get());
     KNN_CHECK_LAUNCH();
-    h.last_build_ms = timer.stop();
-    compute_bbox(pts, N, D, h.bbox.get(), st);
+    compute_bbox_ws(pts, N, D, h.bbox.get(), B.ext.get(), st);
 }
 
-void build_ball(Handle& h, const float* pts, hipStream_t st, uint32_t block) {
+void build_kd(Handle& h, const float* pts, hipStream_t st, uint32_t block) {
     (void)block;
+    alloc_tree(h);
+    LevelBufs B;
+    alloc_levels(B, h.info.n, h.info.split_dims, 1u << h.info.node_depth, h.info.dims, false);
+    Timer timer(st);
+    build_kd_ws(h, pts, st, B);
+    h.last_build_ms = timer.stop();
+}
+
+void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     knn_tree_info& I = h.info;
     const uint32_t N = I.n, D = I.dims;
-    h.nodes.alloc((size_t)I.node_size * (2 * D + 2));
-    h.dims.alloc(I.node_size);
-    h.leaves.alloc((size_t)I.index_size * (1 + D));
-    h.bbox.alloc(2 * (size_t)D);
-    Timer timer(st);
     if (N == 0) {
         hipLaunchKernelGGL(fill_f32_kernel, dim3(nblk((size_t)I.node_size * (2 * D + 2), 256)), dim3(256), 0, st,
                            h.nodes.get(), (size_t)I.node_size * (2 * D + 2), 0.0f);
         launch_fill_u32(h.dims.get(), I.node_size, 0u, st);
         fill_empty_leaves(h, st);
-        KNN_CHECK_LAUNCH();
-        h.last_build_ms = timer.stop();
-        compute_bbox(pts, 0, D, h.bbox.get(), st);
+        compute_bbox_ws(pts, 0, D, h.bbox.get(), B.ext.get(), st);
         return;
     }
-    LevelBufs B;
-    const uint32_t maxSeg = 1u << I.node_depth;
-    alloc_levels(B, N, D, maxSeg);
     presort(B, pts, N, D, D, st);
-    DevBuf<float> cent, cent2;
-    DevBuf<double> var, var2, partial;
-    DevBuf<float> pmax;
-    cent.alloc((size_t)maxSeg * D);
-    cent2.alloc((size_t)maxSeg * D);
-    var.alloc((size_t)maxSeg * D);
-    var2.alloc((size_t)maxSeg * D);
     launch_fill_u32(B.segOf.get(), N, 0u, st);
     launch_fill_u32(B.start.get(), 1, 0u, st);
     launch_fill_u32(B.n.get(), 1, N, st);
+    uint32_t *start = B.start.get(), *n = B.n.get(), *start2 = B.start2.get(), *n2 = B.n2.get();
+    uint32_t *segOf = B.segOf.get(), *segOf2 = B.segOf2.get();
+    float *cent = B.cent.get(), *cent2 = B.cent2.get();
+    double *var = B.var.get(), *var2 = B.var2.get();
     // Root centroid and deviation sums (no BallNode entry for the root).
-    seg_reduce(B.P.get(), pts, N, D, B.start.get(), B.n.get(), 1, 0, cent.get(), nullptr, nullptr, partial, pmax, st);
-    seg_reduce(B.P.get(), pts, N, D, B.start.get(), B.n.get(), 1, 1, cent.get(), var.get(), nullptr, partial, pmax, st);
+    seg_reduce(B.P.get(), pts, N, D, start, n, 1, 0, cent, nullptr, nullptr, B.partial, B.pmax, st);
+    seg_reduce(B.P.get(), pts, N, D, start, n, 1, 1, cent, var, nullptr, B.partial, B.pmax, st);
     for (uint32_t depth = 0; depth < I.node_depth; ++depth) {
         const uint32_t S = 1u << depth;
-        hipLaunchKernelGGL(ball_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, var.get(), D, B.start.get(),
-                           B.n.get(), S, h.dims.get(), B.segDim.get(), B.start2.get(), B.n2.get(), B.Lcount.get());
-        level_common(B, N, S, 0, true, st);
-        partition_lists(B, N, D, 0xFFFFFFFFu, S, st);
-        std::swap(B.segOf.p, B.segOf2.p);
-        std::swap(B.start.p, B.start2.p);
-        std::swap(B.n.p, B.n2.p);
+        hipLaunchKernelGGL(ball_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, var, D, start, n, S, h.dims.get(),
+                           B.segDim.get(), start2, n2, B.Lcount.get());
+        level_common(B, N, S, 0, true, start, n, segOf, segOf2, st);
+        partition_lists(B, N, D, 0xFFFFFFFFu, S, start, n, segOf, st);
+        std::swap(segOf, segOf2);
+        std::swap(start, start2);
+        std::swap(n, n2);
         // Children (level depth + 1): centroids, then deviation sums + radii.
         const uint32_t S2 = 2 * S;
-        seg_reduce(B.P.get(), pts, N, D, B.start.get(), B.n.get(), S2, 0, cent2.get(), nullptr, h.nodes.get(), partial,
-                   pmax, st);
-        seg_reduce(B.P.get(), pts, N, D, B.start.get(), B.n.get(), S2, 1, cent2.get(), var2.get(), h.nodes.get(),
-                   partial, pmax, st);
-        std::swap(cent.p, cent2.p);
-        std::swap(var.p, var2.p);
+        seg_reduce(B.P.get(), pts, N, D, start, n, S2, 0, cent2, nullptr, h.nodes.get(), B.partial, B.pmax, st);
+        seg_reduce(B.P.get(), pts, N, D, start, n, S2, 1, cent2, var2, h.nodes.get(), B.partial, B.pmax, st);
+        std::swap(cent, cent2);
+        std::swap(var, var2);
     }
     hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N, 0u,
-                       h.dims.get(), I.node_size, B.start.get(), B.n.get(), I.depth_skips, I.index_size,
-                       h.leaves.get());
+                       h.dims.get(), I.node_size, start, n, I.depth_skips, I.index_size, h.leaves.get());
     KNN_CHECK_LAUNCH();
+    compute_bbox_ws(pts, N, D, h.bbox.get(), B.ext.get(), st);
+}
+
+void build_ball(Handle& h, const float* pts, hipStream_t st, uint32_t block) {
+    (void)block;
+    alloc_tree(h);
+    LevelBufs B;
+    alloc_levels(B, h.info.n, h.info.dims, 1u << h.info.node_depth, h.info.dims, true);
+    Timer timer(st);
+    build_ball_ws(h, pts, st, B);
     h.last_build_ms = timer.stop();
-    compute_bbox(pts, N, D, h.bbox.get(), st);
 }
 
 }  // namespace knn

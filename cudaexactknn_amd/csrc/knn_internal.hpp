@@ -175,9 +175,18 @@ struct EventTimer {
     }
 };
 
-// Builders (knn_build.hip)
+// Builders (knn_build.hip).  build_* allocate and time; alloc_tree +
+// build_*_ws only launch kernels and device copies on a prepared workspace
+// (graph-capture safe).
+struct LevelBufs;
 void build_kd(Handle& h, const float* d_pts, hipStream_t st, uint32_t block);
 void build_ball(Handle& h, const float* d_pts, hipStream_t st, uint32_t block);
+void alloc_tree(Handle& h);
+void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32_t D, bool ball);
+void build_kd_ws(Handle& h, const float* d_pts, hipStream_t st, LevelBufs& B);
+void build_ball_ws(Handle& h, const float* d_pts, hipStream_t st, LevelBufs& B);
+LevelBufs* new_level_bufs();
+void delete_level_bufs(LevelBufs* b);
 void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hipStream_t st);
 
 // Queries

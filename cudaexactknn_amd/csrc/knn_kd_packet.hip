@@ -44,24 +44,21 @@ __device__ __forceinline__ float box_bound(const float (&q)[D], const float (&lo
     return r;
 }
 
+// One wave's traversal for the ordered queries i = (this lane's index).
+// `stack` is this wave's LDS stack, `kbuf` this lane's LDS K-buffer slot
+// (stride BS), used only by LDS buffer kinds.
 template <int D, int KIND>
-__global__ void __launch_bounds__(256) kd_packet_kernel(TreeArgs A) {
+__device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uint32_t* stack, uint64_t* kbuf,
+                                              uint32_t BS) {
     constexpr int ENT = 1 + 2 * D;  // words per stack entry: node, lo[D], hi[D]
-    extern __shared__ __align__(16) uint8_t lds_raw[];
-    const uint32_t BS = blockDim.x, tid = threadIdx.x;
-    const uint32_t wave = tid >> 6;
-    const uint32_t nwaves = (BS + 63) >> 6;
-    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth * ENT;
-    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, ENT));
-
-    const uint32_t i = blockIdx.x * BS + tid;
+    const uint32_t tid = threadIdx.x;
     const bool active = i < A.nq;
     const uint32_t qi = active ? (A.perm ? A.perm[i] : i) : 0u;
     float q[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) q[d] = active ? A.qs[(size_t)qi * D + d] : 0.0f;
     typename BufOf<KIND>::T buf;
-    bind_lds(buf, kbuf + tid, BS);
+    bind_lds(buf, kbuf, BS);
     bool tightened = false;
     uint64_t worst = 0;
     if (active) worst = buf.init(start_key(A, qi, tightened), A.K);
@@ -101,6 +98,11 @@ __global__ void __launch_bounds__(256) kd_packet_kernel(TreeArgs A) {
         scan_slots<D>(A.leaves, stride, land0, landN, pl == A.node_depth, qp, buf, worst, whi, nslots);
     }
 
+    // Visit counters: the priming descent is per lane; the shared traversal
+    // issues each node and slot load once per wave, so it is counted once,
+    // on the wave's first active lane.
+    const uint32_t prime_bytes = nbytes, prime_slots = nslots;
+    nbytes = nslots = 0;
     uint32_t node = 0, level = 0, dim = 0, sp = 0;
     float cur = 0.0f;  // this lane's bound for the current node's box
 
@@ -208,7 +210,45 @@ __global__ void __launch_bounds__(256) kd_packet_kernel(TreeArgs A) {
             if (!found) break;
         }
     }
-    if (active) finish_query(A, qi, buf, worst, tightened, nbytes, nslots);
+    const uint64_t act = __ballot(active);
+    const bool lead = act != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)act) - 1);
+    if (active)
+        finish_query(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
+                     prime_slots + (lead ? nslots : 0u));
+}
+
+template <int D, int KIND>
+__global__ void __launch_bounds__(256) kd_packet_kernel(TreeArgs A) {
+    constexpr int ENT = 1 + 2 * D;
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const uint32_t BS = blockDim.x, tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const uint32_t nwaves = (BS + 63) >> 6;
+    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth * ENT;
+    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, ENT));
+    kd_packet_run<D, KIND>(A, blockIdx.x * BS + tid, stack, kbuf + tid, BS);
+}
+
+// Persistent form: waves pull (dataset, first ordered query) items from a
+// device work queue until it is empty; every wave exits once the counter
+// passes the item count.  Register K-buffers only (no LDS K-buffer).
+template <int D, int KIND>
+__global__ void __launch_bounds__(256) kd_packet_persistent(const TreeArgs* __restrict__ sets,
+                                                            const uint2* __restrict__ items, uint32_t nitems,
+                                                            uint32_t* __restrict__ counter, uint32_t stack_depth) {
+    constexpr int ENT = 1 + 2 * D;
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * stack_depth * ENT;
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
+        if (it >= nitems) break;
+        const uint2 w = items[it];
+        const TreeArgs A = sets[w.x];
+        kd_packet_run<D, KIND>(A, w.y + lane, stack, nullptr, 64);
+    }
 }
 
 using TreeFn = void (*)(TreeArgs);
