@@ -103,6 +103,8 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--leaf-span", type=int, default=0)
     ap.add_argument("--stack-lds", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -110,6 +112,16 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.calibrate_copy:
+        # A plain copy kernel of known size: reads and writes 1 GiB each.
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 3 x 1 GiB")
+        return 0
 
     import cudaexactknn_amd as knn
     from cudaexactknn_amd import datasets
@@ -235,7 +247,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles("kd_query_kernel"),
+            "traffic": traffic_from_profiles("kd_packet_kernel"),
             "kernel": "kd_packet_kernel<3, REG16> (wave-cooperative traversal)",
             "kernel_ms": round(kavg, 4),
             "alg_bytes_per_launch": int(alg_bytes),

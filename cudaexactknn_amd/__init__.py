@@ -27,7 +27,8 @@ from ._lib import KnnError, QueryOpts, TreeInfo  # noqa: F401
 
 __all__ = [
     "BruteForceGpu", "KdTreeGpu", "BallTreeGpu", "Query", "KnnError", "pinned_malloc", "pinned_free",
-    "FORM_PLAIN", "FORM_BF16", "FORM_LEARNED", "library_path", "version",
+    "FORM_PLAIN", "FORM_BF16", "FORM_LEARNED", "library_path", "version", "Batch", "BatchItem",
+    "KIND_BRUTE", "KIND_KD", "KIND_BALL",
 ]
 
 FORM_PLAIN, FORM_BF16, FORM_LEARNED = _lib.KNN_FORM_PLAIN, _lib.KNN_FORM_BF16, _lib.KNN_FORM_LEARNED
@@ -134,7 +135,7 @@ class _Structure:
 
 
 def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, leaf_span=0, stream=None,
-          count_visits=False, per_lane=False, morton=False):
+          count_visits=False, per_lane=False, morton=False, xcd_remap=True, tile=0):
     o = QueryOpts()
     o.block = int(BLOCK_SIZE or 0)
     o.heap = int(bool(heap))
@@ -142,8 +143,10 @@ def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, le
     o.stack_lds = int(bool(stack))
     o.form = int(form)
     o.flags = ((0 if sort_queries else _lib.KNN_FLAG_NO_QUERY_SORT) | (_lib.KNN_FLAG_COUNT_VISITS if count_visits else 0)
-               | (_lib.KNN_FLAG_PER_LANE if per_lane else 0) | (_lib.KNN_FLAG_MORTON_ORDER if morton else 0))
+               | (_lib.KNN_FLAG_PER_LANE if per_lane else 0) | (_lib.KNN_FLAG_MORTON_ORDER if morton else 0)
+               | (0 if xcd_remap else _lib.KNN_FLAG_NO_XCD_REMAP))
     o.leaf_span = int(leaf_span)
+    o.tile = int(tile)
     o.stream = stream
     return o
 
@@ -160,22 +163,22 @@ class BruteForceGpu(_Structure):
         _lib.check(_lib.load().knn_bf_create(_ptr(points), n, d, ctypes.byref(self._h)))
 
     def batchKnn(self, query: Query, K: int, BLOCK_SIZE: int = 0, USE_HEAP_TRACKING: bool = False,
-                 USE_TRACKING_SHMEM: bool = False, form: int = FORM_PLAIN, stream=None) -> None:
+                 USE_TRACKING_SHMEM: bool = False, form: int = FORM_PLAIN, stream=None, tile: int = 0) -> None:
         """batchKnn<K, BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM>(Query&)
-        (src/BruteForceGpu.cuh:74-83)."""
+        (src/BruteForceGpu.cuh:74-83).  `tile`: bf16 form, point tiles staged per barrier."""
         self._query(query, K, _opts(BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, False, form,
-                                    stream=stream))
+                                    stream=stream, tile=tile))
 
 
 class _Tree(_Structure):
     def batchKnn(self, query: Query, K: int, BLOCK_SIZE: int = 0, USE_HEAP_TRACKING: bool = False,
                  USE_TRACKING_SHMEM: bool = False, USE_STACK_SHMEM: bool = False, form: int = FORM_PLAIN,
                  sort_queries: bool = True, leaf_span: int = 0, stream=None, count_visits: bool = False,
-                 per_lane: bool = False, morton: bool = False) -> None:
+                 per_lane: bool = False, morton: bool = False, xcd_remap: bool = True) -> None:
         """batchKnn<K, BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM>(Query&)
         (src/KdTreeGpu.cuh:393-410, src/BallTreeGpu.cuh:631-648)."""
         self._query(query, K, _opts(BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM, form,
-                                    sort_queries, leaf_span, stream, count_visits, per_lane, morton))
+                                    sort_queries, leaf_span, stream, count_visits, per_lane, morton, xcd_remap))
 
     def build_ms(self) -> float:
         return float(_lib.load().knn_last_build_ms(self._h))
@@ -239,3 +242,77 @@ def pinned_malloc(nbytes: int) -> int:
 def pinned_free(ptr: int) -> None:
     """pinnedFree (src/KnnTrees.cuh:240-243)."""
     _lib.check(_lib.load().knn_pinned_free(ctypes.c_void_p(ptr)))
+
+
+KIND_BRUTE, KIND_KD, KIND_BALL = _lib.KNN_KIND_BRUTE, _lib.KNN_KIND_KD, _lib.KNN_KIND_BALL
+
+
+@dataclass
+class BatchItem:
+    """One dataset of a Batch: `kind` KIND_KD / KIND_BALL (dims 2, 3, 4, 8) or
+    KIND_BRUTE (bf16 form, dims <= 256), K <= 32.  Tensors stay referenced
+    by the item: every replay reads points and queries and writes results."""
+    kind: int
+    points: Any
+    queries: Any
+    k: int
+    r_indexes: Any
+    r_distances: Any
+    max_distance2: float = float("inf")
+    split_dimensions: int = 0
+    depth_skips: int = 0
+
+
+class Batch:
+    """Many independent datasets built and queried by two captured HIP graphs
+    (knn_batch_* in include/knn_mi355x.h); the queries run as one persistent
+    launch over (dataset, query block) work items."""
+
+    STAT_NAMES = ("build_ms", "query_ms", "overflow_queries", "work_items", "workgroups", "build_graph_nodes",
+                  "query_graph_nodes", "queries")
+
+    def __init__(self, items: list, build_streams: int = 0, blocks_per_cu: int = 0, tile: int = 0,
+                 leaf_span: int = 0):
+        self._h = ctypes.c_void_p(0)
+        self.items = list(items)
+        arr = (_lib.BatchItem * max(1, len(self.items)))()
+        for j, it in enumerate(self.items):
+            n, d = _shape2(it.points, "points")
+            nq, dq = _shape2(it.queries, "queries")
+            if dq != d:
+                raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"batch item {j}: queries have {dq} dims, points {d}")
+            a = arr[j]
+            a.kind, a.dims, a.n, a.nq, a.k = int(it.kind), d, n, nq, int(it.k)
+            a.split_dims, a.depth_skips, a.max_d2 = int(it.split_dimensions), int(it.depth_skips), float(it.max_distance2)
+            a.d_pts, a.d_q = _ptr(it.points), _ptr(it.queries)
+            a.d_idx, a.d_dist = _ptr(it.r_indexes), _ptr(it.r_distances)
+        o = _lib.BatchOpts(int(build_streams), int(blocks_per_cu), int(tile), int(leaf_span))
+        _lib.check(_lib.load().knn_batch_create(arr, len(self.items), ctypes.byref(o), ctypes.byref(self._h)))
+
+    def run(self, build: bool = True, query: bool = True, stream=None) -> None:
+        """Replay the build graph, the query graph, or both (synchronous when
+        stream is None)."""
+        what = (1 if build else 0) | (2 if query else 0)
+        _lib.check(_lib.load().knn_batch_run(self._h, what, stream))
+
+    def stats(self) -> dict:
+        out = (ctypes.c_float * 8)()
+        _lib.check(_lib.load().knn_batch_stats(self._h, out))
+        return dict(zip(self.STAT_NAMES, list(out)))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.check(_lib.load().knn_batch_destroy(self._h))
+            self._h = ctypes.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

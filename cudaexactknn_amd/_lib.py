@@ -24,13 +24,14 @@ KNN_FLAG_NO_QUERY_SORT = 0x1
 KNN_FLAG_COUNT_VISITS = 0x2
 KNN_FLAG_PER_LANE = 0x4
 KNN_FLAG_MORTON_ORDER = 0x8
+KNN_FLAG_NO_XCD_REMAP = 0x10
 
 # Every symbol include/knn_mi355x.h declares.
 EXPORTED = (
     "knn_version", "knn_bf_create", "knn_kd_build", "knn_ball_build", "knn_query", "knn_destroy",
     "knn_last_error", "knn_pinned_alloc", "knn_pinned_free", "knn_tree_get_info", "knn_tree_export",
     "knn_last_build_ms", "knn_learned_train", "knn_last_query_stats", "knn_mlp_param_count", "knn_mlp_forward",
-    "knn_mlp_gradient",
+    "knn_mlp_gradient", "knn_batch_create", "knn_batch_run", "knn_batch_stats", "knn_batch_destroy",
 )
 
 
@@ -38,8 +39,21 @@ class QueryOpts(ctypes.Structure):
     _fields_ = [
         ("block", ctypes.c_uint32), ("heap", ctypes.c_uint32), ("track_lds", ctypes.c_uint32),
         ("stack_lds", ctypes.c_uint32), ("form", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-        ("leaf_span", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("stream", ctypes.c_void_p),
+        ("leaf_span", ctypes.c_uint32), ("tile", ctypes.c_uint32), ("stream", ctypes.c_void_p),
     ]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_uint32), ("dims", ctypes.c_uint32), ("n", ctypes.c_uint32), ("nq", ctypes.c_uint32),
+        ("k", ctypes.c_uint32), ("split_dims", ctypes.c_uint32), ("depth_skips", ctypes.c_uint32),
+        ("max_d2", ctypes.c_float), ("d_pts", ctypes.c_void_p), ("d_q", ctypes.c_void_p),
+        ("d_idx", ctypes.c_void_p), ("d_dist", ctypes.c_void_p),
+    ]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("build_streams", "blocks_per_cu", "tile", "leaf_span")]
 
 
 class TreeInfo(ctypes.Structure):
@@ -93,10 +107,14 @@ def load() -> ctypes.CDLL:
     L.knn_mlp_param_count.argtypes = [ctypes.POINTER(u32)]
     L.knn_mlp_forward.argtypes = [P, P, u32, u32, P]
     L.knn_mlp_gradient.argtypes = [P, P, P, u32, P, ctypes.POINTER(f32)]
+    L.knn_batch_create.argtypes = [ctypes.POINTER(BatchItem), u32, ctypes.POINTER(BatchOpts), ctypes.POINTER(P)]
+    L.knn_batch_run.argtypes = [P, u32, P]
+    L.knn_batch_stats.argtypes = [P, P]
+    L.knn_batch_destroy.argtypes = [P]
     for name in ("knn_bf_create", "knn_kd_build", "knn_ball_build", "knn_query", "knn_destroy",
                  "knn_pinned_alloc", "knn_pinned_free", "knn_tree_get_info", "knn_tree_export",
                  "knn_learned_train", "knn_last_query_stats", "knn_mlp_param_count", "knn_mlp_forward",
-                 "knn_mlp_gradient"):
+                 "knn_mlp_gradient", "knn_batch_create", "knn_batch_run", "knn_batch_stats", "knn_batch_destroy"):
         getattr(L, name).restype = st
     _LIB = L
     return L

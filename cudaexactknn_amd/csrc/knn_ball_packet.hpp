@@ -1,0 +1,130 @@
+// Ball-tree batch query, wave-cooperative form (see knn_kd_packet.hip for the
+// scheme).  At each node every lane computes its certified lower bounds for
+// both child balls (the same float sequence as the per-lane kernel and the
+// oracle); a child is visited iff some lane's bound passes its own K-th key
+// test.  The wave descends first into the child more lanes prefer (smaller
+// unclamped border distance, the reference's order, src/BallTreeGpu.cuh:70);
+// the other child goes on a per-wave LDS stack as a bare node index, and its
+// per-lane bound is recomputed from the parent's BallNode at pop time.
+#pragma once
+#include "knn_tree_common.hpp"
+
+namespace knn {
+
+__device__ __forceinline__ uint32_t uni_b(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif_b(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+template <int D, int KIND>
+__device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_a, float shrink_b, uint32_t i,
+                                                uint32_t* stack, uint64_t* kbuf, uint32_t BS) {
+    const uint32_t tid = threadIdx.x;
+    const bool active = i < A.nq;
+    const uint32_t qi = active ? (A.perm ? A.perm[i] : i) : 0u;
+    QPoint<D> q;
+#pragma unroll
+    for (int d = 0; d < D; ++d) q.v[d] = active ? A.qs[(size_t)qi * D + d] : 0.0f;
+    typename BufOf<KIND>::T buf;
+    bind_lds(buf, kbuf, BS);
+    bool tightened = false;
+    uint64_t worst = 0;
+    if (active) worst = buf.init(start_key(A, qi, tightened), A.K);
+    uint32_t whi = active ? key_hi(worst) : 0u;
+    uint32_t nbytes = 0, nslots = 0;
+    constexpr uint32_t nstride = 2 * D + 2;
+    const uint32_t stride = 1 + D;
+    const uint32_t stop = A.stop_level;
+    const bool single_bucket = stop == A.node_depth;
+    // Priming: each lane descends to its own landing bucket (nearer centroid
+    // at each level) and scans it; the shared traversal then skips it.
+    uint32_t land0 = 0xFFFFFFFFu, landN = 0;
+    if (active) {
+        const uint32_t pl = A.prime_level;
+        uint32_t n = 0;
+        for (uint32_t l = 0; l < pl; ++l) {
+            const float* nd = A.nodes + (size_t)n * nstride;
+            const float dl = q.dist_to(nd), dr = q.dist_to(nd + D + 1);
+            n = 2 * n + 1 + (dl <= dr ? 0u : 1u);
+        }
+        nbytes += 8u * (D + 1) * pl;
+        leaf_range(A, n, pl, land0, landN);
+        scan_slots<D>(A.leaves, stride, land0, landN, pl == A.node_depth, q, buf, worst, whi, nslots);
+    }
+    // Visit counters: the priming descent is per lane; the shared traversal
+    // issues each node and slot load once per wave, so it is counted once,
+    // on the wave's first active lane.
+    const uint32_t prime_bytes = nbytes, prime_slots = nslots;
+    nbytes = nslots = 0;
+    uint32_t node = 0, level = 0, sp = 0;
+
+    if (__ballot(active) != 0ull) {
+        for (;;) {
+            if (level < stop) {
+                const float* nd = A.nodes + (size_t)node * nstride;
+                float cl[D], cr[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    cl[d] = unif_b(nd[d]);
+                    cr[d] = unif_b(nd[D + 1 + d]);
+                }
+                const float rl = unif_b(nd[D]), rr = unif_b(nd[2 * D + 1]);
+                nbytes += 8u * (D + 1);
+                float lbd, rbd;
+                const float lb = ball_bound(q, cl, rl, shrink_a, shrink_b, lbd);
+                const float rb = ball_bound(q, cr, rr, shrink_a, shrink_b, rbd);
+                const uint64_t needL = __ballot(active && f2u(lb) <= whi);
+                const uint64_t needR = __ballot(active && f2u(rb) <= whi);
+                const uint32_t l = 2 * node + 1;
+                if (needL && needR) {
+                    const uint32_t nleft = __popcll(__ballot(active && lbd < rbd));
+                    const uint32_t nact = __popcll(__ballot(active));
+                    const bool goLeft = 2 * nleft >= nact;
+                    if ((tid & 63) == 0) stack[sp] = goLeft ? l + 1 : l;
+                    __builtin_amdgcn_wave_barrier();
+                    ++sp;
+                    node = goLeft ? l : l + 1;
+                    ++level;
+                    continue;
+                }
+                if (needL || needR) {
+                    node = needL ? l : l + 1;
+                    ++level;
+                    continue;
+                }
+            } else {
+                uint32_t slot0, count;
+                leaf_range(A, node, level, slot0, count);
+                packet_scan<D>(A.leaves, uni_b(slot0), uni_b(count), single_bucket, q, active, land0, landN, buf,
+                               worst, whi, nslots);
+            }
+            bool found = false;
+            while (sp > 0) {
+                --sp;
+                const uint32_t n2 = uni_b(stack[sp]);
+                const uint32_t parent = (n2 - 1) / 2;
+                const float* nd = A.nodes + (size_t)parent * nstride + ((n2 & 1u) ? 0 : D + 1);
+                float c[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) c[d] = unif_b(nd[d]);
+                const float r = unif_b(nd[D]);
+                nbytes += 4u * (D + 1);
+                float bd;
+                const float b = ball_bound(q, c, r, shrink_a, shrink_b, bd);
+                if (__ballot(active && f2u(b) <= whi) == 0ull) continue;
+                node = n2;
+                level = 31u - __clz(node + 1);
+                found = true;
+                break;
+            }
+            if (!found) break;
+        }
+    }
+    const uint64_t act = __ballot(active);
+    const bool lead = act != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)act) - 1);
+    if (active)
+        finish_query(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
+                     prime_slots + (lead ? nslots : 0u));
+}
+
+}  // namespace knn

@@ -1,52 +1,18 @@
-// BF16-MFMA filtered brute force (the "bf16" form of BruteForceGpu::batchKnn).
-// Results are identical to the plain FP32 path: the MFMA pass only selects
-// candidates, with a certified error bound, and every candidate is reranked
-// with the reference's exact FP32 arithmetic (src/KnnTrees.cuh:65-77).
-//
-// 1. Each coordinate x is split into two bf16 parts, hi = bf16(x) and
-//    lo = bf16(x - hi) (x - hi is exact in FP32).  q.p is approximated by
-//    s = q_hi.p_hi + q_hi.p_lo + q_lo.p_hi, three v_mfma_f32_32x32x16_bf16
-//    products accumulated in FP32, and A = (|q|^2 + |p|^2) - 2 s with FP32
-//    norms.
-// 2. Forward error analysis (Higham, Accuracy and Stability of Numerical
-//    Algorithms, ch. 3) bounds |A - d2| by E = cA (|q|^2 + |p|^2) + tiny, with
-//    cA covering the dropped lo.lo and rounding terms (3.2 u_b^2, u_b = 2^-8),
-//    the FP32 accumulation of 3*Dpad products (gamma_{3 Dpad}) and the norm
-//    errors (gamma_{D+1}), plus 25 % slack; and the reference's computed
-//    distance f differs from d2 by at most gamma_{D+2} d2.  So
-//    L = (A - E) dn <= f <= (A + E) up = U.
-// 3. Per query, the lanes keep the K smallest U seen (an upper bound tau of
-//    the K-th smallest f) and emit every point with L <= tau (and <= maxD2
-//    when that is a non-negative number).  The true top-K always qualifies,
-//    whatever the processing order, because L <= f <= f_K <= tau.
-// 4. Rerank: exact FP32 distance and the usual key insertion over the
-//    candidates only.  A query whose candidates overflow the buffer is
-//    recomputed by the plain kernel.
-//
-// MFMA layout (32x32x16, wave64): A = 32 points x 16 dims, lane l holds point
-// l%32, dims 8*(l/32)..+7; B = 16 dims x 32 queries, lane l holds query l%32,
-// same dims; D: lane l holds query l%32 and points 8*(r/4) + 4*(l/32) + r%4
-// for its 16 accumulators r.
+// BF16-MFMA filtered brute force (the "bf16" form of BruteForceGpu::batchKnn)
+// on one dataset.  Results are identical to the plain FP32 path: the MFMA pass
+// only selects candidates, with a certified error bound, and every candidate
+// is reranked with the reference's exact FP32 arithmetic
+// (src/KnnTrees.cuh:65-77).  The screen and rerank bodies and their error
+// analysis are in knn_bf16_common.hpp.
 #include <cmath>
 
-#include "knn_device.hpp"
+#include "knn_bf16_common.hpp"
 #include "knn_internal.hpp"
 
 namespace knn {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
 void query_bf(Handle& h, const QueryArgs& a);
 int pick_buf_kind(bool heap, bool lds, uint32_t K);
-
-// FP32 -> bf16 round-to-nearest-even on the bit pattern (inf/NaN kept).
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-    const uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
-__device__ __forceinline__ float bf16_val(uint32_t b) { return __uint_as_float(b << 16); }
 
 // Points -> tiled hi/lo bf16 operands: tile t (32 points), k-step kk (16 dims),
 // lane l: 8 bf16 of point 32t + l%32, dims 16kk + 8(l/32) .. +7.  One thread
@@ -87,153 +53,26 @@ __global__ void bf16_norms_kernel(const float* __restrict__ pts, uint32_t n, uin
     out[i] = r;
 }
 
-struct Bf16Params {
-    float cA, up, dn, absE;
-};
-
+// grid (query blocks of 128, point chunks); chunk y screens tiles
+// [y * tiles_per_chunk, ...).
 template <int D16, int KC>
-__global__ void __launch_bounds__(256) bf16_screen_kernel(const uint4* __restrict__ phi, const uint4* __restrict__ plo,
-                                                          const float* __restrict__ pnorm, uint32_t n, uint32_t tiles,
-                                                          uint32_t tiles_per_chunk, const float* __restrict__ qs,
-                                                          uint32_t nq, uint32_t D, uint32_t K, float maxd2,
-                                                          Bf16Params P, uint32_t* __restrict__ cnt,
-                                                          uint32_t* __restrict__ cand, uint32_t cap) {
-#pragma clang fp contract(off)
-    __shared__ uint4 s_hi[D16 * 64];
-    __shared__ uint4 s_lo[D16 * 64];
-    __shared__ float s_np[32];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t qcol = lane & 31, half = lane >> 5;
-    const uint32_t qi = blockIdx.x * 128 + wave * 32 + qcol;
-    const bool active = qi < nq;
-
-    // B operands (this lane's query, dims 16kk + 8*half .. +7), hi and lo parts.
-    bf16x8 bhi[D16], blo[D16];
-#pragma unroll
-    for (int kk = 0; kk < D16; ++kk) {
-        uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const uint32_t d = 16 * kk + 8 * half + e;
-            const float x = (active && d < D) ? qs[(size_t)qi * D + d] : 0.0f;
-            const uint32_t hb = bf16_bits(x);
-            const uint32_t lb = bf16_bits(x - bf16_val(hb));
-            hw[e >> 1] |= hb << (16 * (e & 1));
-            lw[e >> 1] |= lb << (16 * (e & 1));
-        }
-        const uint4 vh = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-        const uint4 vl = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-        bhi[kk] = __builtin_bit_cast(bf16x8, vh);
-        blo[kk] = __builtin_bit_cast(bf16x8, vl);
-    }
-    float qn = 0.0f;
-    if (active) {
-        const float* q = qs + (size_t)qi * D;
-        qn = q[0] * q[0];
-        for (uint32_t d = 1; d < D; ++d) {
-            const float s = q[d] * q[d];
-            qn = qn + s;
-        }
-    }
-    // Candidate cap from maxD2: only a non-negative, non-NaN radius bounds f.
-    const uint32_t mb = f2u(maxd2);
-    const float maxcap = ((mb >> 31) == 0 && mb < 0x7F800001u) ? maxd2 : INFINITY;
-
-    // This lane's K smallest upper bounds, ascending; KC-K leading -inf pads
-    // keep the K-th at a static slot.
-    float ul[KC];
-#pragma unroll
-    for (int i = 0; i < KC; ++i) ul[i] = (i >= KC - (int)K) ? INFINITY : -INFINITY;
-    float lane_tau = INFINITY;
-    float tau = fminf(INFINITY, maxcap);
-
+__global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t tiles_per_chunk, uint32_t stage) {
+    __shared__ uint4 s_hi[bf16_stage_tiles<D16>() * D16 * 64];
+    __shared__ uint4 s_lo[bf16_stage_tiles<D16>() * D16 * 64];
+    __shared__ float s_np[bf16_stage_tiles<D16>() * 32];
     const uint32_t t0 = blockIdx.y * tiles_per_chunk;
-    const uint32_t t1 = min(tiles, t0 + tiles_per_chunk);
-    for (uint32_t t = t0; t < t1; ++t) {
-        __syncthreads();
-        const size_t base = (size_t)t * D16 * 64;
-        for (uint32_t i = tid; i < D16 * 64; i += 256) {
-            s_hi[i] = phi[base + i];
-            s_lo[i] = plo[base + i];
-        }
-        if (tid < 32) s_np[tid] = (t * 32 + tid < n) ? pnorm[t * 32 + tid] : 0.0f;
-        __syncthreads();
-        f32x16 acc = {};
-#pragma unroll
-        for (int kk = 0; kk < D16; ++kk) {
-            const bf16x8 ahi = __builtin_bit_cast(bf16x8, s_hi[kk * 64 + lane]);
-            const bf16x8 alo = __builtin_bit_cast(bf16x8, s_lo[kk * 64 + lane]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi[kk], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo[kk], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi[kk], acc, 0, 0, 0);
-        }
-        if (active) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
-                const uint32_t pt = t * 32 + row;
-                if (pt < n) {
-                    const float sumn = qn + s_np[row];
-                    const float A = sumn - 2.0f * acc[r];
-                    const float E = P.cA * sumn + P.absE;
-                    const float L = (A - E) * P.dn;
-                    const float U = (A + E) * P.up;
-                    if (!(L > tau)) {
-                        const uint32_t pos = atomicAdd(&cnt[qi], 1u);
-                        if (pos < cap) cand[(size_t)qi * cap + pos] = pt;
-                    }
-                    if (U < lane_tau) {
-                        float key = U;
-#pragma unroll
-                        for (int i = 0; i < KC; ++i) {
-                            const float e = ul[i];
-                            ul[i] = fminf(e, key);
-                            key = fmaxf(e, key);
-                        }
-                        lane_tau = ul[KC - 1];
-                    }
-                }
-            }
-        }
-        // Both lanes of a query hold upper bounds of the same K-th distance.
-        const float other = __shfl_xor(lane_tau, 32, 64);
-        tau = fminf(fminf(lane_tau, other), maxcap);
-    }
+    const uint32_t t1 = min(S.tiles, t0 + tiles_per_chunk);
+    bf16_screen_run<D16, KC>(S, blockIdx.x, t0, t1, stage, s_hi, s_lo, s_np);
 }
 
 // Exact rerank of the candidates of each query (one lane per query); queries
 // whose candidates overflowed are listed for the plain fallback.
 template <int KIND>
-__global__ void __launch_bounds__(256) bf16_rerank_kernel(const float* __restrict__ pts, uint32_t D,
-                                                          const float* __restrict__ qs, uint32_t nq, float maxd2,
-                                                          uint32_t K, const uint32_t* __restrict__ cnt,
-                                                          const uint32_t* __restrict__ cand, uint32_t cap,
-                                                          int32_t* __restrict__ oidx, float* __restrict__ odist,
-                                                          uint32_t* __restrict__ ovf, uint32_t* __restrict__ novf) {
+__global__ void __launch_bounds__(256) bf16_rerank_kernel(Bf16Set S, uint32_t* __restrict__ ovf,
+                                                          uint32_t* __restrict__ novf) {
     const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= nq) return;
-    const uint32_t c = cnt[qi];
-    if (c > cap) {
-        ovf[atomicAdd(novf, 1u)] = qi;
-        return;
-    }
-    typename BufOf<KIND>::T buf;
-    const uint64_t empty = ((uint64_t)f2u(maxd2) << 32) | kEmptyLo;
-    uint64_t worst = buf.init(empty, K);
-    uint32_t whi = key_hi(worst);
-    const float* q = qs + (size_t)qi * D;
-    for (uint32_t j = 0; j < c; ++j) {
-        const uint32_t pt = cand[(size_t)qi * cap + j];
-        const float d2 = dist2_rt(q, 1, pts + (size_t)pt * D, D);
-        if (f2u(d2) <= whi) {
-            const uint64_t key = make_key(d2, pt);
-            if (key < worst) {
-                worst = buf.consider(key);
-                whi = key_hi(worst);
-            }
-        }
-    }
-    buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
+    if (qi >= S.nq) return;
+    if (!bf16_rerank_one<KIND>(S, qi)) ovf[atomicAdd(novf, 1u)] = qi;
 }
 
 __global__ void gather_rows_kernel(const float* __restrict__ src, const uint32_t* __restrict__ ids, uint32_t m,
@@ -253,7 +92,7 @@ __global__ void scatter_rows_kernel(const int32_t* __restrict__ si, const float*
     od[dst] = sd[g];
 }
 
-static uint32_t d16_of(uint32_t D) {
+uint32_t bf16_d16_of(uint32_t D) {
     uint32_t d16 = (D + 15) / 16;
     if (d16 <= 1) return 1;
     if (d16 <= 2) return 2;
@@ -262,26 +101,33 @@ static uint32_t d16_of(uint32_t D) {
     return 16;
 }
 
+// Tiled hi/lo operands and norms of n x D points (tiles of 32 points).
+void launch_bf16_prep(const float* pts, uint32_t n, uint32_t D, uint4* phi, uint4* plo, float* norms,
+                      hipStream_t st) {
+    const uint32_t D16 = bf16_d16_of(D);
+    const uint32_t tiles = (n + 31) / 32;
+    const size_t words = (size_t)tiles * D16 * 64;
+    if (words) {
+        hipLaunchKernelGGL(bf16_prep_points_kernel, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, st, pts, n,
+                           D, D16, tiles, phi, plo);
+    }
+    if (n) hipLaunchKernelGGL(bf16_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, pts, n, D, norms);
+    KNN_CHECK_LAUNCH();
+}
+
 void prepare_bf16(Handle& h, hipStream_t st) {
     if (h.pts_bf16.n) return;
     const uint32_t n = h.info.n, D = h.info.dims;
-    const uint32_t D16 = d16_of(D);
+    const uint32_t D16 = bf16_d16_of(D);
     const uint32_t tiles = (n + 31) / 32;
     const size_t words = (size_t)tiles * D16 * 64;  // uint4 per part
     h.pts_bf16.alloc(words * 16);                   // hi then lo, 8 bf16 per uint4
     h.norms.alloc(n ? n : 1);
     uint4* phi = reinterpret_cast<uint4*>(h.pts_bf16.get());
-    uint4* plo = phi + words;
-    if (words) {
-        hipLaunchKernelGGL(bf16_prep_points_kernel, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, st,
-                           h.pts.get(), n, D, D16, tiles, phi, plo);
-    }
-    if (n) hipLaunchKernelGGL(bf16_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, h.pts.get(), n, D, h.norms.get());
-    KNN_CHECK_LAUNCH();
+    launch_bf16_prep(h.pts.get(), n, D, phi, phi + words, h.norms.get(), st);
 }
 
-using ScreenFn = void (*)(const uint4*, const uint4*, const float*, uint32_t, uint32_t, uint32_t, const float*, uint32_t,
-                          uint32_t, uint32_t, float, Bf16Params, uint32_t*, uint32_t*, uint32_t);
+using ScreenFn = void (*)(Bf16Set, uint32_t, uint32_t);
 
 template <int D16>
 static ScreenFn screen_for(uint32_t K) {
@@ -290,8 +136,7 @@ static ScreenFn screen_for(uint32_t K) {
     return bf16_screen_kernel<D16, 32>;
 }
 
-using RerankFn = void (*)(const float*, uint32_t, const float*, uint32_t, float, uint32_t, const uint32_t*,
-                          const uint32_t*, uint32_t, int32_t*, float*, uint32_t*, uint32_t*);
+using RerankFn = void (*)(Bf16Set, uint32_t*, uint32_t*);
 
 static RerankFn rerank_for(int kind) {
     switch (kind) {
@@ -305,7 +150,7 @@ static RerankFn rerank_for(int kind) {
 }
 
 // Error-bound constants for dimension D (computed in double on the host).
-static Bf16Params bf16_params(uint32_t D, uint32_t D16) {
+Bf16Params bf16_params(uint32_t D, uint32_t D16) {
     const double u = std::ldexp(1.0, -24), ub = std::ldexp(1.0, -8);
     auto gamma = [&](double k) { return k * u / (1.0 - k * u); };
     const double cnorm = gamma(D + 1.0);
@@ -330,7 +175,7 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
         return;
     }
     prepare_bf16(h, a.stream);
-    const uint32_t D16 = d16_of(D);
+    const uint32_t D16 = bf16_d16_of(D);
     const uint32_t tiles = (n + 31) / 32;
     const size_t words = (size_t)tiles * D16 * 64;
     const uint4* phi = reinterpret_cast<const uint4*>(h.pts_bf16.get());
@@ -350,29 +195,46 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
 
     const uint32_t qblocks = (a.nq + 127) / 128;
     const uint32_t want = 4 * device_limits().cu_count;
-    uint32_t S = 1;
+    uint32_t S_ = 1;
     if (qblocks < want) {
-        S = (want + qblocks - 1) / qblocks;
+        S_ = (want + qblocks - 1) / qblocks;
         const uint32_t max_s = tiles / 32 > 0 ? tiles / 32 : 1;
-        if (S > max_s) S = max_s;
-        if (S > 64) S = 64;
+        if (S_ > max_s) S_ = max_s;
+        if (S_ > 64) S_ = 64;
     }
-    const uint32_t tpc = (tiles + S - 1) / S;
-    const Bf16Params P = bf16_params(D, D16);
+    const uint32_t tpc = (tiles + S_ - 1) / S_;
+    Bf16Set S{};
+    S.phi = phi;
+    S.plo = plo;
+    S.pnorm = h.norms.get();
+    S.pts = h.pts.get();
+    S.n = n;
+    S.tiles = tiles;
+    S.D = D;
+    S.K = a.K;
+    S.qs = a.q;
+    S.nq = a.nq;
+    S.maxd2 = a.maxd2;
+    S.P = bf16_params(D, D16);
+    S.cnt = cnt;
+    S.cand = cand;
+    S.cap = cap;
+    S.oidx = a.oidx;
+    S.odist = a.odist;
     ScreenFn fn = nullptr;
+    uint32_t max_stage = 1;
     switch (D16) {
-        case 1: fn = screen_for<1>(a.K); break;
-        case 2: fn = screen_for<2>(a.K); break;
-        case 4: fn = screen_for<4>(a.K); break;
-        case 8: fn = screen_for<8>(a.K); break;
-        default: fn = screen_for<16>(a.K); break;
+        case 1: fn = screen_for<1>(a.K); max_stage = bf16_stage_tiles<1>(); break;
+        case 2: fn = screen_for<2>(a.K); max_stage = bf16_stage_tiles<2>(); break;
+        case 4: fn = screen_for<4>(a.K); max_stage = bf16_stage_tiles<4>(); break;
+        case 8: fn = screen_for<8>(a.K); max_stage = bf16_stage_tiles<8>(); break;
+        default: fn = screen_for<16>(a.K); max_stage = bf16_stage_tiles<16>(); break;
     }
+    const uint32_t stage = a.tile ? (a.tile < max_stage ? a.tile : max_stage) : max_stage;
     EventTimer timer(a.timed, a.stream);
-    hipLaunchKernelGGL(fn, dim3(qblocks, S), dim3(256), 0, a.stream, phi, plo, h.norms.get(), n, tiles, tpc, a.q, a.nq,
-                       D, a.K, a.maxd2, P, cnt, cand, cap);
+    hipLaunchKernelGGL(fn, dim3(qblocks, S_), dim3(256), 0, a.stream, S, tpc, stage);
     const int kind = pick_buf_kind(a.heap, false, a.K);
-    hipLaunchKernelGGL(rerank_for(kind), dim3((a.nq + 255) / 256), dim3(256), 0, a.stream, h.pts.get(), D, a.q, a.nq,
-                       a.maxd2, a.K, cnt, cand, cap, a.oidx, a.odist, ovf, novf);
+    hipLaunchKernelGGL(rerank_for(kind), dim3((a.nq + 255) / 256), dim3(256), 0, a.stream, S, ovf, novf);
     KNN_CHECK_LAUNCH();
     uint32_t n_ovf = 0;
     KNN_HIP(hipMemcpyAsync(&n_ovf, novf, 4, hipMemcpyDeviceToHost, a.stream));

@@ -1,0 +1,214 @@
+// BF16-MFMA screening and exact rerank, shared by the single-dataset brute
+// force (knn_bf16.hip) and the persistent multi-dataset launch
+// (knn_batch.hip).
+//
+// 1. Each coordinate x is split into two bf16 parts, hi = bf16(x) and
+//    lo = bf16(x - hi) (x - hi is exact in FP32).  q.p is approximated by
+//    s = q_hi.p_hi + q_hi.p_lo + q_lo.p_hi, three v_mfma_f32_32x32x16_bf16
+//    products accumulated in FP32, and A = (|q|^2 + |p|^2) - 2 s with FP32
+//    norms.
+// 2. Forward error analysis (Higham, Accuracy and Stability of Numerical
+//    Algorithms, ch. 3) bounds |A - d2| by E = cA (|q|^2 + |p|^2) + tiny, with
+//    cA covering the dropped lo.lo and rounding terms (3.2 u_b^2, u_b = 2^-8),
+//    the FP32 accumulation of 3*Dpad products (gamma_{3 Dpad}) and the norm
+//    errors (gamma_{D+1}), plus 25 % slack; and the reference's computed
+//    distance f differs from d2 by at most gamma_{D+2} d2.  So
+//    L = (A - E) dn <= f <= (A + E) up = U.
+// 3. Per query, the lanes keep the K smallest U seen (an upper bound tau of
+//    the K-th smallest f) and emit every point with L <= tau (and <= maxD2
+//    when that is a non-negative number).  The true top-K always qualifies,
+//    whatever the processing order, because L <= f <= f_K <= tau.
+// 4. Rerank: exact FP32 distance (src/KnnTrees.cuh:65-77) and the usual key
+//    insertion over the candidates only.  A query whose candidates overflow
+//    the buffer is recomputed by an exact FP32 scan.
+//
+// MFMA layout (32x32x16, wave64): A = 32 points x 16 dims, lane l holds point
+// l%32, dims 8*(l/32)..+7; B = 16 dims x 32 queries, lane l holds query l%32,
+// same dims; D: lane l holds query l%32 and points 8*(r/4) + 4*(l/32) + r%4
+// for its 16 accumulators r.  A workgroup is 4 waves = 128 queries and
+// stages up to 8 point tiles (of 32 points) in LDS per barrier pair.
+#pragma once
+#include "knn_device.hpp"
+
+namespace knn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// FP32 -> bf16 round-to-nearest-even on the bit pattern (inf/NaN kept).
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ float bf16_val(uint32_t b) { return __uint_as_float(b << 16); }
+
+struct Bf16Params {
+    float cA, up, dn, absE;
+};
+
+// One dataset as seen by the screen and rerank passes.
+struct Bf16Set {
+    const uint4* phi;    // tiled hi parts: tile t, k-step kk, lane l -> uint4 (8 bf16)
+    const uint4* plo;    // tiled lo parts, same layout
+    const float* pnorm;  // |p|^2
+    const float* pts;    // FP32 points (rerank and fallback)
+    uint32_t n, tiles, D, K;
+    const float* qs;
+    uint32_t nq;
+    float maxd2;
+    Bf16Params P;
+    uint32_t* cnt;   // per query candidate count
+    uint32_t* cand;  // per query candidate list, `cap` slots
+    uint32_t cap;
+    int32_t* oidx;
+    float* odist;
+};
+
+// Point tiles staged per barrier pair: 8 KiB per part for D16 <= 8.
+template <int D16>
+constexpr uint32_t bf16_stage_tiles() { return D16 >= 8 ? 1u : 8u / D16; }
+template <int D16>
+constexpr size_t bf16_stage_lds() {
+    return 2 * (size_t)bf16_stage_tiles<D16>() * D16 * 64 * 16 + (size_t)bf16_stage_tiles<D16>() * 32 * 4;
+}
+
+// Screen tiles [t0, t1) for the 128 queries of `qblock`, `stage` (<=
+// bf16_stage_tiles<D16>()) tiles per barrier pair.  Every thread of the
+// 256-wide workgroup must call it (it synchronises the workgroup).
+template <int D16, int KC>
+__device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qblock, uint32_t t0, uint32_t t1,
+                                                uint32_t stage, uint4* __restrict__ s_hi, uint4* __restrict__ s_lo,
+                                                float* __restrict__ s_np) {
+#pragma clang fp contract(off)
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t qcol = lane & 31, half = lane >> 5;
+    const uint32_t qi = qblock * 128 + wave * 32 + qcol;
+    const bool active = qi < S.nq;
+    const uint32_t D = S.D;
+
+    // B operands (this lane's query, dims 16kk + 8*half .. +7), hi and lo parts.
+    bf16x8 bhi[D16], blo[D16];
+#pragma unroll
+    for (int kk = 0; kk < D16; ++kk) {
+        uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t d = 16 * kk + 8 * half + e;
+            const float x = (active && d < D) ? S.qs[(size_t)qi * D + d] : 0.0f;
+            const uint32_t hb = bf16_bits(x);
+            const uint32_t lb = bf16_bits(x - bf16_val(hb));
+            hw[e >> 1] |= hb << (16 * (e & 1));
+            lw[e >> 1] |= lb << (16 * (e & 1));
+        }
+        bhi[kk] = __builtin_bit_cast(bf16x8, make_uint4(hw[0], hw[1], hw[2], hw[3]));
+        blo[kk] = __builtin_bit_cast(bf16x8, make_uint4(lw[0], lw[1], lw[2], lw[3]));
+    }
+    float qn = 0.0f;
+    if (active) {
+        const float* q = S.qs + (size_t)qi * D;
+        qn = q[0] * q[0];
+        for (uint32_t d = 1; d < D; ++d) {
+            const float s = q[d] * q[d];
+            qn = qn + s;
+        }
+    }
+    // Candidate cap from maxD2: only a non-negative, non-NaN radius bounds f.
+    const uint32_t mb = f2u(S.maxd2);
+    const float maxcap = ((mb >> 31) == 0 && mb < 0x7F800001u) ? S.maxd2 : INFINITY;
+
+    // This lane's K smallest upper bounds, ascending; KC-K leading -inf pads
+    // keep the K-th at a static slot.
+    float ul[KC];
+#pragma unroll
+    for (int i = 0; i < KC; ++i) ul[i] = (i >= KC - (int)S.K) ? INFINITY : -INFINITY;
+    float lane_tau = INFINITY;
+    float tau = maxcap;
+
+    for (uint32_t tb = t0; tb < t1; tb += stage) {
+        const uint32_t nt = min(stage, t1 - tb);
+        __syncthreads();
+        const size_t base = (size_t)tb * D16 * 64;
+        const uint32_t words = nt * D16 * 64;
+        for (uint32_t i = tid; i < words; i += 256) {
+            s_hi[i] = S.phi[base + i];
+            s_lo[i] = S.plo[base + i];
+        }
+        for (uint32_t i = tid; i < nt * 32; i += 256) {
+            const uint32_t p = tb * 32 + i;
+            s_np[i] = p < S.n ? S.pnorm[p] : 0.0f;
+        }
+        __syncthreads();
+        for (uint32_t u = 0; u < nt; ++u) {
+            const uint32_t t = tb + u;
+            f32x16 acc = {};
+#pragma unroll
+            for (int kk = 0; kk < D16; ++kk) {
+                const bf16x8 ahi = __builtin_bit_cast(bf16x8, s_hi[(u * D16 + kk) * 64 + lane]);
+                const bf16x8 alo = __builtin_bit_cast(bf16x8, s_lo[(u * D16 + kk) * 64 + lane]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi[kk], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo[kk], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi[kk], acc, 0, 0, 0);
+            }
+            if (active) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
+                    const uint32_t pt = t * 32 + row;
+                    if (pt < S.n) {
+                        const float sumn = qn + s_np[u * 32 + row];
+                        const float A = sumn - 2.0f * acc[r];
+                        const float E = S.P.cA * sumn + S.P.absE;
+                        const float L = (A - E) * S.P.dn;
+                        const float U = (A + E) * S.P.up;
+                        if (!(L > tau)) {
+                            const uint32_t pos = atomicAdd(&S.cnt[qi], 1u);
+                            if (pos < S.cap) S.cand[(size_t)qi * S.cap + pos] = pt;
+                        }
+                        if (U < lane_tau) {
+                            float key = U;
+#pragma unroll
+                            for (int i = 0; i < KC; ++i) {
+                                const float e = ul[i];
+                                ul[i] = fminf(e, key);
+                                key = fmaxf(e, key);
+                            }
+                            lane_tau = ul[KC - 1];
+                        }
+                    }
+                }
+            }
+            // Both lanes of a query hold upper bounds of the same K-th distance.
+            const float other = __shfl_xor(lane_tau, 32, 64);
+            tau = fminf(fminf(lane_tau, other), maxcap);
+        }
+    }
+}
+
+// Exact rerank of query qi's candidates (array-mode or heap-mode K-buffer
+// per KIND).  Returns false, writing nothing, when the candidates overflowed.
+template <int KIND>
+__device__ __forceinline__ bool bf16_rerank_one(const Bf16Set& S, uint32_t qi) {
+    const uint32_t c = S.cnt[qi];
+    if (c > S.cap) return false;
+    typename BufOf<KIND>::T buf;
+    const uint64_t empty = ((uint64_t)f2u(S.maxd2) << 32) | kEmptyLo;
+    uint64_t worst = buf.init(empty, S.K);
+    uint32_t whi = key_hi(worst);
+    const float* q = S.qs + (size_t)qi * S.D;
+    for (uint32_t j = 0; j < c; ++j) {
+        const uint32_t pt = S.cand[(size_t)qi * S.cap + j];
+        const float d2 = dist2_rt(q, 1, S.pts + (size_t)pt * S.D, S.D);
+        if (f2u(d2) <= whi) {
+            const uint64_t key = make_key(d2, pt);
+            if (key < worst) {
+                worst = buf.consider(key);
+                whi = key_hi(worst);
+            }
+        }
+    }
+    buf.write(S.oidx + (size_t)qi * S.K, S.odist + (size_t)qi * S.K, S.K);
+    return true;
+}
+
+}  // namespace knn

@@ -11,23 +11,9 @@ namespace knn {
 
 static thread_local std::string g_last_error;
 
-static knn_status fail(knn_status s, const std::string& m) {
+knn_status set_last_error(knn_status s, const std::string& m) {
     g_last_error = m;
     return s;
-}
-
-template <class F>
-static knn_status guarded(F&& f) {
-    try {
-        f();
-        return KNN_OK;
-    } catch (const Error& e) {
-        return fail(e.status, e.what());
-    } catch (const std::bad_alloc&) {
-        return fail(KNN_ERR_OOM, "host allocation failed");
-    } catch (const std::exception& e) {
-        return fail(KNN_ERR_INTERNAL, e.what());
-    }
 }
 
 const DeviceLimits& device_limits() {
@@ -199,12 +185,14 @@ knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max
         a.sort_queries = (o.flags & KNN_FLAG_NO_QUERY_SORT) == 0;
         a.count_visits = (o.flags & KNN_FLAG_COUNT_VISITS) != 0;
         a.per_lane = (o.flags & KNN_FLAG_PER_LANE) != 0;
+        a.xcd_remap = (o.flags & KNN_FLAG_NO_XCD_REMAP) == 0;
         a.order_morton = (o.flags & KNN_FLAG_MORTON_ORDER) != 0 || h.kind == KNN_KIND_BALL;
         a.timed = (opts == nullptr || opts->stream == nullptr);
         for (float& v : h.stats) v = 0.0f;
         a.leaf_span = o.leaf_span == 0 ? default_span(h, k, a.per_lane)
                                        : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
         a.form = (knn_form)o.form;
+        a.tile = o.tile;
         a.stream = (hipStream_t)o.stream;
         hipEvent_t e0, e1;
         KNN_HIP(hipEventCreate(&e0));

@@ -222,6 +222,17 @@ enum BufKind : int {
     BUF_REGH8 = 5, BUF_REGH16 = 6, BUF_REGH32 = 7
 };
 
+// XCD-aware block order.  The dispatcher hands workgroup b to XCD b % 8, and
+// each XCD has its own L2.  Renumbering so that XCD x runs one contiguous
+// range of logical blocks keeps neighbouring (spatially ordered) blocks on
+// one L2.  A bijection on [0, grid) for any grid size.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t grid) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t per = grid / kXcds, rem = grid % kXcds;
+    const uint32_t x = b % kXcds, j = b / kXcds;
+    return x * per + (x < rem ? x : rem) + j;
+}
+
 template <int KIND> struct BufOf;
 template <> struct BufOf<BUF_REG8> { using T = RegArray<8>; static constexpr bool lds = false; };
 template <> struct BufOf<BUF_REG16> { using T = RegArray<16>; static constexpr bool lds = false; };

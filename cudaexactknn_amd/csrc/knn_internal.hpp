@@ -31,6 +31,24 @@ inline void require(bool ok, const std::string& msg, knn_status s = KNN_ERR_INVA
     if (!ok) throw Error(s, msg);
 }
 
+// Records the thread-local message returned by knn_last_error (knn_capi.hip).
+knn_status set_last_error(knn_status s, const std::string& m);
+
+// Runs f, converting exceptions into a status plus the thread-local message.
+template <class F>
+knn_status guarded(F&& f) {
+    try {
+        f();
+        return KNN_OK;
+    } catch (const Error& e) {
+        return set_last_error(e.status, e.what());
+    } catch (const std::bad_alloc&) {
+        return set_last_error(KNN_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return set_last_error(KNN_ERR_INTERNAL, e.what());
+    }
+}
+
 // Owning device buffer (grow-only when used as scratch).
 template <class T>
 struct DevBuf {
@@ -141,7 +159,9 @@ struct QueryArgs {
     uint32_t K;
     uint32_t block;
     bool heap, track_lds, stack_lds, sort_queries, count_visits, per_lane, order_morton;
+    bool xcd_remap = true;  // trees: XCD-contiguous block order
     uint32_t leaf_span;  // extra levels folded into one leaf scan
+    uint32_t tile = 0;   // bf16 form: point tiles staged per barrier (0 = default)
     knn_form form;
     hipStream_t stream;
     bool timed;  // synchronous call: time kernels with events into Handle::stats

@@ -1,0 +1,220 @@
+// kd-tree batch query, wave-cooperative form (MI355X-native default for
+// ordered query batches; results identical to knn_kd_query.hip and to the
+// brute force).
+//
+// The 64 lanes of a wave hold 64 queries that are neighbours in the batch's
+// spatial order.  The wave walks ONE depth-first traversal whose control flow
+// is wave-uniform: at each node every lane evaluates its own certified lower
+// bound for the two children, two wave64 ballots say whether any lane needs
+// each child, and a child is skipped only when no lane needs it.  So no lane
+// ever skips a subtree its own certified test keeps, and every lane's top-K is
+// exact, while node and leaf loads are issued once per wave at uniform
+// addresses and the traversal has no lane divergence.
+//
+// Certified box bound: the child region is the box of split planes on the
+// path.  For point p in the box and each dim, |fl(q_d - face_d)| <=
+// |fl(q_d - p_d)| (rounding is monotone), so summing the squared face gaps in
+// the distance's own sequential FP32 order gives a value <= the computed
+// distance (fl addition of non-negatives is monotone).  Dims where q lies
+// inside the box contribute 0.
+//
+// The stack is per wave (node + box) in LDS; the K-buffer follows the
+// USE_HEAP_TRACKING / USE_TRACKING_SHMEM options as in the per-lane form.
+#pragma once
+#include "knn_tree_common.hpp"
+
+namespace knn {
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float unif(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
+// Sequential-order certified bound of q to the box [lo, hi].
+template <int D>
+__device__ __forceinline__ float box_bound(const float (&q)[D], const float (&lo)[D], const float (&hi)[D]) {
+#pragma clang fp contract(off)
+    float r = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        float t = 0.0f;
+        if (q[d] < lo[d]) t = q[d] - lo[d];
+        else if (q[d] > hi[d]) t = q[d] - hi[d];
+        const float s = t * t;
+        r = d == 0 ? s : r + s;
+    }
+    return r;
+}
+
+// One wave's traversal for the ordered queries i = (this lane's index).
+// `stack` is this wave's LDS stack, `kbuf` this lane's LDS K-buffer slot
+// (stride BS), used only by LDS buffer kinds.
+template <int D, int KIND>
+__device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uint32_t* stack, uint64_t* kbuf,
+                                              uint32_t BS) {
+    constexpr int ENT = 1 + 2 * D;  // words per stack entry: node, lo[D], hi[D]
+    const uint32_t tid = threadIdx.x;
+    const bool active = i < A.nq;
+    const uint32_t qi = active ? (A.perm ? A.perm[i] : i) : 0u;
+    float q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = active ? A.qs[(size_t)qi * D + d] : 0.0f;
+    typename BufOf<KIND>::T buf;
+    bind_lds(buf, kbuf, BS);
+    bool tightened = false;
+    uint64_t worst = 0;
+    if (active) worst = buf.init(start_key(A, qi, tightened), A.K);
+    uint32_t whi = active ? key_hi(worst) : 0u;
+    uint32_t nbytes = 0, nslots = 0;
+
+    const uint32_t SD = A.split_dims, stop = A.stop_level;
+    const uint32_t stride = 1 + D;
+    const bool single_bucket = stop == A.node_depth;
+
+    float lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        lo[d] = -INFINITY;
+        hi[d] = INFINITY;
+    }
+    // Priming: each lane descends to its own landing bucket and scans it, so
+    // every lane enters the shared traversal with a tight K-th key.  The
+    // shared traversal skips that bucket for this lane.
+    uint32_t land0 = 0xFFFFFFFFu, landN = 0;
+    if (active) {
+        const uint32_t pl = A.prime_level;
+        uint32_t n = 0, dd = 0;
+        for (uint32_t l = 0; l < pl; ++l) {
+            const float sv = A.nodes[n];
+            float x = q[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) x = ((uint32_t)d == dd) ? q[d] : x;
+            n = 2 * n + 1 + (x < sv ? 0u : 1u);
+            dd = (dd + 1 == SD) ? 0 : dd + 1;
+        }
+        nbytes += 4u * pl;
+        leaf_range(A, n, pl, land0, landN);
+        QPoint<D> qp;
+#pragma unroll
+        for (int d = 0; d < D; ++d) qp.v[d] = q[d];
+        scan_slots<D>(A.leaves, stride, land0, landN, pl == A.node_depth, qp, buf, worst, whi, nslots);
+    }
+
+    // Visit counters: the priming descent is per lane; the shared traversal
+    // issues each node and slot load once per wave, so it is counted once,
+    // on the wave's first active lane.
+    const uint32_t prime_bytes = nbytes, prime_slots = nslots;
+    nbytes = nslots = 0;
+    uint32_t node = 0, level = 0, dim = 0, sp = 0;
+    float cur = 0.0f;  // this lane's bound for the current node's box
+
+    if (__ballot(active) != 0ull) {
+        for (;;) {
+            if (level < stop) {
+                const float split = unif(A.nodes[node]);
+                nbytes += 4u;
+                // Far-side bound: the current box with the dim face moved to the
+                // split.  Every dim-indexed update is written as a select over all
+                // dims, so no array is ever indexed by the runtime dim (that would
+                // put it in scratch).
+                float qd = q[0];
+#pragma unroll
+                for (int d = 1; d < D; ++d) qd = ((uint32_t)d == dim) ? q[d] : qd;
+                const bool left_side = qd < split;  // near child of this lane is the left one
+                float flo[D], fhi[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const bool here = (uint32_t)d == dim;
+                    flo[d] = (here && left_side) ? split : lo[d];
+                    fhi[d] = (here && !left_side) ? split : hi[d];
+                }
+                const float far = box_bound<D>(q, flo, fhi);
+                const float bl = left_side ? cur : far;
+                const float br = left_side ? far : cur;
+                const uint64_t needL = __ballot(active && f2u(bl) <= whi);
+                const uint64_t needR = __ballot(active && f2u(br) <= whi);
+                const uint32_t l = 2 * node + 1;
+                if (needL && needR) {
+                    const uint32_t nleft = __popcll(__ballot(active && left_side));
+                    const uint32_t nact = __popcll(__ballot(active));
+                    const bool goLeft = 2 * nleft >= nact;
+                    // push the other child with its box
+                    uint32_t* e = stack + (size_t)sp * ENT;
+                    if ((tid & 63) == 0) {
+                        e[0] = goLeft ? l + 1 : l;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            const bool here = (uint32_t)d == dim;
+                            const float a = (here && goLeft) ? split : lo[d];
+                            const float b = (here && !goLeft) ? split : hi[d];
+                            e[1 + d] = __float_as_uint(a);
+                            e[1 + D + d] = __float_as_uint(b);
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    ++sp;
+                    node = goLeft ? l : l + 1;
+                    cur = goLeft ? bl : br;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const bool here = (uint32_t)d == dim;
+                        hi[d] = (here && goLeft) ? split : hi[d];
+                        lo[d] = (here && !goLeft) ? split : lo[d];
+                    }
+                    ++level;
+                    dim = (dim + 1 == SD) ? 0 : dim + 1;
+                    continue;
+                }
+                if (needL || needR) {
+                    const bool goLeft = needL != 0ull;
+                    node = goLeft ? l : l + 1;
+                    cur = goLeft ? bl : br;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const bool here = (uint32_t)d == dim;
+                        hi[d] = (here && goLeft) ? split : hi[d];
+                        lo[d] = (here && !goLeft) ? split : lo[d];
+                    }
+                    ++level;
+                    dim = (dim + 1 == SD) ? 0 : dim + 1;
+                    continue;
+                }
+                // neither child needed: fall through to pop
+            } else {
+                uint32_t slot0, count;
+                leaf_range(A, node, level, slot0, count);
+                QPoint<D> qp;
+#pragma unroll
+                for (int d = 0; d < D; ++d) qp.v[d] = q[d];
+                packet_scan<D>(A.leaves, uni(slot0), uni(count), single_bucket, qp, active, land0, landN, buf, worst,
+                               whi, nslots);
+            }
+            // pop until some lane needs the entry
+            bool found = false;
+            while (sp > 0) {
+                --sp;
+                const uint32_t* e = stack + (size_t)sp * ENT;
+                const uint32_t n2 = uni(e[0]);
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    lo[d] = unif(__uint_as_float(e[1 + d]));
+                    hi[d] = unif(__uint_as_float(e[1 + D + d]));
+                }
+                const float b = box_bound<D>(q, lo, hi);
+                if (__ballot(active && f2u(b) <= whi) == 0ull) continue;
+                node = n2;
+                level = 31u - __clz(node + 1);
+                dim = level_dim(level, SD, A.sd_inv);
+                cur = b;
+                found = true;
+                break;
+            }
+            if (!found) break;
+        }
+    }
+    const uint64_t act = __ballot(active);
+    const bool lead = act != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)act) - 1);
+    if (active)
+        finish_query(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
+                     prime_slots + (lead ? nslots : 0u));
+}
+
+}  // namespace knn

@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(1024) kd_query_kernel(TreeArgs A) {
     if (BufOf<KIND>::lds) cur += (size_t)BS * A.K * 8;
     float* qlds = reinterpret_cast<float*>(cur);
 
-    const uint32_t i = blockIdx.x * BS + tid;
+    const uint32_t i = tree_block(A) * BS + tid;
     if (i >= A.nq) return;  // no barriers below
     const uint32_t qi = A.perm ? A.perm[i] : i;
     QPoint<D> q;

@@ -37,7 +37,12 @@ struct TreeArgs {
     const float* init_r2;     // optional per-query starting radius^2 (learned form)
     uint32_t* short_flags;    // optional: set to 1 for a query whose tightened radius left a slot empty
     unsigned long long* counters;  // optional: [0] node bytes read, [1] leaf slots read
+    uint32_t xcd_remap;    // 1: logical block = xcd_block(blockIdx.x, gridDim.x)
 };
+
+__device__ __forceinline__ uint32_t tree_block(const TreeArgs& A) {
+    return A.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+}
 
 // Query coordinates: compile-time D in VGPRs, or runtime D lane-interleaved in LDS.
 template <int D>

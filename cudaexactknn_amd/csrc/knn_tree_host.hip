@@ -166,6 +166,10 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
         else
             bfn = ball_packet_kernel_for(I.dims, kind);
     }
+    // XCD-contiguous block order: measured 10 % faster for the per-lane
+    // traversal on the 16.7M-point mixture; neutral (kd) or slower (ball) for
+    // the wave-cooperative kernels, which keep plain order.
+    A.xcd_remap = (a.xcd_remap && !(kfn || bfn)) ? 1u : 0u;
     if (kfn || bfn) {
         const uint32_t words = h.kind == KNN_KIND_KD ? 1 + 2 * I.dims : 1;
         lds = packet_stack_bytes((BS + 63) / 64, A.stack_depth, words);

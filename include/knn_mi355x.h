@@ -60,6 +60,8 @@ typedef enum knn_form { KNN_FORM_PLAIN = 0, KNN_FORM_BF16 = 1, KNN_FORM_LEARNED 
 /* Flags for knn_query_opts.flags. */
 #define KNN_FLAG_NO_QUERY_SORT 0x1u  /* trees: keep caller order (no spatial reordering) */
 #define KNN_FLAG_COUNT_VISITS 0x2u   /* trees: count node bytes and leaf slots read (stats[5], [6]) */
+#define KNN_FLAG_NO_XCD_REMAP 0x10u  /* per-lane tree traversal: plain blockIdx order instead of
+                                        one contiguous block range per XCD (L2) */
 #define KNN_FLAG_MORTON_ORDER 0x8u   /* kd: order queries by Morton code instead of tree cell
                                         (ball trees always use Morton order) */
 #define KNN_FLAG_PER_LANE 0x4u       /* trees: one independent traversal per lane (the reference's
@@ -74,7 +76,8 @@ typedef struct knn_query_opts {
     uint32_t flags;      /* KNN_FLAG_* */
     uint32_t leaf_span;  /* trees: scan 2^leaf_span leaf buckets per visit (0 = tuned default,
                             0xFFFFFFFF = the tree's own buckets only) */
-    uint32_t reserved;
+    uint32_t tile;       /* bf16 form: 32-point MFMA tiles staged in LDS per barrier, 1..8
+                            (0 = tuned default) */
     void* stream;        /* hipStream_t, NULL = synchronous on the null stream */
 } knn_query_opts;
 
@@ -169,6 +172,51 @@ knn_status knn_mlp_param_count(uint32_t* out);
 knn_status knn_mlp_forward(const float* d_params, const float* d_x, uint32_t n, uint32_t bf16, float* d_out);
 knn_status knn_mlp_gradient(const float* d_params, const float* d_x, const float* d_t, uint32_t n, float* d_grad,
                             float* h_loss);
+
+/* ---- Batched queries over many independent datasets (no reference
+ * counterpart: the reference builds and queries one structure per object and
+ * synchronises after each call).  A batch is planned once: every tree, work
+ * buffer and candidate list is allocated, then the builds and the queries are
+ * captured as two HIP graphs.  knn_batch_run replays them; the query graph is
+ * ONE persistent launch whose workgroups pull (dataset, query block) items
+ * from a device work counter, followed by the BF16 rerank and an exact
+ * fallback for overflowed candidate lists.  Caller buffers (points, queries,
+ * results) must stay valid, at the same addresses, for the batch's lifetime:
+ * every replay reads and writes them again.  Results per item are identical
+ * to knn_query on the same structure (array-mode slot order). */
+typedef struct knn_batch_item {
+    uint32_t kind;         /* KNN_KIND_KD, KNN_KIND_BALL, or KNN_KIND_BRUTE (bf16 form) */
+    uint32_t dims;         /* D: trees 2, 3, 4 or 8; brute force 1..256 */
+    uint32_t n;            /* points (>= 1) */
+    uint32_t nq;           /* queries (0 = item skipped) */
+    uint32_t k;            /* 1..32 */
+    uint32_t split_dims;   /* kd: SPLIT_DIMENSIONS (0 = dims) */
+    uint32_t depth_skips;  /* trees: depthSkips (ball: clamped to >= 1) */
+    float max_d2;          /* maxDistance^2 */
+    const float* d_pts;    /* n x dims, device */
+    const float* d_q;      /* nq x dims, device */
+    int32_t* d_idx;        /* nq x k, device */
+    float* d_dist;         /* nq x k, device */
+} knn_batch_item;
+
+typedef struct knn_batch_opts {
+    uint32_t build_streams;  /* parallel graph branches for the builds (0 = 4) */
+    uint32_t blocks_per_cu;  /* persistent workgroups per CU (0 = occupancy limit) */
+    uint32_t tile;           /* bf16 items: 32-point tiles staged per barrier (0 = 8 / D16) */
+    uint32_t leaf_span;      /* tree items: leaf levels per scan (0 = default per item) */
+} knn_batch_opts;
+
+/* Statistics of the last synchronous knn_batch_run: [0] build ms, [1] query
+ * ms, [2] queries whose bf16 candidate list overflowed (rescanned exactly),
+ * [3] work items, [4] persistent workgroups, [5] graph nodes (build),
+ * [6] graph nodes (query), [7] total queries. */
+knn_status knn_batch_create(const knn_batch_item* items, uint32_t count, const knn_batch_opts* opts,
+                            void** out_batch);
+/* what: 1 = build graph, 2 = query graph, 3 = both.  stream NULL: synchronous
+ * and timed into the statistics. */
+knn_status knn_batch_run(void* batch, uint32_t what, void* stream);
+knn_status knn_batch_stats(const void* batch, float* out8);
+knn_status knn_batch_destroy(void* batch);
 
 #ifdef __cplusplus
 }
