@@ -1,15 +1,22 @@
 """Benchmark: exact kNN queries/sec on one MI355X.
 
-Workload (the real-size tree configuration): 16,777,216 float32 points and
-4,194,304 queries in R^3 from a mixture of 32 isotropic Gaussians (centres
-uniform in [0,100)^3, sigma log-uniform in [0.5, 5], seed 1),
-KdTreeGpu<3,3,256> (depthSkips 0) and BallTreeGpu<3,256> (depthSkips 1),
-batchKnn<K=16>, maxRadius = +inf.
+Headline workload (the real-size tree configuration): 16,777,216 float32
+points and 4,194,304 queries in R^3 from a mixture of 32 isotropic Gaussians
+(centres uniform in [0,100)^3, sigma log-uniform in [0.5, 5], seed 1),
+KdTreeGpu<3,3,256> (depthSkips 0) batchKnn<K=16>, maxRadius = +inf.
 
-A step is one batchKnn over all 4,194,304 queries on the kd tree (inputs
-resident in HBM).  `value` is queries/sec over the timed steps.  The ball
-tree, both build times and the plain brute force at the small uniform
-configuration are reported alongside.
+A step is one batchKnn over all 4,194,304 queries on the kd tree with the
+library's chosen form (wave-cooperative traversal, inputs resident in HBM).
+`value` is queries/sec over the timed steps; the plain HIP path (the
+reference's per-lane traversal shape, src/KdTreeGpu.cuh:37-90) is timed on
+the same tree and reported beside it, with results checked identical.
+
+Also reported (each with its plain counterpart on the same GPU, results
+checked bit-identical): the ball tree and both builds on the same points;
+the small uniform brute force (65,536 x 16,384, K=8, heap); the
+BF16-filtered brute force (1,048,576 x 65,536 scaled normals, D=128, K=32);
+the learned radius on the 8-D anisotropic mixture (4,194,304 x 1,048,576,
+K=32); the batched path over 256 uneven datasets (seed 4).
 
 Roofline: the dominant kernel is the kd traversal.  Its algorithmic bytes
 per launch are Q * (4D query + 8K results + 4 order index) plus the node
@@ -91,6 +98,142 @@ def cpu_baseline(pts, qs, gpu_idx, gpu_dist, sample: int):
     }
 
 
+def _same(a, b) -> bool:
+    import torch
+    return bool(torch.equal(a[0], b[0]) and torch.equal(a[1].view(torch.int32), b[1].view(torch.int32)))
+
+
+def _rows(nq, k, dev):
+    import torch
+    return (torch.empty((nq, k), dtype=torch.int32, device=dev), torch.empty((nq, k), dtype=torch.float32, device=dev))
+
+
+def _timed_query(struct, q, nq, k, dev, reps=2, **opts):
+    """Best wall-clock ms of `reps` synchronous batchKnn calls (after one warm-up)."""
+    import time as _t
+
+    import torch
+
+    import cudaexactknn_amd as knn
+    ri, rd = _rows(nq, k, dev)
+    Q = knn.Query(q, nq, float("inf"), ri, rd)
+    struct.batchKnn(Q, k, **opts)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = _t.perf_counter()
+        struct.batchKnn(Q, k, **opts)
+        torch.cuda.synchronize()
+        best = min(best, (_t.perf_counter() - t0) * 1e3)
+    return best, struct.last_query_stats(), (ri, rd)
+
+
+def leg_bf16(dev):
+    """BF16-filtered vs plain brute force: 1,048,576 x 65,536 scaled normals, D=128, K=32."""
+    import torch
+
+    import cudaexactknn_amd as knn
+    from cudaexactknn_amd import datasets
+    D, K = 128, 32
+    pts, qs = datasets.scaled_normal(1 << 20, 1 << 16, D, 2)
+    bf = knn.BruteForceGpu(torch.from_numpy(pts).to(dev))
+    q = torch.from_numpy(qs).to(dev)
+    nq = len(qs)
+    t_plain, s_plain, r_plain = _timed_query(bf, q, nq, K, dev, reps=1)
+    t_bf16, s_bf16, r_bf16 = _timed_query(bf, q, nq, K, dev, reps=2, form=knn.FORM_BF16)
+    bf.close()
+    out = {"workload": "BruteForceGpu<128> batchKnn<K=32>, 1,048,576 x 65,536 scaled normals, seed 2",
+           "plain_qps": round(nq / t_plain * 1e3, 1), "bf16_qps": round(nq / t_bf16 * 1e3, 1),
+           "speedup": round(t_plain / t_bf16, 3), "identical": _same(r_plain, r_bf16),
+           "survivor_fraction": s_bf16["bf16_survivor_fraction"], "overflow_reruns": s_bf16["rerun_queries"],
+           "plain_kernel_ms": round(s_plain["kernel_ms"], 3), "bf16_kernel_ms": round(s_bf16["kernel_ms"], 3),
+           "plain_tops_3D_ops": round(3 * D * nq * (1 << 20) / (s_plain["kernel_ms"] * 1e-3) / 1e12, 2),
+           "bf16_mfma_tflops_q_dot_p": round(2 * D * nq * (1 << 20) / (s_bf16["kernel_ms"] * 1e-3) / 1e12, 2)}
+    log(f"bf16 D=128: plain {t_plain:.1f} ms, bf16 {t_bf16:.1f} ms ({out['speedup']}x), identical {out['identical']}")
+    return out
+
+
+def leg_learned(dev):
+    """Learned radius vs plain on the 8-D anisotropic mixture, K=32 (kd and ball)."""
+    import torch
+
+    import cudaexactknn_amd as knn
+    from cudaexactknn_amd import datasets
+    K = 32
+    pts, qs = datasets.anisotropic_mixture_8d(1 << 22, 1 << 20, 64, 8, seed=3)
+    sample = datasets.anisotropic_mixture_8d(1 << 16, 1 << 16, 64, 8, seed=33)[1]
+    p, q = torch.from_numpy(pts).to(dev), torch.from_numpy(qs).to(dev)
+    nq = len(qs)
+    out = {"workload": "KdTreeGpu<8,8,256> / BallTreeGpu<8,256> batchKnn<K=32>, 4,194,304 x 1,048,576 "
+                       "anisotropic 8-D mixture (20 % uniform queries), seed 3; MLP trained on 65,536 samples"}
+    for kind in ("ball", "kd"):
+        tree = knn.KdTreeGpu(p, depth_skips=0) if kind == "kd" else knn.BallTreeGpu(p, depth_skips=1)
+        tr = tree.train_radius(torch.from_numpy(sample).to(dev), K)
+        t_plain, _, r_plain = _timed_query(tree, q, nq, K, dev, reps=1)
+        t_l, s_l, r_l = _timed_query(tree, q, nq, K, dev, reps=1, form=knn.FORM_LEARNED)
+        out[kind] = {"plain_qps": round(nq / t_plain * 1e3, 1), "learned_qps": round(nq / t_l * 1e3, 1),
+                     "speedup": round(t_plain / t_l, 3), "identical": _same(r_plain, r_l),
+                     "miss_rate": round(s_l["rerun_queries"] / nq, 5),
+                     "mean_predicted_over_exact_radius": round(s_l["radius_ratio"], 4),
+                     "train_ms": round(tr["train_ms"], 1), "build_ms": round(tree.build_ms(), 2)}
+        log(f"learned {kind}: plain {t_plain:.1f} ms, learned {t_l:.1f} ms, misses {s_l['rerun_queries']:.0f}, "
+            f"identical {out[kind]['identical']}")
+        tree.close()
+    return out
+
+
+def leg_batch(dev):
+    """256 uneven datasets (kd for D=3, bf16 brute force for D=16) in one batch
+    vs one plain call per dataset (per-lane kd, FP32 brute force)."""
+    import time as _t
+
+    import torch
+
+    import cudaexactknn_amd as knn
+    from cudaexactknn_amd import datasets
+    sets = datasets.uneven_batch(256, 1024, 1 << 20, seed=4)
+    items = []
+    for st in sets:
+        nq, k = len(st["queries"]), st["k"]
+        kind = knn.KIND_KD if st["dims"] == 3 else knn.KIND_BRUTE
+        items.append(knn.BatchItem(kind, torch.from_numpy(st["points"]).to(dev),
+                                   torch.from_numpy(st["queries"]).to(dev), k, *_rows(nq, k, dev)))
+    nqt = sum(len(st["queries"]) for st in sets)
+    b = knn.Batch(items)
+    b.run()
+    reps = [b.stats()]
+    for _ in range(2):
+        b.run()
+        reps.append(b.stats())
+    best = min(reps, key=lambda r: r["build_ms"] + r["query_ms"])
+    torch.cuda.synchronize()
+    t0 = _t.perf_counter()
+    identical = True
+    for st, it in zip(sets, items):
+        nq, k = len(st["queries"]), st["k"]
+        s = knn.KdTreeGpu(it.points, depth_skips=0) if it.kind == knn.KIND_KD else knn.BruteForceGpu(it.points)
+        r = _rows(nq, k, dev)
+        s.batchKnn(knn.Query(it.queries, nq, float("inf"), *r), k, per_lane=True) if it.kind == knn.KIND_KD \
+            else s.batchKnn(knn.Query(it.queries, nq, float("inf"), *r), k)
+        identical &= _same(r, (it.r_indexes, it.r_distances))
+        s.close()
+    torch.cuda.synchronize()
+    seq_ms = (_t.perf_counter() - t0) * 1e3
+    total = best["build_ms"] + best["query_ms"]
+    out = {"workload": "256 datasets, N log-uniform in [1,024, 1,048,576] (40.1M points), Q = N/4 (10.0M "
+                       "queries), U[0,1)^D, D in {3,16}, K in {4,8,16}, seed 4",
+           "batch_build_ms": round(best["build_ms"], 2), "batch_query_ms": round(best["query_ms"], 2),
+           "batch_qps_incl_build": round(nqt / total * 1e3, 1), "batch_qps_query_only": round(nqt / best["query_ms"] * 1e3, 1),
+           "sequential_plain_ms_incl_build": round(seq_ms, 1), "sequential_plain_qps": round(nqt / seq_ms * 1e3, 1),
+           "speedup_incl_build": round(seq_ms / total, 3), "identical": identical,
+           "overflow_queries": best["overflow_queries"], "work_items": best["work_items"],
+           "build_graph_nodes": best["build_graph_nodes"]}
+    log(f"batch: build {best['build_ms']:.1f} ms + query {best['query_ms']:.1f} ms vs sequential plain "
+        f"{seq_ms:.1f} ms ({out['speedup_incl_build']}x), identical {identical}")
+    b.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +246,7 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--leaf-span", type=int, default=0)
     ap.add_argument("--stack-lds", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the bf16 / learned / batch legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -180,6 +324,10 @@ def main() -> int:
     issued_bytes = ws["node_bytes"] + ws["leaf_slots"] * (4 + 4 * 3) + io_bytes
     kavg = float(np.mean(kernel_ms))
     achieved = alg_bytes / (kavg * 1e-3) / 1e9
+    # The plain HIP path: the reference's per-lane traversal shape on the same tree.
+    t_plain, s_plain, r_plain = _timed_query(kd, q, nq, K, dev, reps=2, per_lane=True)
+    plain_same = _same(r_plain, (ri, rd))
+    log(f"kd plain per-lane: {t_plain:.3f} ms, identical {plain_same}")
     log(f"kd per-query traversal: node bytes {cs['node_bytes'] / nq:.1f}, leaf slots {cs['leaf_slots'] / nq:.1f}; "
         f"algorithmic {alg_bytes / 1e9:.3f} GB per launch -> {achieved:.1f} GB/s; wave-issued "
         f"{issued_bytes / 1e9:.3f} GB")
@@ -216,6 +364,12 @@ def main() -> int:
     bf_ops = 3 * 3 * 65536 * 16384
     log(f"bf 65,536 x 16,384: kernel {bf_kernel_ms:.3f} ms, {bf_ops / bf_kernel_ms / 1e9:.1f} TFLOP/s (3D ops/pair)")
 
+    extra_legs = {}
+    if not args.no_extra:
+        extra_legs["bf16_brute_force"] = leg_bf16(dev)
+        extra_legs["learned_radius"] = leg_learned(dev)
+        extra_legs["batched"] = leg_batch(dev)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle kd tree on this host)")
@@ -224,7 +378,7 @@ def main() -> int:
             f"sample bit-exact: {cpu['sample_bit_exact_vs_gpu']}")
 
     out = {
-        "metric": "exact kNN queries/sec (kd tree, K=16, idx bit-exact)",
+        "metric": "exact kNN queries/sec, chosen form vs plain HIP on same GPU (idx bit-exact)",
         "value": round(qps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -254,6 +408,10 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "extra": {
+            "plain_hip_qps": round(nq / t_plain * 1e3, 1),
+            "plain_hip_path": "per-lane kd traversal (reference shape), same tree, same GPU",
+            "speedup_vs_plain_hip": round(qps / (nq / t_plain * 1e3), 3),
+            "plain_identical": plain_same,
             "kd_build_ms": [round(x, 2) for x in kd_build],
             "kd_order_ms": round(float(np.mean(order_ms)), 4),
             "kd_node_bytes_per_query": round(cs["node_bytes"] / nq, 2),
@@ -265,6 +423,7 @@ def main() -> int:
             "ball_identical_to_kd": ball_same,
             "bf_uniform_65536x16384_k8_kernel_ms": round(bf_kernel_ms, 4),
             "bf_uniform_tflops_3D_ops": round(bf_ops / bf_kernel_ms / 1e9, 2),
+            **extra_legs,
         },
     }
     print(json.dumps(out))

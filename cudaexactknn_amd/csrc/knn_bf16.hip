@@ -59,7 +59,7 @@ template <int D16, int KC>
 __global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t tiles_per_chunk, uint32_t stage) {
     __shared__ uint4 s_hi[bf16_stage_tiles<D16>() * D16 * 64];
     __shared__ uint4 s_lo[bf16_stage_tiles<D16>() * D16 * 64];
-    __shared__ float s_np[bf16_stage_tiles<D16>() * 32];
+    __shared__ __align__(16) float s_np[bf16_stage_tiles<D16>() * 33];
     const uint32_t t0 = blockIdx.y * tiles_per_chunk;
     const uint32_t t1 = min(S.tiles, t0 + tiles_per_chunk);
     bf16_screen_run<D16, KC>(S, blockIdx.x, t0, t1, stage, s_hi, s_lo, s_np);
@@ -163,6 +163,8 @@ Bf16Params bf16_params(uint32_t D, uint32_t D16) {
     P.up = (float)((1.0 + gf) * (1.0 + 4.0 * u) * (1.0 + 1e-7));
     P.dn = (float)((1.0 - gf) * (1.0 - 4.0 * u) * (1.0 - 1e-7));
     P.absE = std::ldexp(1.0f, -100);
+    P.inv_dn = (float)((1.0 / (1.0 - gf)) * (1.0 + 1e-6));
+    P.inv_up = (float)(1.0 / ((1.0 + gf) * (1.0 + 4.0 * u)));
     return P;
 }
 

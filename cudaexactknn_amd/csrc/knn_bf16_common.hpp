@@ -18,6 +18,15 @@
 //    the K-th smallest f) and emit every point with L <= tau (and <= maxD2
 //    when that is a non-negative number).  The true top-K always qualifies,
 //    whatever the processing order, because L <= f <= f_K <= tau.
+//    Evaluated per pair with one FMA and two compares: with E_t = cA (|q|^2 +
+//    max_tile |p|^2) + tiny >= E for every point of a 32-point tile, the
+//    emit test L <= tau becomes A <= T_t = (tau / (1 - 2 gamma_{D+2}) + E_t)
+//    (1 + 4u) (each step of T_t computed on non-negatives, so the final
+//    factor covers its roundings), since A <= d2 + E <= f / (1 - gamma) + E_t;
+//    an A that is NaN is emitted (as an L that is NaN was).  The upper bound
+//    inserted into the K smallest is (A + E_t) up >= (A + E) up >= f; whether
+//    to insert is decided on A against an approximate threshold, which only
+//    affects how tight tau gets, never correctness.
 // 4. Rerank: exact FP32 distance (src/KnnTrees.cuh:65-77) and the usual key
 //    insertion over the candidates only.  A query whose candidates overflow
 //    the buffer is recomputed by an exact FP32 scan.
@@ -45,6 +54,8 @@ __device__ __forceinline__ float bf16_val(uint32_t b) { return __uint_as_float(b
 
 struct Bf16Params {
     float cA, up, dn, absE;
+    float inv_dn;  // >= 1 / (1 - 2 gamma_{D+2}), rounded up
+    float inv_up;  // ~ 1 / up (insertion threshold only)
 };
 
 // One dataset as seen by the screen and rerank passes.
@@ -70,7 +81,7 @@ template <int D16>
 constexpr uint32_t bf16_stage_tiles() { return D16 >= 8 ? 1u : 8u / D16; }
 template <int D16>
 constexpr size_t bf16_stage_lds() {
-    return 2 * (size_t)bf16_stage_tiles<D16>() * D16 * 64 * 16 + (size_t)bf16_stage_tiles<D16>() * 32 * 4;
+    return 2 * (size_t)bf16_stage_tiles<D16>() * D16 * 64 * 16 + (size_t)bf16_stage_tiles<D16>() * 33 * 4;
 }
 
 // Screen tiles [t0, t1) for the 128 queries of `qblock`, `stage` (<=
@@ -81,6 +92,8 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                                                 uint32_t stage, uint4* __restrict__ s_hi, uint4* __restrict__ s_lo,
                                                 float* __restrict__ s_np) {
 #pragma clang fp contract(off)
+    // s_np: stage x 32 norms, then stage per-tile maxima.
+    float* __restrict__ s_npmax = s_np + bf16_stage_tiles<D16>() * 32;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t qcol = lane & 31, half = lane >> 5;
     const uint32_t qi = qblock * 128 + wave * 32 + qcol;
@@ -134,53 +147,78 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
             s_hi[i] = S.phi[base + i];
             s_lo[i] = S.plo[base + i];
         }
-        for (uint32_t i = tid; i < nt * 32; i += 256) {
-            const uint32_t p = tb * 32 + i;
-            s_np[i] = p < S.n ? S.pnorm[p] : 0.0f;
+        {
+            // nt * 32 <= 256 norms, one per thread; per-tile maxima by a
+            // 32-lane butterfly.  Norms are >= 0 or NaN; fmaxf drops NaN,
+            // which is safe: a pair with a NaN norm has A = NaN and is
+            // always emitted.
+            const uint32_t p = tb * 32 + tid;
+            float v = (tid < nt * 32 && p < S.n) ? S.pnorm[p] : 0.0f;
+            if (tid < nt * 32) s_np[tid] = v;
+#pragma unroll
+            for (int off = 16; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+            if (tid < nt * 32 && (tid & 31) == 0) s_npmax[tid >> 5] = v;
         }
         __syncthreads();
-        for (uint32_t u = 0; u < nt; ++u) {
-            const uint32_t t = tb + u;
-            f32x16 acc = {};
+        // Software-pipelined: the next tile's three MFMAs are issued before
+        // this tile's epilogue, so their latency overlaps it.
+        auto tile_dot = [&](uint32_t u) {
+            f32x16 a = {};
 #pragma unroll
             for (int kk = 0; kk < D16; ++kk) {
                 const bf16x8 ahi = __builtin_bit_cast(bf16x8, s_hi[(u * D16 + kk) * 64 + lane]);
                 const bf16x8 alo = __builtin_bit_cast(bf16x8, s_lo[(u * D16 + kk) * 64 + lane]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi[kk], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo[kk], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi[kk], acc, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi[kk], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo[kk], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi[kk], a, 0, 0, 0);
             }
+            return a;
+        };
+        f32x16 acc = tile_dot(0), acc_next = acc;
+        for (uint32_t u = 0; u < nt; ++u) {
+            const uint32_t t = tb + u;
+            if (u + 1 < nt) acc_next = tile_dot(u + 1);
             if (active) {
+                // This lane's 16 rows are 8j + 4 half + i (j, i < 4): four
+                // 16-byte LDS reads, all issued before any use.
+                float4 n4[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    n4[j] = *reinterpret_cast<const float4*>(s_np + u * 32 + 8 * j + 4 * half);
+                const float Et = S.P.cA * (qn + s_npmax[u]) + S.P.absE;
+                const float T = (tau * S.P.inv_dn + Et) * 0x1.000004p+0f;  // (1 + 4u)
+                const float Ithr = lane_tau * S.P.inv_up - Et;
+                const uint32_t nvalid = S.n - t * 32;
+                float A[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const float4 v = n4[r >> 2];
+                    const float np = (r & 3) == 0 ? v.x : (r & 3) == 1 ? v.y : (r & 3) == 2 ? v.z : v.w;
+                    A[r] = __builtin_fmaf(-2.0f, acc[r], qn + np);  // == (qn + np) - 2 acc, rounded once
+                }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
-                    const uint32_t pt = t * 32 + row;
-                    if (pt < S.n) {
-                        const float sumn = qn + s_np[u * 32 + row];
-                        const float A = sumn - 2.0f * acc[r];
-                        const float E = S.P.cA * sumn + S.P.absE;
-                        const float L = (A - E) * S.P.dn;
-                        const float U = (A + E) * S.P.up;
-                        if (!(L > tau)) {
-                            const uint32_t pos = atomicAdd(&S.cnt[qi], 1u);
-                            if (pos < S.cap) S.cand[(size_t)qi * S.cap + pos] = pt;
-                        }
-                        if (U < lane_tau) {
-                            float key = U;
+                    if (!(A[r] > T) && row < nvalid) {
+                        const uint32_t pos = atomicAdd(&S.cnt[qi], 1u);
+                        if (pos < S.cap) S.cand[(size_t)qi * S.cap + pos] = t * 32 + row;
+                    }
+                    if (A[r] < Ithr && row < nvalid) {
+                        float key = (A[r] + Et) * S.P.up;
 #pragma unroll
-                            for (int i = 0; i < KC; ++i) {
-                                const float e = ul[i];
-                                ul[i] = fminf(e, key);
-                                key = fmaxf(e, key);
-                            }
-                            lane_tau = ul[KC - 1];
+                        for (int i = 0; i < KC; ++i) {
+                            const float e = ul[i];
+                            ul[i] = fminf(e, key);
+                            key = fmaxf(e, key);
                         }
+                        lane_tau = ul[KC - 1];
                     }
                 }
             }
             // Both lanes of a query hold upper bounds of the same K-th distance.
             const float other = __shfl_xor(lane_tau, 32, 64);
             tau = fminf(fminf(lane_tau, other), maxcap);
+            acc = acc_next;
         }
     }
 }
