@@ -138,11 +138,11 @@ __device__ __forceinline__ void batch_bf16_item(const Bf16Set& S, uint32_t qbloc
         uint4* s_lo = s_hi + bf16_stage_tiles<D16>() * D16 * 64;
         float* s_np = reinterpret_cast<float*>(s_lo + bf16_stage_tiles<D16>() * D16 * 64);
         if (S.K <= 8)
-            bf16_screen_run<D16, 8>(S, qblock, t0, t1, stage, s_hi, s_lo, s_np);
+            bf16_screen_run<D16, 8>(S, qblock, 0, t0, t1, stage, s_hi, s_lo, s_np);
         else if (KMAX <= 16 || S.K <= 16)
-            bf16_screen_run<D16, 16>(S, qblock, t0, t1, stage, s_hi, s_lo, s_np);
+            bf16_screen_run<D16, 16>(S, qblock, 0, t0, t1, stage, s_hi, s_lo, s_np);
         else if constexpr (KMAX > 16)
-            bf16_screen_run<D16, 32>(S, qblock, t0, t1, stage, s_hi, s_lo, s_np);
+            bf16_screen_run<D16, 32>(S, qblock, 0, t0, t1, stage, s_hi, s_lo, s_np);
     }
 }
 
@@ -360,13 +360,15 @@ static uint32_t ceil_log2_u(uint64_t x) {
     return r;
 }
 
-// Candidate-list capacity for a bf16 dataset: the running K-th bound admits
-// about K (1 + ln(N / K)) points over one pass, doubled, plus slack.
+// Capacity of each of a bf16 query's two candidate lists (one per half-wave
+// lane, n / 2 points each): the running K-th bound admits about
+// K (1 + ln(m / K)) of m points over one pass; doubled, plus slack.
 static uint32_t bf16_cap(uint32_t n, uint32_t K) {
-    const double e = 2.0 * K * (1.0 + std::log(std::max(1.0, (double)n / K))) + 64.0;
+    const double m = std::max(1.0, (double)n / 2.0 / K);
+    const double e = 2.0 * K * (1.0 + std::log(m)) + 64.0;
     uint32_t c = (uint32_t)std::min(2048.0, e);
     c = (c + 15) & ~15u;
-    return std::min(c, std::max(n, 16u));
+    return std::min(c, std::max(((n + 1) / 2 + 15) & ~15u, 16u));
 }
 
 static void validate(const knn_batch_item& it, uint32_t i) {
@@ -472,15 +474,16 @@ static void plan(Batch& b) {
         S.maxd2 = it.max_d2;
         S.P = bf16_params(it.dims, D16);
         S.cap = bf16_cap(it.n, it.k);
+        S.lists = 2;
         S.oidx = it.d_idx;
         S.odist = it.d_dist;
-        cnt_off[s] = b.bf_total;
+        cnt_off[s] = (size_t)b.bf_total * 2;
         cand_off[s] = cand_words;
         b.bf_total += it.nq;
-        cand_words += (size_t)it.nq * S.cap;
+        cand_words += (size_t)it.nq * 2 * S.cap;
         b.bfs.push_back(std::move(bb));
     }
-    b.cnt.alloc(std::max<size_t>(b.bf_total, 1));
+    b.cnt.alloc(std::max<size_t>((size_t)b.bf_total * 2, 1));
     b.cand.alloc(std::max<size_t>(cand_words, 1));
     b.ovf.alloc(std::max<size_t>(b.bf_total, 1));
     for (uint32_t s = 0; s < n_bf; ++s) {
@@ -713,7 +716,6 @@ static void plan(Batch& b) {
     try {
         hipStream_t st = b.cap;
         launch_fill_u32(b.ctr.get(), nl + 2, 0u, st);
-        if (b.bf_total) launch_fill_u32(b.cnt.get(), b.bf_total, 0u, st);
         if (b.tree_total) {
             hipLaunchKernelGGL(batch_order_kernel, dim3(b.tree_total_padded / 256), dim3(256), 0, st,
                                b.d_osets.get(), b.d_blk2set.get(), b.tree_total_padded, b.keys.get(), b.vals.get());

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t ti
     __shared__ __align__(16) float s_np[bf16_stage_tiles<D16>() * 33];
     const uint32_t t0 = blockIdx.y * tiles_per_chunk;
     const uint32_t t1 = min(S.tiles, t0 + tiles_per_chunk);
-    bf16_screen_run<D16, KC>(S, blockIdx.x, t0, t1, stage, s_hi, s_lo, s_np);
+    bf16_screen_run<D16, KC>(S, blockIdx.x, blockIdx.y, t0, t1, stage, s_hi, s_lo, s_np);
 }
 
 // Exact rerank of the candidates of each query (one lane per query); queries
@@ -183,18 +183,6 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     const uint4* phi = reinterpret_cast<const uint4*>(h.pts_bf16.get());
     const uint4* plo = phi + words;
 
-    uint32_t cap = 2048;
-    while (cap > 64 && (size_t)a.nq * cap * 4 > ((size_t)1 << 30)) cap /= 2;
-    // scratch: cnt[nq] | cand[nq*cap] | ovf[nq] | novf
-    const size_t need = ((size_t)a.nq * (cap + 2) + 4) * 4;
-    h.q_scratch.ensure(need);
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(h.q_scratch.get());
-    uint32_t* cand = cnt + a.nq;
-    uint32_t* ovf = cand + (size_t)a.nq * cap;
-    uint32_t* novf = ovf + a.nq;
-    launch_fill_u32(cnt, a.nq, 0u, a.stream);
-    launch_fill_u32(novf, 1, 0u, a.stream);
-
     const uint32_t qblocks = (a.nq + 127) / 128;
     const uint32_t want = 4 * device_limits().cu_count;
     uint32_t S_ = 1;
@@ -205,6 +193,23 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
         if (S_ > 64) S_ = 64;
     }
     const uint32_t tpc = (tiles + S_ - 1) / S_;
+    // One candidate list per (query, chunk, half-wave lane).  A lane that sees
+    // m points admits about K (1 + ln(m / K)) of them while its running bound
+    // tightens; the capacity doubles that, plus slack.
+    const uint32_t lists = 2 * S_;
+    const double m = std::max(1.0, (double)n / (2.0 * S_) / a.K);
+    uint32_t cap = (uint32_t)std::min(2048.0, 2.0 * a.K * (1.0 + std::log(m)) + 64.0);
+    cap = (cap + 15) & ~15u;
+    while (cap > 64 && (size_t)a.nq * lists * cap * 4 > ((size_t)1 << 30)) cap /= 2;
+    // scratch: cnt[nq * lists] | cand[nq * lists * cap] | ovf[nq] | novf
+    const size_t need = ((size_t)a.nq * lists * (cap + 1) + a.nq + 4) * 4;
+    h.q_scratch.ensure(need);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(h.q_scratch.get());
+    uint32_t* cand = cnt + (size_t)a.nq * lists;
+    uint32_t* ovf = cand + (size_t)a.nq * lists * cap;
+    uint32_t* novf = ovf + a.nq;
+    launch_fill_u32(novf, 1, 0u, a.stream);
+
     Bf16Set S{};
     S.phi = phi;
     S.plo = plo;
@@ -221,6 +226,7 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     S.cnt = cnt;
     S.cand = cand;
     S.cap = cap;
+    S.lists = lists;
     S.oidx = a.oidx;
     S.odist = a.odist;
     ScreenFn fn = nullptr;
@@ -244,8 +250,8 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     h.stats[4] = timer.stop();
     if (a.timed) {
         // survivor fraction: candidates per (query, point) pair
-        std::vector<uint32_t> hc(a.nq);
-        KNN_HIP(hipMemcpy(hc.data(), cnt, 4ull * a.nq, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> hc((size_t)a.nq * lists);
+        KNN_HIP(hipMemcpy(hc.data(), cnt, 4ull * a.nq * lists, hipMemcpyDeviceToHost));
         double tot = 0;
         for (uint32_t v : hc) tot += v;
         h.stats[1] = (float)(tot / ((double)a.nq * n));

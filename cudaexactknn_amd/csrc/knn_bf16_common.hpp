@@ -14,7 +14,7 @@
 //    errors (gamma_{D+1}), plus 25 % slack; and the reference's computed
 //    distance f differs from d2 by at most gamma_{D+2} d2.  So
 //    L = (A - E) dn <= f <= (A + E) up = U.
-// 3. Per query, the lanes keep the K smallest U seen (an upper bound tau of
+// 3. Per query, each of its two lanes keeps the K smallest U it has seen (an upper bound tau of
 //    the K-th smallest f) and emit every point with L <= tau (and <= maxD2
 //    when that is a non-negative number).  The true top-K always qualifies,
 //    whatever the processing order, because L <= f <= f_K <= tau.
@@ -69,9 +69,13 @@ struct Bf16Set {
     uint32_t nq;
     float maxd2;
     Bf16Params P;
-    uint32_t* cnt;   // per query candidate count
-    uint32_t* cand;  // per query candidate list, `cap` slots
-    uint32_t cap;
+    // Candidate lists: one per (query, point chunk, half-wave lane), so a lane
+    // appends with a register counter and a plain store (no atomics).  List
+    // l of query q is cand[(q * lists + l) * cap ...], its count cnt[q * lists + l].
+    uint32_t* cnt;
+    uint32_t* cand;
+    uint32_t cap;    // slots per list
+    uint32_t lists;  // 2 x point chunks
     int32_t* oidx;
     float* odist;
 };
@@ -88,9 +92,9 @@ constexpr size_t bf16_stage_lds() {
 // bf16_stage_tiles<D16>()) tiles per barrier pair.  Every thread of the
 // 256-wide workgroup must call it (it synchronises the workgroup).
 template <int D16, int KC>
-__device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qblock, uint32_t t0, uint32_t t1,
-                                                uint32_t stage, uint4* __restrict__ s_hi, uint4* __restrict__ s_lo,
-                                                float* __restrict__ s_np) {
+__device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qblock, uint32_t chunk, uint32_t t0,
+                                                uint32_t t1, uint32_t stage, uint4* __restrict__ s_hi,
+                                                uint4* __restrict__ s_lo, float* __restrict__ s_np) {
 #pragma clang fp contract(off)
     // s_np: stage x 32 norms, then stage per-tile maxima.
     float* __restrict__ s_npmax = s_np + bf16_stage_tiles<D16>() * 32;
@@ -137,6 +141,10 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
     for (int i = 0; i < KC; ++i) ul[i] = (i >= KC - (int)S.K) ? INFINITY : -INFINITY;
     float lane_tau = INFINITY;
     float tau = maxcap;
+    // This lane's candidate list and its length.
+    const size_t list = (size_t)qi * S.lists + chunk * 2 + half;
+    uint32_t* __restrict__ mine = S.cand + list * S.cap;
+    uint32_t count = 0;
 
     for (uint32_t tb = t0; tb < t1; tb += stage) {
         const uint32_t nt = min(stage, t1 - tb);
@@ -200,8 +208,8 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                 for (int r = 0; r < 16; ++r) {
                     const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
                     if (!(A[r] > T) && row < nvalid) {
-                        const uint32_t pos = atomicAdd(&S.cnt[qi], 1u);
-                        if (pos < S.cap) S.cand[(size_t)qi * S.cap + pos] = t * 32 + row;
+                        if (count < S.cap) mine[count] = t * 32 + row;
+                        ++count;
                     }
                     if (A[r] < Ithr && row < nvalid) {
                         float key = (A[r] + Et) * S.P.up;
@@ -221,27 +229,33 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
             acc = acc_next;
         }
     }
+    if (active) S.cnt[list] = count;
 }
 
 // Exact rerank of query qi's candidates (array-mode or heap-mode K-buffer
 // per KIND).  Returns false, writing nothing, when the candidates overflowed.
 template <int KIND>
 __device__ __forceinline__ bool bf16_rerank_one(const Bf16Set& S, uint32_t qi) {
-    const uint32_t c = S.cnt[qi];
-    if (c > S.cap) return false;
+    const uint32_t* cnt = S.cnt + (size_t)qi * S.lists;
+    for (uint32_t l = 0; l < S.lists; ++l)
+        if (cnt[l] > S.cap) return false;
     typename BufOf<KIND>::T buf;
     const uint64_t empty = ((uint64_t)f2u(S.maxd2) << 32) | kEmptyLo;
     uint64_t worst = buf.init(empty, S.K);
     uint32_t whi = key_hi(worst);
     const float* q = S.qs + (size_t)qi * S.D;
-    for (uint32_t j = 0; j < c; ++j) {
-        const uint32_t pt = S.cand[(size_t)qi * S.cap + j];
-        const float d2 = dist2_rt(q, 1, S.pts + (size_t)pt * S.D, S.D);
-        if (f2u(d2) <= whi) {
-            const uint64_t key = make_key(d2, pt);
-            if (key < worst) {
-                worst = buf.consider(key);
-                whi = key_hi(worst);
+    for (uint32_t l = 0; l < S.lists; ++l) {
+        const uint32_t* list = S.cand + ((size_t)qi * S.lists + l) * S.cap;
+        const uint32_t c = cnt[l];
+        for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t pt = list[j];
+            const float d2 = dist2_rt(q, 1, S.pts + (size_t)pt * S.D, S.D);
+            if (f2u(d2) <= whi) {
+                const uint64_t key = make_key(d2, pt);
+                if (key < worst) {
+                    worst = buf.consider(key);
+                    whi = key_hi(worst);
+                }
             }
         }
     }
