@@ -61,7 +61,7 @@ def candidates(kind: str, dims: int, k: int) -> list[dict]:
 
 def _default_span(kind: str, info: dict, k: int, per_lane: bool) -> int:
     """The library's default leaf span (knn_capi.hip default_span)."""
-    target = 2 * k if (kind == "kd" and not per_lane) else k
+    target = k
     mc = max(1, info["max_children"])
     s = 0 if target <= mc else math.ceil(math.log2((target + mc - 1) // mc))
     return min(s, info["node_depth"])

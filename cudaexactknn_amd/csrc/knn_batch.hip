@@ -543,7 +543,7 @@ static void plan(Batch& b) {
         A.sd_inv = (65536u + I.split_dims - 1) / I.split_dims;
         uint32_t span = b.opts.leaf_span;
         if (span == 0) {
-            const uint64_t target = h.kind == KNN_KIND_KD ? 2ull * it.k : (uint64_t)it.k;
+            const uint64_t target = it.k;
             const uint64_t mc = I.max_children;
             span = target > mc ? ceil_log2_u((target + mc - 1) / mc) : 0;
         } else if (span == 0xFFFFFFFFu) {

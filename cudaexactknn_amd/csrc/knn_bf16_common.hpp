@@ -37,6 +37,8 @@
 // for its 16 accumulators r.  A workgroup is 4 waves = 128 queries and
 // stages up to 8 point tiles (of 32 points) in LDS per barrier pair.
 #pragma once
+#include <type_traits>
+
 #include "knn_device.hpp"
 
 namespace knn {
@@ -195,8 +197,7 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                     n4[j] = *reinterpret_cast<const float4*>(s_np + u * 32 + 8 * j + 4 * half);
                 const float Et = S.P.cA * (qn + s_npmax[u]) + S.P.absE;
                 const float T = (tau * S.P.inv_dn + Et) * 0x1.000004p+0f;  // (1 + 4u)
-                const float Ithr = lane_tau * S.P.inv_up - Et;
-                const uint32_t nvalid = S.n - t * 32;
+                float Ithr = lane_tau * S.P.inv_up - Et;
                 float A[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
@@ -204,28 +205,45 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                     const float np = (r & 3) == 0 ? v.x : (r & 3) == 1 ? v.y : (r & 3) == 2 ? v.z : v.w;
                     A[r] = __builtin_fmaf(-2.0f, acc[r], qn + np);  // == (qn + np) - 2 acc, rounded once
                 }
+                // Rows at or beyond n exist only in the last tile: the row
+                // test is compiled into a separate copy of the loop, chosen by
+                // a tile-uniform branch.
+                const uint32_t nvalid = S.n - t * 32;
+                auto pairs = [&](auto all_valid) {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
-                    if (!(A[r] > T) && row < nvalid) {
-                        if (count < S.cap) mine[count] = t * 32 + row;
-                        ++count;
-                    }
-                    if (A[r] < Ithr && row < nvalid) {
-                        float key = (A[r] + Et) * S.P.up;
-#pragma unroll
-                        for (int i = 0; i < KC; ++i) {
-                            const float e = ul[i];
-                            ul[i] = fminf(e, key);
-                            key = fmaxf(e, key);
+                    for (int r = 0; r < 16; ++r) {
+                        const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
+                        const bool valid = decltype(all_valid)::value || row < nvalid;
+                        if (!(A[r] > T) && valid) {
+                            if (count < S.cap) mine[count] = t * 32 + row;
+                            ++count;
                         }
-                        lane_tau = ul[KC - 1];
+                        if (A[r] < Ithr && valid) {
+                            float key = (A[r] + Et) * S.P.up;
+#pragma unroll
+                            for (int i = 0; i < KC; ++i) {
+                                const float e = ul[i];
+                                ul[i] = fminf(e, key);
+                                key = fmaxf(e, key);
+                            }
+                            lane_tau = ul[KC - 1];
+                            Ithr = lane_tau * S.P.inv_up - Et;
+                        }
                     }
-                }
+                };
+                if (nvalid >= 32)
+                    pairs(std::true_type{});
+                else
+                    pairs(std::false_type{});
             }
-            // Both lanes of a query hold upper bounds of the same K-th distance.
-            const float other = __shfl_xor(lane_tau, 32, 64);
-            tau = fminf(fminf(lane_tau, other), maxcap);
+            // Both lanes of a query hold upper bounds of the same K-th
+            // distance: swap halves (lane l <-> l ^ 32) without LDS; the min
+            // of the two swapped copies is min(own, other) on every lane.
+            {
+                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(lane_tau),
+                                                                 __float_as_uint(lane_tau), false, false);
+                tau = fminf(fminf(__uint_as_float(sw[0]), __uint_as_float(sw[1])), maxcap);
+            }
             acc = acc_next;
         }
     }

@@ -57,7 +57,7 @@ static hipStream_t null_stream() { return nullptr; }
 // measured on the 16.7M-point / 4.2M-query 3-D mixture with K = 16:
 //  - trees: one wave per workgroup (the wave-cooperative traversal keeps its
 //    stack per wave; 64 measured fastest);
-//  - kd: leaf groups of ~2K points, queries in tree-cell order;
+//  - kd: leaf groups of ~K points, queries in tree-cell order;
 //  - ball: leaf groups of ~K points, queries in Morton order;
 //  - per-lane traversal: groups of ~K points.
 static uint32_t default_block(const Handle& h) { return h.kind == KNN_KIND_BRUTE ? 256 : 64; }
@@ -68,7 +68,8 @@ static uint32_t ceil_log2(uint64_t x) {
 }
 static uint32_t default_span(const Handle& h, uint32_t K, bool per_lane) {
     if (h.kind == KNN_KIND_BRUTE || h.info.max_children == 0) return 0;
-    const uint64_t target = (h.kind == KNN_KIND_KD && !per_lane) ? 2ull * K : (uint64_t)K;
+    (void)per_lane;
+    const uint64_t target = K;
     const uint64_t mc = h.info.max_children;
     uint32_t s = target > mc ? ceil_log2((target + mc - 1) / mc) : 0;
     if (s > h.info.node_depth) s = h.info.node_depth;
