@@ -310,6 +310,21 @@ def main() -> int:
     kd_idx = ri.cpu().numpy()
     kd_dist = rd.cpu().numpy()
 
+    # Host-to-host rate (queries uploaded from and results downloaded to
+    # pinned host memory, overlapped in chunks): the PCIe-inclusive figure.
+    pq = knn.pinned_array(qs.shape, np.float32)
+    pq.array[:] = qs
+    pi = knn.pinned_array((nq, K), np.int32)
+    pd = knn.pinned_array((nq, K), np.float32)
+    kd.batchKnnHost(pq.array, K, r_indexes=pi.array, r_distances=pd.array, **opts)
+    t0 = time.perf_counter()
+    kd.batchKnnHost(pq.array, K, r_indexes=pi.array, r_distances=pd.array, **opts)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    host_same = bool(np.array_equal(pi.array, kd_idx) and np.array_equal(pd.array.view(np.uint32), kd_dist.view(np.uint32)))
+    for a in (pq, pi, pd):
+        a.close()
+    log(f"kd host-to-host (pinned, chunked): {host_ms:.2f} ms, {nq / host_ms / 1e3:.2f} M q/s, identical {host_same}")
+
     # Untimed counting runs.  Algorithmic bytes follow the reference's
     # per-query traversal (per-lane DFS, src/KdTreeGpu.cuh:37-90): the nodes
     # and leaf slots each query's own search reads, plus query, order index
@@ -412,6 +427,8 @@ def main() -> int:
             "plain_hip_path": "per-lane kd traversal (reference shape), same tree, same GPU",
             "speedup_vs_plain_hip": round(qps / (nq / t_plain * 1e3), 3),
             "plain_identical": plain_same,
+            "kd_host_to_host_qps_pcie_inclusive": round(nq / host_ms * 1e3, 1),
+            "kd_host_to_host_identical": host_same,
             "kd_build_ms": [round(x, 2) for x in kd_build],
             "kd_order_ms": round(float(np.mean(order_ms)), 4),
             "kd_node_bytes_per_query": round(cs["node_bytes"] / nq, 2),

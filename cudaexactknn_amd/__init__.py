@@ -28,7 +28,7 @@ from ._lib import KnnError, QueryOpts, TreeInfo  # noqa: F401
 __all__ = [
     "BruteForceGpu", "KdTreeGpu", "BallTreeGpu", "Query", "KnnError", "pinned_malloc", "pinned_free",
     "FORM_PLAIN", "FORM_BF16", "FORM_LEARNED", "library_path", "version", "Batch", "BatchItem",
-    "KIND_BRUTE", "KIND_KD", "KIND_BALL",
+    "KIND_BRUTE", "KIND_KD", "KIND_BALL", "pinned_array",
 ]
 
 FORM_PLAIN, FORM_BF16, FORM_LEARNED = _lib.KNN_FORM_PLAIN, _lib.KNN_FORM_BF16, _lib.KNN_FORM_LEARNED
@@ -90,6 +90,28 @@ class _Structure:
     @property
     def handle(self) -> int:
         return self._h.value or 0
+
+    def batchKnnHost(self, queries: Any, K: int, max_distance2: float = float("inf"), r_indexes: Any = None,
+                     r_distances: Any = None, chunk: int = 0, **opts) -> tuple:
+        """Queries in host memory (numpy float32 [q][D]); results returned in
+        (or written to) host int32 / float32 [q][K] arrays.  Uploads, searches
+        and downloads overlap across chunks (knn_query_host).  Arrays in
+        pinned memory (pinned_array) let the copies run concurrently."""
+        import numpy as np
+        qs = np.ascontiguousarray(queries, dtype=np.float32)
+        if qs.ndim != 2 or qs.shape[1] != self.dims:
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"queries must be [q][{self.dims}] float32")
+        q = qs.shape[0]
+        ri = np.empty((q, K), np.int32) if r_indexes is None else r_indexes
+        rd = np.empty((q, K), np.float32) if r_distances is None else r_distances
+        for a, dt in ((ri, np.int32), (rd, np.float32)):
+            if a.dtype != dt or a.shape != (q, K) or not a.flags.c_contiguous:
+                raise KnnError(_lib.KNN_ERR_INVALID_ARG, "result arrays must be C-contiguous [q][K]")
+        o = _opts(opts.pop("BLOCK_SIZE", 0), opts.pop("USE_HEAP_TRACKING", False), opts.pop("USE_TRACKING_SHMEM", False),
+                  opts.pop("USE_STACK_SHMEM", False), **opts)
+        _lib.check(_lib.load().knn_query_host(self._h, qs.ctypes.data, q, float(max_distance2), ri.ctypes.data,
+                                              rd.ctypes.data, int(K), ctypes.byref(o), int(chunk)))
+        return ri, rd
 
     def info(self) -> dict:
         ti = TreeInfo()
@@ -196,6 +218,39 @@ class _Tree(_Structure):
         leaf_points = leaves[:, 1:].copy()
         return nodes, leaf_index, leaf_points, dims[: i["node_size"]]
 
+    def serialize(self) -> bytes:
+        """The tree in the library's stable serialized format (knn_tree_serialize)."""
+        L = _lib.load()
+        n = ctypes.c_size_t(0)
+        _lib.check(L.knn_tree_serialized_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _lib.check(L.knn_tree_serialize(self._h, buf, n.value))
+        return buf.raw
+
+    def save(self, path: str) -> None:
+        with open(path, "wb") as f:
+            f.write(self.serialize())
+
+    @staticmethod
+    def from_bytes(data: bytes) -> "_Tree":
+        """A KdTreeGpu or BallTreeGpu restored from serialize() output."""
+        h = ctypes.c_void_p(0)
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _lib.check(_lib.load().knn_tree_deserialize(buf, len(data), ctypes.byref(h)))
+        ti = TreeInfo()
+        _lib.check(_lib.load().knn_tree_get_info(h, ctypes.byref(ti)))
+        cls = KdTreeGpu if ti.kind == _lib.KNN_KIND_KD else BallTreeGpu
+        t = cls.__new__(cls)
+        _Structure.__init__(t)
+        t._h = h
+        t.dims, t.size = ti.dims, ti.n
+        return t
+
+    @staticmethod
+    def load(path: str) -> "_Tree":
+        with open(path, "rb") as f:
+            return _Tree.from_bytes(f.read())
+
     def train_radius(self, sample: Any, K: int, epochs: int = 0, lr: float = 0.0) -> dict:
         """Train the learned-radius predictor (KNN_FORM_LEARNED) on sample queries."""
         n, d = _shape2(sample, "sample")
@@ -242,6 +297,31 @@ def pinned_malloc(nbytes: int) -> int:
 def pinned_free(ptr: int) -> None:
     """pinnedFree (src/KnnTrees.cuh:240-243)."""
     _lib.check(_lib.load().knn_pinned_free(ctypes.c_void_p(ptr)))
+
+
+class pinned_array:
+    """A numpy array in page-locked host memory (knn_pinned_alloc), freed by
+    close() or on garbage collection."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        self.dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * self.dtype.itemsize
+        self.ptr = pinned_malloc(max(1, n))
+        buf = (ctypes.c_uint8 * max(1, n)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def close(self) -> None:
+        if self.ptr:
+            self.array = None
+            pinned_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 KIND_BRUTE, KIND_KD, KIND_BALL = _lib.KNN_KIND_BRUTE, _lib.KNN_KIND_KD, _lib.KNN_KIND_BALL

@@ -176,6 +176,11 @@ __global__ void __launch_bounds__(PT_THREADS) part_count_kernel(const uint32_t* 
     }
 }
 
+// Items are taken in striped order (item i*256 + tid of the tile), so every
+// load and store instruction covers 64 consecutive positions.  An item's
+// left-rank = tile base (scan of part_count) + left items of earlier
+// (iteration, wave) slots of the tile (a 64-entry prefix table) + its rank
+// inside its wave's ballot.
 __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ Pout,
                                                                   uint32_t N, uint32_t skip,
                                                                   const uint8_t* __restrict__ flag,
@@ -185,40 +190,56 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
                                                                   const uint32_t* __restrict__ segN,
                                                                   const uint32_t* __restrict__ leftBefore,
                                                                   const uint32_t* __restrict__ Lcount, uint32_t S) {
-    __shared__ uint32_t lw[PT_THREADS / 64 + 1];
+    static_assert(PT_THREADS == 256 && PT_ITEMS * (PT_THREADS / 64) == 64, "one prefix-table entry per lane");
+    __shared__ uint32_t wcnt[PT_ITEMS * (PT_THREADS / 64)];
     const uint32_t y = blockIdx.y;
     const size_t lo = (size_t)list_of(y, skip) * N;
     const uint32_t* L = P + lo;
     uint32_t* O = Pout + lo;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Every list holds the same number of left points, so list y's block
     // offsets in the concatenated scan start at y * totalLeft.
     const uint32_t totalLeft = leftBefore[S - 1] + Lcount[S - 1];
-    const size_t base = (size_t)blockIdx.x * PT_TILE + (size_t)threadIdx.x * PT_ITEMS;
+    const size_t base = (size_t)blockIdx.x * PT_TILE;
     uint32_t id[PT_ITEMS];
-    uint32_t fl = 0, cnt = 0;
+    uint32_t rank[PT_ITEMS];  // rank among this wave's left items of the same iteration
+    uint32_t fl = 0;
 #pragma unroll
     for (int i = 0; i < PT_ITEMS; ++i) {
-        const size_t j = base + i;
+        const size_t j = base + (size_t)i * PT_THREADS + tid;
         id[i] = j < N ? L[j] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + (size_t)i * PT_THREADS + tid;
         const uint32_t f = j < N ? flag[id[i]] : 0u;
         fl |= f << i;
-        cnt += f;
+        const uint64_t m = __ballot(f != 0);
+        rank[i] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (lane == 0) wcnt[i * (PT_THREADS / 64) + wave] = (uint32_t)__popcll(m);
     }
-    uint32_t total;
-    uint32_t g = block_excl_scan(cnt, lw, &total);
-    g += off[(size_t)y * nb + blockIdx.x] - y * totalLeft;
+    __syncthreads();
+    if (wave == 0) {
+        // Exclusive prefix over the 64 (iteration, wave) slots in item order.
+        const uint32_t v = wcnt[lane];
+        const uint32_t inc = wave_incl_scan(v);
+        __builtin_amdgcn_wave_barrier();
+        wcnt[lane] = inc - v;
+    }
+    __syncthreads();
+    const uint32_t g0 = off[(size_t)y * nb + blockIdx.x] - y * totalLeft;
 #pragma unroll
     for (int i = 0; i < PT_ITEMS; ++i) {
-        const size_t j = base + i;
+        const size_t j = base + (size_t)i * PT_THREADS + tid;
         if (j < N) {
             const uint32_t s = segOf[j];
             const uint32_t st = segStart[s];
             const uint32_t Ls = (segN[s] + 1) / 2;
+            const uint32_t g = g0 + wcnt[i * (PT_THREADS / 64) + wave] + rank[i];
             const uint32_t lb = g - leftBefore[s];
             const bool f = (fl >> i) & 1u;
             const uint32_t pos = f ? st + lb : st + Ls + ((uint32_t)j - st - lb);
             O[pos] = id[i];
-            g += f ? 1u : 0u;
         }
     }
 }

@@ -2,6 +2,7 @@
 // Each converts C++ exceptions into a status plus a thread-local message, the
 // C-ABI counterpart of the reference's Cuda::check -> CudaException
 // (src/KnnTrees.cuh:20-23, 197-202).
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 
@@ -74,6 +75,138 @@ static uint32_t default_span(const Handle& h, uint32_t K, bool per_lane) {
     uint32_t s = target > mc ? ceil_log2((target + mc - 1) / mc) : 0;
     if (s > h.info.node_depth) s = h.info.node_depth;
     return s;
+}
+
+static void check_query(const Handle& h, uint32_t k, const knn_query_opts& o) {
+    require(k >= 1, "k must be >= 1");
+    require(k <= 4096, "k above 4096 is not supported", KNN_ERR_UNSUPPORTED);
+    require(o.form <= KNN_FORM_LEARNED, "unknown form");
+    require(!(o.form == KNN_FORM_BF16 && h.kind != KNN_KIND_BRUTE), "the bf16 form applies to brute force only");
+    require(!(o.form == KNN_FORM_LEARNED && h.kind == KNN_KIND_BRUTE), "the learned form applies to trees only");
+}
+
+// Options -> resolved query parameters (tuned defaults for zero fields).
+static QueryArgs resolve_args(const Handle& h, const float* d_q, uint32_t q, float max_d2, int32_t* d_idx,
+                              float* d_dist, uint32_t k, const knn_query_opts& o) {
+    QueryArgs a{};
+    a.q = d_q;
+    a.nq = q;
+    a.maxd2 = max_d2;
+    a.oidx = d_idx;
+    a.odist = d_dist;
+    a.K = k;
+    a.block = o.block ? o.block : default_block(h);
+    require(a.block <= device_limits().max_threads, "BLOCK_SIZE above the device maximum");
+    a.heap = o.heap != 0;
+    a.track_lds = o.track_lds != 0;
+    a.stack_lds = o.stack_lds != 0;
+    a.sort_queries = (o.flags & KNN_FLAG_NO_QUERY_SORT) == 0;
+    a.count_visits = (o.flags & KNN_FLAG_COUNT_VISITS) != 0;
+    a.per_lane = (o.flags & KNN_FLAG_PER_LANE) != 0;
+    a.xcd_remap = (o.flags & KNN_FLAG_NO_XCD_REMAP) == 0;
+    a.order_morton = (o.flags & KNN_FLAG_MORTON_ORDER) != 0 || h.kind == KNN_KIND_BALL;
+    a.leaf_span = o.leaf_span == 0 ? default_span(h, k, a.per_lane) : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
+    a.form = (knn_form)o.form;
+    a.tile = o.tile;
+    a.stream = (hipStream_t)o.stream;
+    return a;
+}
+
+static void dispatch_query(Handle& h, const QueryArgs& a) {
+    if (h.kind == KNN_KIND_BRUTE) {
+        if (a.form == KNN_FORM_BF16)
+            query_bf_bf16(h, a);
+        else
+            query_bf(h, a);
+    } else if (a.form == KNN_FORM_LEARNED) {
+        query_tree_learned(h, a);
+    } else {
+        uint32_t* perm = nullptr;
+        if (a.sort_queries && a.nq > 1) order_queries(h, a, &perm);
+        query_tree(h, a, perm, nullptr, nullptr);
+    }
+}
+
+// Host-resident queries in chunks.  Three streams: copies in, search, copies
+// out; two device slots alternate, so chunk i+1 is uploaded and chunk i-1
+// downloaded while chunk i is searched.  The search of every chunk runs on
+// the one compute stream, because a handle's scratch buffers serve one query
+// at a time.  Event order per chunk i (slot s = i % 2):
+//   up:   wait search(i-2) done  -> copy q_i into slot s -> record up[s]
+//   comp: wait up[s], wait down(i-2) done -> search -> record comp[s]
+//   down: wait comp[s] -> copy results out of slot s -> record down[s]
+static void host_pipeline(Handle& h, const float* h_q, uint32_t q, float max_d2, int32_t* h_idx, float* h_dist,
+                          uint32_t k, const knn_query_opts& o, uint32_t chunk) {
+    const uint32_t D = h.info.dims;
+    if (chunk == 0) chunk = 1u << 20;
+    if (chunk > q) chunk = q;
+    DevBuf<float> dq[2], dd[2];
+    DevBuf<int32_t> di[2];
+    // Declared after the buffers so that it is destroyed first: its destructor
+    // drains the streams before the buffers are freed.
+    struct Res {
+        hipStream_t st[3] = {nullptr, nullptr, nullptr};
+        hipEvent_t up[2] = {}, comp[2] = {}, down[2] = {}, t0 = {}, t1 = {};
+        ~Res() {
+            for (hipStream_t s : st)
+                if (s) (void)hipStreamSynchronize(s);
+            for (int i = 0; i < 2; ++i) {
+                if (up[i]) (void)hipEventDestroy(up[i]);
+                if (comp[i]) (void)hipEventDestroy(comp[i]);
+                if (down[i]) (void)hipEventDestroy(down[i]);
+            }
+            if (t0) (void)hipEventDestroy(t0);
+            if (t1) (void)hipEventDestroy(t1);
+            for (hipStream_t s : st)
+                if (s) (void)hipStreamDestroy(s);
+        }
+    } r;
+    for (int i = 0; i < 2; ++i) {
+        dq[i].alloc((size_t)chunk * D);
+        di[i].alloc((size_t)chunk * k);
+        dd[i].alloc((size_t)chunk * k);
+    }
+    for (hipStream_t& s : r.st) KNN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        KNN_HIP(hipEventCreateWithFlags(&r.up[i], hipEventDisableTiming));
+        KNN_HIP(hipEventCreateWithFlags(&r.comp[i], hipEventDisableTiming));
+        KNN_HIP(hipEventCreateWithFlags(&r.down[i], hipEventDisableTiming));
+    }
+    KNN_HIP(hipEventCreate(&r.t0));
+    KNN_HIP(hipEventCreate(&r.t1));
+    hipStream_t up = r.st[0], comp = r.st[1], down = r.st[2];
+    KNN_HIP(hipDeviceSynchronize());
+    KNN_HIP(hipEventRecord(r.t0, up));
+    KNN_HIP(hipStreamWaitEvent(comp, r.t0, 0));
+    KNN_HIP(hipStreamWaitEvent(down, r.t0, 0));
+    const uint32_t nchunks = (q + chunk - 1) / chunk;
+    for (uint32_t c = 0; c < nchunks; ++c) {
+        const uint32_t s = c & 1;
+        const uint32_t q0 = c * chunk;
+        const uint32_t m = std::min(chunk, q - q0);
+        if (c >= 2) KNN_HIP(hipStreamWaitEvent(up, r.comp[s], 0));
+        KNN_HIP(hipMemcpyAsync(dq[s].get(), h_q + (size_t)q0 * D, sizeof(float) * (size_t)m * D,
+                               hipMemcpyHostToDevice, up));
+        KNN_HIP(hipEventRecord(r.up[s], up));
+        KNN_HIP(hipStreamWaitEvent(comp, r.up[s], 0));
+        if (c >= 2) KNN_HIP(hipStreamWaitEvent(comp, r.down[s], 0));
+        QueryArgs a = resolve_args(h, dq[s].get(), m, max_d2, di[s].get(), dd[s].get(), k, o);
+        a.stream = comp;
+        a.timed = false;
+        dispatch_query(h, a);
+        KNN_HIP(hipEventRecord(r.comp[s], comp));
+        KNN_HIP(hipStreamWaitEvent(down, r.comp[s], 0));
+        KNN_HIP(hipMemcpyAsync(h_idx + (size_t)q0 * k, di[s].get(), sizeof(int32_t) * (size_t)m * k,
+                               hipMemcpyDeviceToHost, down));
+        KNN_HIP(hipMemcpyAsync(h_dist + (size_t)q0 * k, dd[s].get(), sizeof(float) * (size_t)m * k,
+                               hipMemcpyDeviceToHost, down));
+        KNN_HIP(hipEventRecord(r.down[s], down));
+    }
+    KNN_HIP(hipEventRecord(r.t1, down));
+    KNN_HIP(hipEventSynchronize(r.t1));
+    float ms = 0;
+    KNN_HIP(hipEventElapsedTime(&ms, r.t0, r.t1));
+    h.stats[3] = ms;
 }
 
 }  // namespace knn
@@ -164,53 +297,17 @@ knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max
         Handle& h = *as_handle(handle);
         knn_query_opts o{};
         if (opts) o = *opts;
-        require(k >= 1, "k must be >= 1");
-        require(k <= 4096, "k above 4096 is not supported", KNN_ERR_UNSUPPORTED);
+        check_query(h, k, o);
         if (q == 0) return;
         require(d_q && d_idx && d_dist, "query or result pointer is NULL");
-        require(o.form <= KNN_FORM_LEARNED, "unknown form");
-        require(!(o.form == KNN_FORM_BF16 && h.kind != KNN_KIND_BRUTE), "the bf16 form applies to brute force only");
-        require(!(o.form == KNN_FORM_LEARNED && h.kind == KNN_KIND_BRUTE), "the learned form applies to trees only");
-        QueryArgs a{};
-        a.q = d_q;
-        a.nq = q;
-        a.maxd2 = max_d2;
-        a.oidx = d_idx;
-        a.odist = d_dist;
-        a.K = k;
-        a.block = o.block ? o.block : default_block(h);
-        require(a.block <= device_limits().max_threads, "BLOCK_SIZE above the device maximum");
-        a.heap = o.heap != 0;
-        a.track_lds = o.track_lds != 0;
-        a.stack_lds = o.stack_lds != 0;
-        a.sort_queries = (o.flags & KNN_FLAG_NO_QUERY_SORT) == 0;
-        a.count_visits = (o.flags & KNN_FLAG_COUNT_VISITS) != 0;
-        a.per_lane = (o.flags & KNN_FLAG_PER_LANE) != 0;
-        a.xcd_remap = (o.flags & KNN_FLAG_NO_XCD_REMAP) == 0;
-        a.order_morton = (o.flags & KNN_FLAG_MORTON_ORDER) != 0 || h.kind == KNN_KIND_BALL;
+        QueryArgs a = resolve_args(h, d_q, q, max_d2, d_idx, d_dist, k, o);
         a.timed = (opts == nullptr || opts->stream == nullptr);
         for (float& v : h.stats) v = 0.0f;
-        a.leaf_span = o.leaf_span == 0 ? default_span(h, k, a.per_lane)
-                                       : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
-        a.form = (knn_form)o.form;
-        a.tile = o.tile;
-        a.stream = (hipStream_t)o.stream;
         hipEvent_t e0, e1;
         KNN_HIP(hipEventCreate(&e0));
         KNN_HIP(hipEventCreate(&e1));
         KNN_HIP(hipEventRecord(e0, a.stream));
-        if (h.kind == KNN_KIND_BRUTE) {
-            if (a.form == KNN_FORM_BF16)
-                query_bf_bf16(h, a);
-            else
-                query_bf(h, a);
-        } else if (a.form == KNN_FORM_LEARNED) {
-            query_tree_learned(h, a);
-        } else {
-            uint32_t* perm = nullptr;
-            if (a.sort_queries && q > 1) order_queries(h, a, &perm);
-            query_tree(h, a, perm, nullptr, nullptr);
-        }
+        dispatch_query(h, a);
         KNN_HIP(hipEventRecord(e1, a.stream));
         if (opts == nullptr || opts->stream == nullptr) {
             KNN_HIP(hipEventSynchronize(e1));
@@ -221,6 +318,21 @@ knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    });
+}
+
+knn_status knn_query_host(const void* handle, const float* h_q, uint32_t q, float max_d2, int32_t* h_idx,
+                          float* h_dist, uint32_t k, const knn_query_opts* opts, uint32_t chunk) {
+    return guarded([&] {
+        Handle& h = *as_handle(handle);
+        knn_query_opts o{};
+        if (opts) o = *opts;
+        require(o.stream == nullptr, "knn_query_host manages its own streams: opts->stream must be NULL");
+        check_query(h, k, o);
+        for (float& v : h.stats) v = 0.0f;
+        if (q == 0) return;
+        require(h_q && h_idx && h_dist, "query or result pointer is NULL");
+        host_pipeline(h, h_q, q, max_d2, h_idx, h_dist, k, o, chunk);
     });
 }
 

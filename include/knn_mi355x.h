@@ -120,6 +120,19 @@ knn_status knn_ball_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_
 knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max_d2,
                      int32_t* d_idx, float* d_dist, uint32_t k, const knn_query_opts* opts);
 
+/* Host-resident queries (no reference counterpart; the reference's callers
+ * copy queries in and results out around batchKnn, with pinnedMalloc,
+ * src/KnnTrees.cuh:233-243, as the only host-memory hook).  h_q (q x dims),
+ * h_idx and h_dist (q x k) are host pointers; queries are answered in chunks
+ * of `chunk` (0 = 1,048,576) with the upload of chunk i+1 and the download
+ * of chunk i-1 overlapping the search of chunk i.  Pinned buffers from
+ * knn_pinned_alloc let the copies run concurrently; pageable memory works
+ * but serialises them.  Synchronous; results are identical to knn_query on
+ * the same queries.  opts->stream must be NULL.  The statistics' [3] is the
+ * host-to-host wall time in ms (hipEvent). */
+knn_status knn_query_host(const void* handle, const float* h_q, uint32_t q, float max_d2, int32_t* h_idx,
+                          float* h_dist, uint32_t k, const knn_query_opts* opts, uint32_t chunk);
+
 /* Frees a handle (the reference destructors, e.g. src/KdTreeGpu.cuh:387-390). */
 knn_status knn_destroy(void* handle);
 
@@ -138,6 +151,21 @@ knn_status knn_pinned_free(void* p);
  * split dimension to h_dims (may be NULL). */
 knn_status knn_tree_get_info(const void* handle, knn_tree_info* out);
 knn_status knn_tree_export(const void* handle, void* h_nodes, void* h_leaves, uint32_t* h_dims);
+
+/* Stable serialized tree format (no reference counterpart: the reference
+ * rebuilds its trees on every run).  Little-endian, version 1:
+ *   header (104 bytes): "KNNTREE1", u32 version, u32 0x01020304,
+ *     knn_tree_info (12 x u32), u64 node bytes, u64 split-dim bytes (ball),
+ *     u64 leaf bytes, u64 bbox bytes, u64 FNV-1a 64 of the payload;
+ *   payload: node array, split dims (ball trees), leaves (IndexPoint<D>),
+ *     bounding box (2 x D floats, min then max).
+ * serialize writes into a host buffer of knn_tree_serialized_size bytes;
+ * deserialize validates the header, sizes and checksum before any device
+ * allocation and returns a new tree handle that answers queries exactly like
+ * the original. */
+knn_status knn_tree_serialized_size(const void* handle, size_t* out_bytes);
+knn_status knn_tree_serialize(const void* handle, void* h_buf, size_t buf_bytes);
+knn_status knn_tree_deserialize(const void* h_buf, size_t buf_bytes, void** out_handle);
 
 /* Milliseconds the last build on this handle spent on the device
  * (hipEvent-timed), or -1. */
