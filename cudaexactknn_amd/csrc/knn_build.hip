@@ -131,11 +131,17 @@ __global__ void ball_node_kernel(const double* __restrict__ var, uint32_t D, con
     Lcount[s] = L;
 }
 
-// Left/right flag per point id and next-level segment per position.
+// Left/right bit per point id (bit id of a zeroed bitmap, set for left) and
+// next-level segment per position.  The bitmap (N / 8 bytes: 2 MiB at 2^24
+// points) stays in L2 for the partition kernels' random gathers.
+__device__ __forceinline__ uint32_t left_bit(const uint32_t* __restrict__ bits, uint32_t id) {
+    return (bits[id >> 5] >> (id & 31u)) & 1u;
+}
+
 __global__ void flag_kernel(const uint32_t* __restrict__ P, uint32_t N, uint32_t uniformDim,
                             const uint32_t* __restrict__ segDim, const uint32_t* __restrict__ segOf,
                             const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
-                            uint8_t* __restrict__ flag, uint32_t* __restrict__ segOfNext) {
+                            uint32_t* __restrict__ bits, uint32_t* __restrict__ segOfNext) {
     size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     const uint32_t s = segOf[j];
@@ -144,7 +150,7 @@ __global__ void flag_kernel(const uint32_t* __restrict__ P, uint32_t N, uint32_t
     const uint32_t d = segDim ? segDim[s] : uniformDim;
     const uint32_t id = P[(size_t)d * N + j];
     const bool left = (uint32_t)j - st < L;
-    flag[id] = left ? 1 : 0;
+    if (left) atomicOr(&bits[id >> 5], 1u << (id & 31u));
     segOfNext[j] = 2 * s + (left ? 0 : 1);
 }
 
@@ -156,7 +162,7 @@ constexpr uint32_t PT_TILE = PT_THREADS * PT_ITEMS;
 __device__ __forceinline__ uint32_t list_of(uint32_t y, uint32_t skip) { return y + (y >= skip ? 1u : 0u); }
 
 __global__ void __launch_bounds__(PT_THREADS) part_count_kernel(const uint32_t* __restrict__ P, uint32_t N, uint32_t skip,
-                                                                const uint8_t* __restrict__ flag,
+                                                                const uint32_t* __restrict__ bits,
                                                                 uint32_t* __restrict__ cnt, uint32_t nb) {
     __shared__ uint32_t red[PT_THREADS / 64];
     const uint32_t* L = P + (size_t)list_of(blockIdx.y, skip) * N;
@@ -164,7 +170,7 @@ __global__ void __launch_bounds__(PT_THREADS) part_count_kernel(const uint32_t* 
     uint32_t c = 0;
     for (uint32_t i = threadIdx.x; i < PT_TILE; i += PT_THREADS) {
         size_t j = base + i;
-        if (j < N) c += flag[L[j]];
+        if (j < N) c += left_bit(bits, L[j]);
     }
     for (int off = 32; off >= 1; off >>= 1) c += (uint32_t)__shfl_xor((int)c, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
@@ -183,7 +189,7 @@ __global__ void __launch_bounds__(PT_THREADS) part_count_kernel(const uint32_t* 
 // inside its wave's ballot.
 __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ Pout,
                                                                   uint32_t N, uint32_t skip,
-                                                                  const uint8_t* __restrict__ flag,
+                                                                  const uint32_t* __restrict__ bits,
                                                                   const uint32_t* __restrict__ off, uint32_t nb,
                                                                   const uint32_t* __restrict__ segOf,
                                                                   const uint32_t* __restrict__ segStart,
@@ -212,7 +218,7 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
 #pragma unroll
     for (int i = 0; i < PT_ITEMS; ++i) {
         const size_t j = base + (size_t)i * PT_THREADS + tid;
-        const uint32_t f = j < N ? flag[id[i]] : 0u;
+        const uint32_t f = j < N ? left_bit(bits, id[i]) : 0u;
         fl |= f << i;
         const uint64_t m = __ballot(f != 0);
         rank[i] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -404,7 +410,7 @@ namespace {
 struct LevelBufs {
     DevBuf<uint32_t> P, P2;              // D (or SD) lists of N ids, double-buffered
     DevBuf<uint32_t> segOf, segOf2;      // segment per position
-    DevBuf<uint8_t> flag;                // left flag per point id
+    DevBuf<uint8_t> flag;                // left bit per point id (bitmap of 32-bit words)
     DevBuf<uint32_t> start, n, start2, n2;  // segment tables (cur / next level)
     DevBuf<uint32_t> Lcount, leftBefore, segDim;
     DevBuf<uint32_t> cnt, temp;          // partition block counts, scan temp
@@ -439,7 +445,7 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
     B.P2.ensure((size_t)nl * N);
     B.segOf.ensure(N);
     B.segOf2.ensure(N);
-    B.flag.ensure(N);
+    B.flag.ensure(((size_t)N + 31) / 32 * 4);
     B.start.ensure(maxSeg);
     B.n.ensure(maxSeg);
     B.start2.ensure(maxSeg);
@@ -479,10 +485,11 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
     if (nlists == 0) return;
     const uint32_t nb = nblk(N, PT_TILE);
     hipLaunchKernelGGL(part_count_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), N, skip,
-                       B.flag.get(), B.cnt.get(), nb);
+                       reinterpret_cast<const uint32_t*>(B.flag.get()), B.cnt.get(), nb);
     exclusive_scan_u32(B.cnt.get(), B.cnt.get(), nb * nlists, B.temp.get(), st);
     hipLaunchKernelGGL(part_scatter_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), B.P2.get(), N, skip,
-                       B.flag.get(), B.cnt.get(), nb, segOf, start, n, B.leftBefore.get(), B.Lcount.get(), S);
+                       reinterpret_cast<const uint32_t*>(B.flag.get()), B.cnt.get(), nb, segOf, start, n,
+                       B.leftBefore.get(), B.Lcount.get(), S);
     // The partitioned lists live in P2; the skipped list (already in order)
     // is carried over so that P2 is complete before the swap.
     if (skip < nl)
@@ -494,8 +501,10 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
 void level_common(LevelBufs& B, uint32_t N, uint32_t S, uint32_t uniformDim, bool perSegDim, const uint32_t* start,
                   const uint32_t* n, const uint32_t* segOf, uint32_t* segOf2, hipStream_t st) {
     exclusive_scan_u32(B.Lcount.get(), B.leftBefore.get(), S, B.temp.get(), st);
+    uint32_t* bits = reinterpret_cast<uint32_t*>(B.flag.get());
+    launch_fill_u32(bits, (N + 31) / 32, 0u, st);
     hipLaunchKernelGGL(flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, B.P.get(), N, uniformDim,
-                       perSegDim ? B.segDim.get() : nullptr, segOf, start, n, B.flag.get(), segOf2);
+                       perSegDim ? B.segDim.get() : nullptr, segOf, start, n, bits, segOf2);
 }
 
 struct Timer {
