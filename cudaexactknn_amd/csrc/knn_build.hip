@@ -195,7 +195,9 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
                                                                   const uint32_t* __restrict__ segStart,
                                                                   const uint32_t* __restrict__ segN,
                                                                   const uint32_t* __restrict__ leftBefore,
-                                                                  const uint32_t* __restrict__ Lcount, uint32_t S) {
+                                                                  const uint32_t* __restrict__ Lcount, uint32_t S,
+                                                                  const float* __restrict__ Xin,
+                                                                  float* __restrict__ Xout, uint32_t D) {
     static_assert(PT_THREADS == 256 && PT_ITEMS * (PT_THREADS / 64) == 64, "one prefix-table entry per lane");
     __shared__ uint32_t wcnt[PT_ITEMS * (PT_THREADS / 64)];
     const uint32_t y = blockIdx.y;
@@ -246,8 +248,20 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
             const bool f = (fl >> i) & 1u;
             const uint32_t pos = f ? st + lb : st + Ls + ((uint32_t)j - st - lb);
             O[pos] = id[i];
+            // List 0 carries the coordinates (structure of arrays) along.
+            if (Xin && list_of(y, skip) == 0)
+                for (uint32_t d = 0; d < D; ++d) Xout[(size_t)d * N + pos] = Xin[(size_t)d * N + j];
         }
     }
+}
+
+// Coordinates in list-0 order, one array per dimension.
+__global__ void soa_gather_kernel(const float* __restrict__ pts, uint32_t D, const uint32_t* __restrict__ P0,
+                                  uint32_t N, float* __restrict__ X) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const float* p = pts + (size_t)P0[j] * D;
+    for (uint32_t d = 0; d < D; ++d) X[(size_t)d * N + j] = p[d];
 }
 
 // --- segmented reductions (ball centroids, deviation sums, radii) ----------
@@ -256,7 +270,9 @@ constexpr int DB = 4;  // dims per reduction pass
 // Work item w = (segment s = w / chunks, chunk k = w % chunks), handled by a
 // group of G lanes (G a power of two <= 64).  Lanes stride over the chunk,
 // then a butterfly over the group gives a fixed summation order.
-__global__ void seg_reduce_kernel(const uint32_t* __restrict__ P0, const float* __restrict__ pts, uint32_t D,
+// X: coordinates in list-0 order, one array of N per dimension, so every
+// segment's values are read sequentially in list order.
+__global__ void seg_reduce_kernel(const float* __restrict__ X, uint32_t N, uint32_t D,
                                   uint32_t dbase, const uint32_t* __restrict__ segStart,
                                   const uint32_t* __restrict__ segN, uint32_t S, uint32_t chunks, uint32_t G,
                                   const float* __restrict__ cent, double* __restrict__ partial,
@@ -278,16 +294,16 @@ __global__ void seg_reduce_kernel(const uint32_t* __restrict__ P0, const float* 
         const uint32_t e0 = min(st + n, b0 + len);
         const float* c = cent ? cent + (size_t)s * D : nullptr;
         for (uint32_t j = b0 + r; j < e0; j += G) {
-            const float* p = pts + (size_t)P0[j] * D;
+            const float* p = X + j;  // p[d * N] is dimension d
             if (!c) {
 #pragma unroll
                 for (int b = 0; b < DB; ++b)
-                    if (dbase + b < D) acc[b] += (double)p[dbase + b];
+                    if (dbase + b < D) acc[b] += (double)p[(size_t)(dbase + b) * N];
             } else {
 #pragma unroll
                 for (int b = 0; b < DB; ++b)
                     if (dbase + b < D) {
-                        const double u = (double)p[dbase + b] - (double)c[dbase + b];
+                        const double u = (double)p[(size_t)(dbase + b) * N] - (double)c[dbase + b];
                         acc[b] += u * u;
                     }
                 if (pmax) {
@@ -295,7 +311,7 @@ __global__ void seg_reduce_kernel(const uint32_t* __restrict__ P0, const float* 
                     float tt = c[0] - p[0];
                     float d2 = tt * tt;
                     for (uint32_t d = 1; d < D; ++d) {
-                        const float u = c[d] - p[d];
+                        const float u = c[d] - p[(size_t)d * N];
                         const float sq = u * u;
                         d2 = d2 + sq;
                     }
@@ -419,6 +435,7 @@ struct LevelBufs {
     DevBuf<float> cent, cent2;           // ball: centroids per segment
     DevBuf<double> var, var2, partial;   // ball: deviation sums, chunk partials
     DevBuf<float> pmax;                  // ball: chunk radius maxima
+    DevBuf<float> X, X2;                 // ball: coordinates in list-0 order (D arrays of N)
 };
 
 namespace {
@@ -472,6 +489,8 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
         const size_t items = (size_t)maxSeg + N / 4096 + 2;
         B.partial.ensure(items * 4);
         B.pmax.ensure(items);
+        B.X.ensure((size_t)D * N);
+        B.X2.ensure((size_t)D * N);
     }
 }
 
@@ -480,7 +499,7 @@ namespace {
 // Partition every list except `skip` by the point flags (segment tables of
 // the current level: start, n, segOf).
 void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint32_t S, const uint32_t* start,
-                     const uint32_t* n, const uint32_t* segOf, hipStream_t st) {
+                     const uint32_t* n, const uint32_t* segOf, hipStream_t st, uint32_t coordD = 0) {
     const uint32_t nlists = (skip < nl) ? nl - 1 : nl;
     if (nlists == 0) return;
     const uint32_t nb = nblk(N, PT_TILE);
@@ -489,13 +508,15 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
     exclusive_scan_u32(B.cnt.get(), B.cnt.get(), nb * nlists, B.temp.get(), st);
     hipLaunchKernelGGL(part_scatter_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), B.P2.get(), N, skip,
                        reinterpret_cast<const uint32_t*>(B.flag.get()), B.cnt.get(), nb, segOf, start, n,
-                       B.leftBefore.get(), B.Lcount.get(), S);
+                       B.leftBefore.get(), B.Lcount.get(), S, coordD ? B.X.get() : nullptr,
+                       coordD ? B.X2.get() : nullptr, coordD);
     // The partitioned lists live in P2; the skipped list (already in order)
     // is carried over so that P2 is complete before the swap.
     if (skip < nl)
         KNN_HIP(hipMemcpyAsync(B.P2.get() + (size_t)skip * N, B.P.get() + (size_t)skip * N, sizeof(uint32_t) * N,
                                hipMemcpyDeviceToDevice, st));
     std::swap(B.P.p, B.P2.p);
+    if (coordD) std::swap(B.X.p, B.X2.p);
 }
 
 void level_common(LevelBufs& B, uint32_t N, uint32_t S, uint32_t uniformDim, bool perSegDim, const uint32_t* start,
@@ -537,7 +558,7 @@ void fill_empty_leaves(Handle& h, hipStream_t st) {
 // Segmented reduction over the segments (start, n) of a level: mode 0 sums
 // coordinates into centroids, mode 1 sums squared deviations from `cent` and
 // (pmax) the max distance for radii.
-void seg_reduce(const uint32_t* P0, const float* pts, uint32_t N, uint32_t D, const uint32_t* start, const uint32_t* n,
+void seg_reduce(const float* X, uint32_t N, uint32_t D, const uint32_t* start, const uint32_t* n,
                 uint32_t S, int mode, float* cent, double* var, float* ballNodes, DevBuf<double>& partial,
                 DevBuf<float>& pmax, hipStream_t st) {
     const uint64_t avg = (N + (uint64_t)S - 1) / S;
@@ -557,7 +578,7 @@ void seg_reduce(const uint32_t* P0, const float* pts, uint32_t N, uint32_t D, co
     const float inflate = 1.0f + (float)(D + 8) * 0x1.0p-23f;
     for (uint32_t dbase = 0; dbase < D; dbase += DB) {
         const bool withMax = (mode == 1) && dbase == 0;
-        hipLaunchKernelGGL(seg_reduce_kernel, dim3(grid), dim3(256), 0, st, P0, pts, D, dbase, start, n, S, chunks, G,
+        hipLaunchKernelGGL(seg_reduce_kernel, dim3(grid), dim3(256), 0, st, X, N, D, dbase, start, n, S, chunks, G,
                            mode == 1 ? cent : nullptr, partial.get(), withMax ? pmax.get() : nullptr);
         hipLaunchKernelGGL(seg_finalize_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, mode, S, chunks, D, dbase,
                            partial.get(), withMax ? pmax.get() : nullptr, n, cent, var, ballNodes, inflate);
@@ -641,6 +662,7 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         return;
     }
     presort(B, pts, N, D, D, st);
+    hipLaunchKernelGGL(soa_gather_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, D, B.P.get(), N, B.X.get());
     launch_fill_u32(B.segOf.get(), N, 0u, st);
     launch_fill_u32(B.start.get(), 1, 0u, st);
     launch_fill_u32(B.n.get(), 1, N, st);
@@ -649,21 +671,21 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     float *cent = B.cent.get(), *cent2 = B.cent2.get();
     double *var = B.var.get(), *var2 = B.var2.get();
     // Root centroid and deviation sums (no BallNode entry for the root).
-    seg_reduce(B.P.get(), pts, N, D, start, n, 1, 0, cent, nullptr, nullptr, B.partial, B.pmax, st);
-    seg_reduce(B.P.get(), pts, N, D, start, n, 1, 1, cent, var, nullptr, B.partial, B.pmax, st);
+    seg_reduce(B.X.get(), N, D, start, n, 1, 0, cent, nullptr, nullptr, B.partial, B.pmax, st);
+    seg_reduce(B.X.get(), N, D, start, n, 1, 1, cent, var, nullptr, B.partial, B.pmax, st);
     for (uint32_t depth = 0; depth < I.node_depth; ++depth) {
         const uint32_t S = 1u << depth;
         hipLaunchKernelGGL(ball_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, var, D, start, n, S, h.dims.get(),
                            B.segDim.get(), start2, n2, B.Lcount.get());
         level_common(B, N, S, 0, true, start, n, segOf, segOf2, st);
-        partition_lists(B, N, D, 0xFFFFFFFFu, S, start, n, segOf, st);
+        partition_lists(B, N, D, 0xFFFFFFFFu, S, start, n, segOf, st, D);
         std::swap(segOf, segOf2);
         std::swap(start, start2);
         std::swap(n, n2);
         // Children (level depth + 1): centroids, then deviation sums + radii.
         const uint32_t S2 = 2 * S;
-        seg_reduce(B.P.get(), pts, N, D, start, n, S2, 0, cent2, nullptr, h.nodes.get(), B.partial, B.pmax, st);
-        seg_reduce(B.P.get(), pts, N, D, start, n, S2, 1, cent2, var2, h.nodes.get(), B.partial, B.pmax, st);
+        seg_reduce(B.X.get(), N, D, start, n, S2, 0, cent2, nullptr, h.nodes.get(), B.partial, B.pmax, st);
+        seg_reduce(B.X.get(), N, D, start, n, S2, 1, cent2, var2, h.nodes.get(), B.partial, B.pmax, st);
         std::swap(cent, cent2);
         std::swap(var, var2);
     }
