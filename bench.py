@@ -251,8 +251,9 @@ def main() -> int:
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and not measured here", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the library's multi-GPU layout "
+              "(replicated tree, split queries; sharded points with a key merge) is a design, and no "
+              "figure on more than one GPU is measured", file=sys.stderr)
         return 2
 
     import torch

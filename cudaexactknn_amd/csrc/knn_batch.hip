@@ -221,23 +221,51 @@ static size_t bf16_lds_of(uint32_t D16) {
 }
 static uint32_t bf16_stage_max(uint32_t D16) { return D16 >= 8 ? 1u : 8u / D16; }
 
-// Exact rerank of every bf16 dataset's candidates: one thread per query,
-// blocks of 256 queries of one dataset; overflowed queries are queued.
+// Exact rerank of every bf16 dataset's candidates: one lane per (query,
+// candidate list), adjacent lanes holding a query's two lists; each writes
+// its list's K smallest keys for batch_merge_kernel.  A query with an
+// overflowed list is queued once for the exact fallback.
 __global__ void __launch_bounds__(256) batch_rerank_kernel(const Bf16Set* __restrict__ sets,
                                                            const uint2* __restrict__ blocks,
                                                            uint2* __restrict__ ovf, uint32_t* __restrict__ novf) {
     const uint2 b = blocks[blockIdx.x];
     const Bf16Set& S = sets[b.x];
+    const uint32_t qi = b.y + (threadIdx.x >> 1), l = threadIdx.x & 1u;
+    bool ok = true;
+    if (qi < S.nq) {
+        uint64_t* out = S.partial + ((size_t)qi * 2 + l) * S.K;
+        if (S.K <= 8)
+            ok = bf16_rerank_list<BUF_REG8>(S, qi, l, out);
+        else if (S.K <= 16)
+            ok = bf16_rerank_list<BUF_REG16>(S, qi, l, out);
+        else
+            ok = bf16_rerank_list<BUF_REG32>(S, qi, l, out);
+    }
+    const bool both = ok && __shfl_xor((int)ok, 1, 64) != 0;
+    if (qi < S.nq && l == 0 && !both) ovf[atomicAdd(novf, 1u)] = make_uint2(b.x, qi);
+}
+
+// Two-way merge of each query's two partial lists into its result row
+// (ascending keys, empty slots as index -1 / maxD2); one lane per query.
+__global__ void __launch_bounds__(256) batch_merge_kernel(const Bf16Set* __restrict__ sets,
+                                                          const uint2* __restrict__ blocks) {
+    const uint2 b = blocks[blockIdx.x];
+    const Bf16Set& S = sets[b.x];
     const uint32_t qi = b.y + threadIdx.x;
     if (qi >= S.nq) return;
-    bool ok;
-    if (S.K <= 8)
-        ok = bf16_rerank_one<BUF_REG8>(S, qi);
-    else if (S.K <= 16)
-        ok = bf16_rerank_one<BUF_REG16>(S, qi);
-    else
-        ok = bf16_rerank_one<BUF_REG32>(S, qi);
-    if (!ok) ovf[atomicAdd(novf, 1u)] = make_uint2(b.x, qi);
+    const uint32_t K = S.K;
+    const uint64_t* a = S.partial + (size_t)qi * 2 * K;
+    const uint64_t* c = a + K;
+    uint32_t i = 0, j = 0;
+    for (uint32_t o = 0; o < K; ++o) {
+        const uint64_t x = a[i], y = c[j];
+        const bool takeA = x <= y;
+        const uint64_t v = takeA ? x : y;
+        i += takeA ? 1u : 0u;
+        j += takeA ? 0u : 1u;
+        S.oidx[(size_t)qi * K + o] = (int32_t)(uint32_t)v;
+        S.odist[(size_t)qi * K + o] = u2f(key_hi(v));
+    }
 }
 
 // Exact FP32 rescan of the overflowed queries: one wave per query, lanes
@@ -320,13 +348,14 @@ struct Batch {
     DevBuf<OrderSet> d_osets;
     DevBuf<uint4> d_items;
     DevBuf<uint32_t> d_blk2set;
-    DevBuf<uint2> d_rblocks;
+    DevBuf<uint2> d_rblocks, d_mblocks;
     DevBuf<uint32_t> keys, vals, keys_alt, vals_alt, temp, perm;
     DevBuf<uint32_t> ctr;  // [0..nl) work counters, [nl] fallback counter, [nl+1] overflow count
     DevBuf<uint2> ovf;
     DevBuf<uint32_t> cnt, cand;
+    DevBuf<uint64_t> partial;  // bf16: two partial top-K lists per query
     std::vector<Launch> launches;
-    uint32_t n_rblocks = 0, tree_total_padded = 0, tree_total = 0, bf_total = 0, sort_end_bit = 0;
+    uint32_t n_rblocks = 0, n_mblocks = 0, tree_total_padded = 0, tree_total = 0, bf_total = 0, sort_end_bit = 0;
     hipStream_t cap = nullptr;
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> evs;
@@ -484,10 +513,18 @@ static void plan(Batch& b) {
         b.bfs.push_back(std::move(bb));
     }
     b.cnt.alloc(std::max<size_t>((size_t)b.bf_total * 2, 1));
+    size_t partial_words = 0;
+    std::vector<size_t> partial_off(n_bf);
+    for (uint32_t s = 0; s < n_bf; ++s) {
+        partial_off[s] = partial_words;
+        partial_words += (size_t)bsets[s].nq * 2 * bsets[s].K;
+    }
+    b.partial.alloc(std::max<size_t>(partial_words, 1));
     b.cand.alloc(std::max<size_t>(cand_words, 1));
     b.ovf.alloc(std::max<size_t>(b.bf_total, 1));
     for (uint32_t s = 0; s < n_bf; ++s) {
         bsets[s].cnt = b.cnt.get() + cnt_off[s];
+        bsets[s].partial = b.partial.get() + partial_off[s];
         bsets[s].cand = b.cand.get() + cand_off[s];
     }
 
@@ -637,10 +674,13 @@ static void plan(Batch& b) {
         if (!(d16s[g] == 1 && tree_dims)) add_launch(0, d16s[g], bf_items[g], {});
 
     // rerank blocks
-    std::vector<uint2> rblocks;
+    std::vector<uint2> rblocks, mblocks;
     for (uint32_t s = 0; s < n_bf; ++s)
-        for (uint32_t q0 = 0; q0 < bsets[s].nq; q0 += 256) rblocks.push_back(make_uint2(s, q0));
+        for (uint32_t q0 = 0; q0 < bsets[s].nq; q0 += 128) rblocks.push_back(make_uint2(s, q0));
+    for (uint32_t s = 0; s < n_bf; ++s)
+        for (uint32_t q0 = 0; q0 < bsets[s].nq; q0 += 256) mblocks.push_back(make_uint2(s, q0));
     b.n_rblocks = (uint32_t)rblocks.size();
+    b.n_mblocks = (uint32_t)mblocks.size();
 
     // ---- upload
     auto up = [](auto& buf, const auto& v) {
@@ -654,6 +694,7 @@ static void plan(Batch& b) {
     up(b.d_items, all_items);
     up(b.d_blk2set, blk2set);
     up(b.d_rblocks, rblocks);
+    up(b.d_mblocks, mblocks);
     b.ctr.alloc(b.launches.size() + 2);
     b.stats[3] = (float)all_items.size();
     uint32_t wg = 0;
@@ -736,6 +777,8 @@ static void plan(Batch& b) {
         if (b.n_rblocks) {
             hipLaunchKernelGGL(batch_rerank_kernel, dim3(b.n_rblocks), dim3(256), 0, st, b.d_bsets.get(),
                                b.d_rblocks.get(), b.ovf.get(), b.ctr.get() + nl + 1);
+            hipLaunchKernelGGL(batch_merge_kernel, dim3(b.n_mblocks), dim3(256), 0, st, b.d_bsets.get(),
+                               b.d_mblocks.get());
             hipLaunchKernelGGL(batch_fallback_kernel, dim3(2 * lim.cu_count), dim3(256), 0, st, b.d_bsets.get(),
                                b.ovf.get(), b.ctr.get() + nl + 1, b.ctr.get() + nl);
         }

@@ -162,6 +162,13 @@ __global__ void __launch_bounds__(256) bf_merge_kernel(const uint64_t* __restric
     }
 }
 
+void launch_bf_merge(const uint64_t* partial, uint32_t nq, uint32_t S, uint32_t K, bool descending, int32_t* oidx,
+                     float* odist, hipStream_t st) {
+    const uint32_t mb = 128;
+    hipLaunchKernelGGL(bf_merge_kernel, dim3((nq + mb - 1) / mb), dim3(mb), (size_t)mb * S * 2, st, partial, nq, S, K,
+                       descending, oidx, odist);
+}
+
 using BfFn = void (*)(const float*, uint32_t, const float*, uint32_t, float, uint32_t, int32_t*, float*, uint32_t,
                      uint32_t, uint64_t*);
 using BfRtFn = void (*)(const float*, uint32_t, uint32_t, const float*, uint32_t, float, uint32_t, int32_t*, float*,

@@ -65,15 +65,23 @@ __global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t ti
     bf16_screen_run<D16, KC>(S, blockIdx.x, blockIdx.y, t0, t1, stage, s_hi, s_lo, s_np);
 }
 
-// Exact rerank of the candidates of each query (one lane per query); queries
-// whose candidates overflowed are listed for the plain fallback.
+// Exact rerank, one lane per (query, candidate list): each lane writes its
+// list's K smallest keys for the K-way merge (bf_merge_kernel).  A query with
+// an overflowed list is listed once for the plain fallback.
 template <int KIND>
-__global__ void __launch_bounds__(256) bf16_rerank_kernel(Bf16Set S, uint32_t* __restrict__ ovf,
-                                                          uint32_t* __restrict__ novf) {
-    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= S.nq) return;
-    if (!bf16_rerank_one<KIND>(S, qi)) ovf[atomicAdd(novf, 1u)] = qi;
+__global__ void __launch_bounds__(256) bf16_rerank_lists_kernel(Bf16Set S, uint64_t* __restrict__ partial,
+                                                                uint32_t* __restrict__ ovf_flag,
+                                                                uint32_t* __restrict__ ovf, uint32_t* __restrict__ novf) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)S.nq * S.lists) return;
+    const uint32_t qi = (uint32_t)(t / S.lists), l = (uint32_t)(t % S.lists);
+    if (!bf16_rerank_list<KIND>(S, qi, l, partial + t * S.K)) {
+        if (atomicExch(&ovf_flag[qi], 1u) == 0u) ovf[atomicAdd(novf, 1u)] = qi;
+    }
 }
+
+void launch_bf_merge(const uint64_t* partial, uint32_t nq, uint32_t S, uint32_t K, bool descending, int32_t* oidx,
+                     float* odist, hipStream_t st);
 
 __global__ void gather_rows_kernel(const float* __restrict__ src, const uint32_t* __restrict__ ids, uint32_t m,
                                    uint32_t w, float* __restrict__ dst) {
@@ -136,17 +144,12 @@ static ScreenFn screen_for(uint32_t K) {
     return bf16_screen_kernel<D16, 32>;
 }
 
-using RerankFn = void (*)(Bf16Set, uint32_t*, uint32_t*);
+using RerankFn = void (*)(Bf16Set, uint64_t*, uint32_t*, uint32_t*, uint32_t*);
 
-static RerankFn rerank_for(int kind) {
-    switch (kind) {
-        case BUF_REG8: return bf16_rerank_kernel<BUF_REG8>;
-        case BUF_REG16: return bf16_rerank_kernel<BUF_REG16>;
-        case BUF_REG32: return bf16_rerank_kernel<BUF_REG32>;
-        case BUF_REGH8: return bf16_rerank_kernel<BUF_REGH8>;
-        case BUF_REGH16: return bf16_rerank_kernel<BUF_REGH16>;
-        default: return bf16_rerank_kernel<BUF_REGH32>;
-    }
+static RerankFn rerank_for(uint32_t K) {
+    if (K <= 8) return bf16_rerank_lists_kernel<BUF_REG8>;
+    if (K <= 16) return bf16_rerank_lists_kernel<BUF_REG16>;
+    return bf16_rerank_lists_kernel<BUF_REG32>;
 }
 
 // Error-bound constants for dimension D (computed in double on the host).
@@ -201,14 +204,19 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     uint32_t cap = (uint32_t)std::min(2048.0, 2.0 * a.K * (1.0 + std::log(m)) + 64.0);
     cap = (cap + 15) & ~15u;
     while (cap > 64 && (size_t)a.nq * lists * cap * 4 > ((size_t)1 << 30)) cap /= 2;
-    // scratch: cnt[nq * lists] | cand[nq * lists * cap] | ovf[nq] | novf
-    const size_t need = ((size_t)a.nq * lists * (cap + 1) + a.nq + 4) * 4;
+    // scratch: partial keys [nq * lists * K] (u64) | cnt[nq * lists] |
+    // cand[nq * lists * cap] | ovf[nq] | ovf_flag[nq] | novf
+    const size_t partial_words = (size_t)a.nq * lists * a.K;
+    const size_t need = partial_words * 8 + ((size_t)a.nq * lists * (cap + 1) + 2 * (size_t)a.nq + 4) * 4;
     h.q_scratch.ensure(need);
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(h.q_scratch.get());
+    uint64_t* partial = reinterpret_cast<uint64_t*>(h.q_scratch.get());
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(partial + partial_words);
     uint32_t* cand = cnt + (size_t)a.nq * lists;
     uint32_t* ovf = cand + (size_t)a.nq * lists * cap;
-    uint32_t* novf = ovf + a.nq;
+    uint32_t* ovf_flag = ovf + a.nq;
+    uint32_t* novf = ovf_flag + a.nq;
     launch_fill_u32(novf, 1, 0u, a.stream);
+    launch_fill_u32(ovf_flag, a.nq, 0u, a.stream);
 
     Bf16Set S{};
     S.phi = phi;
@@ -241,8 +249,10 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     const uint32_t stage = a.tile ? (a.tile < max_stage ? a.tile : max_stage) : max_stage;
     EventTimer timer(a.timed, a.stream);
     hipLaunchKernelGGL(fn, dim3(qblocks, S_), dim3(256), 0, a.stream, S, tpc, stage);
-    const int kind = pick_buf_kind(a.heap, false, a.K);
-    hipLaunchKernelGGL(rerank_for(kind), dim3((a.nq + 255) / 256), dim3(256), 0, a.stream, S, ovf, novf);
+    const size_t lanes = (size_t)a.nq * lists;
+    hipLaunchKernelGGL(rerank_for(a.K), dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, a.stream, S, partial,
+                       ovf_flag, ovf, novf);
+    launch_bf_merge(partial, a.nq, lists, a.K, a.heap, a.oidx, a.odist, a.stream);
     KNN_CHECK_LAUNCH();
     uint32_t n_ovf = 0;
     KNN_HIP(hipMemcpyAsync(&n_ovf, novf, 4, hipMemcpyDeviceToHost, a.stream));

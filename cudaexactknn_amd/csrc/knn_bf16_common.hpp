@@ -78,6 +78,7 @@ struct Bf16Set {
     uint32_t* cand;
     uint32_t cap;    // slots per list
     uint32_t lists;  // 2 x point chunks
+    uint64_t* partial;  // batched path: per-list partial top-K keys
     int32_t* oidx;
     float* odist;
 };
@@ -248,6 +249,37 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
         }
     }
     if (active) S.cnt[list] = count;
+}
+
+// Exact top-K of one candidate list l of query qi (array-mode K-buffer),
+// written ascending to out[0..K) as partial keys for a K-way merge.  Returns
+// false when the list overflowed (out then holds empty keys only).
+template <int KIND>
+__device__ __forceinline__ bool bf16_rerank_list(const Bf16Set& S, uint32_t qi, uint32_t l, uint64_t* __restrict__ out) {
+    const size_t li = (size_t)qi * S.lists + l;
+    const uint32_t c = S.cnt[li];
+    typename BufOf<KIND>::T buf;
+    const uint64_t empty = ((uint64_t)f2u(S.maxd2) << 32) | kEmptyLo;
+    uint64_t worst = buf.init(empty, S.K);
+    uint32_t whi = key_hi(worst);
+    const bool ok = c <= S.cap;
+    if (ok) {
+        const float* q = S.qs + (size_t)qi * S.D;
+        const uint32_t* list = S.cand + li * S.cap;
+        for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t pt = list[j];
+            const float d2 = dist2_rt(q, 1, S.pts + (size_t)pt * S.D, S.D);
+            if (f2u(d2) <= whi) {
+                const uint64_t key = make_key(d2, pt);
+                if (key < worst) {
+                    worst = buf.consider(key);
+                    whi = key_hi(worst);
+                }
+            }
+        }
+    }
+    buf.write_keys(out, S.K);
+    return ok;
 }
 
 // Exact rerank of query qi's candidates (array-mode or heap-mode K-buffer

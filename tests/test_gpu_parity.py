@@ -310,3 +310,27 @@ def test_bf16_form_radius_and_small_sets(oracle, cuda):
             for k in (1, 5, 40):
                 want = oracle.bf_knn(pts, qs, k, max_d2=md)
                 same(run(bf, qs, k, cuda, md=md, form=knn.FORM_BF16), want, f"n={n} md={md} k={k}")
+
+
+@pytest.mark.parametrize("d", [3, 5])
+@pytest.mark.parametrize("k", [1, 8, 32])
+@pytest.mark.parametrize("md", [np.inf, 0.004, 0.0, -1.0])
+def test_bf_larger_sets_radius_and_tie_edges(oracle, cuda, d, k, md):
+    """40,000 points (split scans with partial merges): duplicated points and
+    lattice ties around the K-th neighbour, small, zero and negative radii,
+    every K-buffer variant."""
+    g = np.random.Generator(np.random.Philox(1000 + 10 * d + k))
+    n = 40000
+    pts = g.random((n, d), dtype=np.float32)
+    pts[n // 2:n // 2 + 64] = pts[:64]          # duplicates: index tie-break
+    qs = g.random((1500, d), dtype=np.float32)
+    qs[:32] = pts[:32]                           # distance-0 hits with duplicates
+    qs[32:64] = np.round(qs[32:64] * 8) / 8      # lattice queries against lattice points
+    pts[100:400] = np.round(pts[100:400] * 8) / 8
+    want = oracle.bf_knn(pts, qs, k, md)
+    bf = knn.BruteForceGpu(dev(pts, cuda))
+    for v in BF_VARIANTS:
+        got = run(bf, qs, k, cuda, md, **v)
+        w = want if not v["USE_HEAP_TRACKING"] else (want[0][:, ::-1].copy(), want[1][:, ::-1].copy())
+        same(got, w, f"bf d={d} k={k} md={md} {v}")
+    bf.close()
