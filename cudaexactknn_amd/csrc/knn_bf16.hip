@@ -54,15 +54,16 @@ __global__ void bf16_norms_kernel(const float* __restrict__ pts, uint32_t n, uin
 }
 
 // grid (query blocks of 128, point chunks); chunk y screens tiles
-// [y * tiles_per_chunk, ...).
-template <int D16, int KC>
-__global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t tiles_per_chunk, uint32_t stage) {
+// [y * tiles_per_chunk, ...) of the `vtiles` tiles v * ts.
+template <int D16, int KC, bool SEED>
+__global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t tiles_per_chunk, uint32_t stage,
+                                                          uint32_t ts, uint32_t vtiles) {
     __shared__ uint4 s_hi[bf16_stage_tiles<D16>() * D16 * 64];
     __shared__ uint4 s_lo[bf16_stage_tiles<D16>() * D16 * 64];
     __shared__ __align__(16) float s_np[bf16_stage_tiles<D16>() * 33];
     const uint32_t t0 = blockIdx.y * tiles_per_chunk;
-    const uint32_t t1 = min(S.tiles, t0 + tiles_per_chunk);
-    bf16_screen_run<D16, KC>(S, blockIdx.x, blockIdx.y, t0, t1, stage, s_hi, s_lo, s_np);
+    const uint32_t t1 = min(vtiles, t0 + tiles_per_chunk);
+    bf16_screen_run<D16, KC, SEED>(S, blockIdx.x, blockIdx.y, t0, t1, stage, s_hi, s_lo, s_np, ts);
 }
 
 // Exact rerank, one lane per (query, candidate list): each lane writes its
@@ -135,13 +136,13 @@ void prepare_bf16(Handle& h, hipStream_t st) {
     launch_bf16_prep(h.pts.get(), n, D, phi, phi + words, h.norms.get(), st);
 }
 
-using ScreenFn = void (*)(Bf16Set, uint32_t, uint32_t);
+using ScreenFn = void (*)(Bf16Set, uint32_t, uint32_t, uint32_t, uint32_t);
 
-template <int D16>
+template <int D16, bool SEED>
 static ScreenFn screen_for(uint32_t K) {
-    if (K <= 8) return bf16_screen_kernel<D16, 8>;
-    if (K <= 16) return bf16_screen_kernel<D16, 16>;
-    return bf16_screen_kernel<D16, 32>;
+    if (K <= 8) return bf16_screen_kernel<D16, 8, SEED>;
+    if (K <= 16) return bf16_screen_kernel<D16, 16, SEED>;
+    return bf16_screen_kernel<D16, 32, SEED>;
 }
 
 using RerankFn = void (*)(Bf16Set, uint64_t*, uint32_t*, uint32_t*, uint32_t*);
@@ -205,9 +206,9 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     cap = (cap + 15) & ~15u;
     while (cap > 64 && (size_t)a.nq * lists * cap * 4 > ((size_t)1 << 30)) cap /= 2;
     // scratch: partial keys [nq * lists * K] (u64) | cnt[nq * lists] |
-    // cand[nq * lists * cap] | ovf[nq] | ovf_flag[nq] | novf
+    // cand[nq * lists * cap] | ovf[nq] | ovf_flag[nq] | novf | seed[nq]
     const size_t partial_words = (size_t)a.nq * lists * a.K;
-    const size_t need = partial_words * 8 + ((size_t)a.nq * lists * (cap + 1) + 2 * (size_t)a.nq + 4) * 4;
+    const size_t need = partial_words * 8 + ((size_t)a.nq * lists * (cap + 1) + 3 * (size_t)a.nq + 4) * 4;
     h.q_scratch.ensure(need);
     uint64_t* partial = reinterpret_cast<uint64_t*>(h.q_scratch.get());
     uint32_t* cnt = reinterpret_cast<uint32_t*>(partial + partial_words);
@@ -215,6 +216,7 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     uint32_t* ovf = cand + (size_t)a.nq * lists * cap;
     uint32_t* ovf_flag = ovf + a.nq;
     uint32_t* novf = ovf_flag + a.nq;
+    uint32_t* seed = novf + 4;
     launch_fill_u32(novf, 1, 0u, a.stream);
     launch_fill_u32(ovf_flag, a.nq, 0u, a.stream);
 
@@ -237,18 +239,33 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     S.lists = lists;
     S.oidx = a.oidx;
     S.odist = a.odist;
-    ScreenFn fn = nullptr;
+    ScreenFn fn = nullptr, sfn = nullptr;
     uint32_t max_stage = 1;
     switch (D16) {
-        case 1: fn = screen_for<1>(a.K); max_stage = bf16_stage_tiles<1>(); break;
-        case 2: fn = screen_for<2>(a.K); max_stage = bf16_stage_tiles<2>(); break;
-        case 4: fn = screen_for<4>(a.K); max_stage = bf16_stage_tiles<4>(); break;
-        case 8: fn = screen_for<8>(a.K); max_stage = bf16_stage_tiles<8>(); break;
-        default: fn = screen_for<16>(a.K); max_stage = bf16_stage_tiles<16>(); break;
+        case 1: fn = screen_for<1, false>(a.K); sfn = screen_for<1, true>(a.K); max_stage = bf16_stage_tiles<1>(); break;
+        case 2: fn = screen_for<2, false>(a.K); sfn = screen_for<2, true>(a.K); max_stage = bf16_stage_tiles<2>(); break;
+        case 4: fn = screen_for<4, false>(a.K); sfn = screen_for<4, true>(a.K); max_stage = bf16_stage_tiles<4>(); break;
+        case 8: fn = screen_for<8, false>(a.K); sfn = screen_for<8, true>(a.K); max_stage = bf16_stage_tiles<8>(); break;
+        default: fn = screen_for<16, false>(a.K); sfn = screen_for<16, true>(a.K); max_stage = bf16_stage_tiles<16>(); break;
     }
     const uint32_t stage = a.tile ? (a.tile < max_stage ? a.tile : max_stage) : max_stage;
+    // Seeding pass over every 16th tile when the point set is large and
+    // D > 16: it gives each query a starting bound near the K-th distance of
+    // a 1/16 sample, so the full pass emits about 3x fewer candidates while
+    // its own bound tightens (measured: D = 128, 151 -> 134 ms).  At D <= 16
+    // candidates are cheap to rerank and the extra pass costs more than it
+    // saves (45 -> 51 ms).
+    constexpr uint32_t kSeedStride = 16;
+    const bool seeded = tiles >= kSeedStride * 64 && D16 >= 2;
     EventTimer timer(a.timed, a.stream);
-    hipLaunchKernelGGL(fn, dim3(qblocks, S_), dim3(256), 0, a.stream, S, tpc, stage);
+    if (seeded) {
+        S.seed = seed;
+        launch_fill_u32(seed, a.nq, 0x7F800000u, a.stream);  // +inf
+        const uint32_t vt = (tiles + kSeedStride - 1) / kSeedStride;
+        const uint32_t vpc = (vt + S_ - 1) / S_;
+        hipLaunchKernelGGL(sfn, dim3(qblocks, S_), dim3(256), 0, a.stream, S, vpc, stage, kSeedStride, vt);
+    }
+    hipLaunchKernelGGL(fn, dim3(qblocks, S_), dim3(256), 0, a.stream, S, tpc, stage, 1u, tiles);
     const size_t lanes = (size_t)a.nq * lists;
     hipLaunchKernelGGL(rerank_for(a.K), dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, a.stream, S, partial,
                        ovf_flag, ovf, novf);

@@ -79,6 +79,7 @@ struct Bf16Set {
     uint32_t cap;    // slots per list
     uint32_t lists;  // 2 x point chunks
     uint64_t* partial;  // batched path: per-list partial top-K keys
+    uint32_t* seed;     // optional per-query starting bound (float bits), from a seeding pass
     int32_t* oidx;
     float* odist;
 };
@@ -94,10 +95,15 @@ constexpr size_t bf16_stage_lds() {
 // Screen tiles [t0, t1) for the 128 queries of `qblock`, `stage` (<=
 // bf16_stage_tiles<D16>()) tiles per barrier pair.  Every thread of the
 // 256-wide workgroup must call it (it synchronises the workgroup).
-template <int D16, int KC>
+// Tile v of the range is point tile v * ts.  SEED: a pass over every ts-th
+// tile that emits nothing and only lowers S.seed[q] to the query's running
+// bound: the K-th smallest upper bound over a subset of the points is an
+// upper bound of the K-th distance, so the full pass may start from it.
+template <int D16, int KC, bool SEED = false>
 __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qblock, uint32_t chunk, uint32_t t0,
                                                 uint32_t t1, uint32_t stage, uint4* __restrict__ s_hi,
-                                                uint4* __restrict__ s_lo, float* __restrict__ s_np) {
+                                                uint4* __restrict__ s_lo, float* __restrict__ s_np,
+                                                uint32_t ts = 1) {
 #pragma clang fp contract(off)
     // s_np: stage x 32 norms, then stage per-tile maxima.
     float* __restrict__ s_npmax = s_np + bf16_stage_tiles<D16>() * 32;
@@ -133,9 +139,11 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
             qn = qn + s;
         }
     }
-    // Candidate cap from maxD2: only a non-negative, non-NaN radius bounds f.
+    // Candidate cap from maxD2 (only a non-negative, non-NaN radius bounds f)
+    // and from the seeding pass.
     const uint32_t mb = f2u(S.maxd2);
-    const float maxcap = ((mb >> 31) == 0 && mb < 0x7F800001u) ? S.maxd2 : INFINITY;
+    float maxcap = ((mb >> 31) == 0 && mb < 0x7F800001u) ? S.maxd2 : INFINITY;
+    if (!SEED && S.seed && active) maxcap = fminf(maxcap, __uint_as_float(S.seed[qi]));
 
     // This lane's K smallest upper bounds, ascending; KC-K leading -inf pads
     // keep the K-th at a static slot.
@@ -145,25 +153,27 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
     float lane_tau = INFINITY;
     float tau = maxcap;
     // This lane's candidate list and its length.
-    const size_t list = (size_t)qi * S.lists + chunk * 2 + half;
-    uint32_t* __restrict__ mine = S.cand + list * S.cap;
+    const size_t list = SEED ? 0 : (size_t)qi * S.lists + chunk * 2 + half;
+    uint32_t* __restrict__ mine = SEED ? nullptr : S.cand + list * S.cap;
     uint32_t count = 0;
+    constexpr uint32_t W = D16 * 64;  // uint4 words per tile and part
 
     for (uint32_t tb = t0; tb < t1; tb += stage) {
         const uint32_t nt = min(stage, t1 - tb);
         __syncthreads();
-        const size_t base = (size_t)tb * D16 * 64;
-        const uint32_t words = nt * D16 * 64;
+        const uint32_t words = nt * W;
         for (uint32_t i = tid; i < words; i += 256) {
-            s_hi[i] = S.phi[base + i];
-            s_lo[i] = S.plo[base + i];
+            const uint32_t u = i / W;
+            const size_t src = (size_t)(tb + u) * ts * W + (i - u * W);
+            s_hi[i] = S.phi[src];
+            s_lo[i] = S.plo[src];
         }
         {
             // nt * 32 <= 256 norms, one per thread; per-tile maxima by a
             // 32-lane butterfly.  Norms are >= 0 or NaN; fmaxf drops NaN,
             // which is safe: a pair with a NaN norm has A = NaN and is
             // always emitted.
-            const uint32_t p = tb * 32 + tid;
+            const uint32_t p = (tb + (tid >> 5)) * ts * 32 + (tid & 31);
             float v = (tid < nt * 32 && p < S.n) ? S.pnorm[p] : 0.0f;
             if (tid < nt * 32) s_np[tid] = v;
 #pragma unroll
@@ -187,7 +197,7 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
         };
         f32x16 acc = tile_dot(0), acc_next = acc;
         for (uint32_t u = 0; u < nt; ++u) {
-            const uint32_t t = tb + u;
+            const uint32_t t = (tb + u) * ts;
             if (u + 1 < nt) acc_next = tile_dot(u + 1);
             if (active) {
                 // This lane's 16 rows are 8j + 4 half + i (j, i < 4): four
@@ -215,9 +225,11 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                     for (int r = 0; r < 16; ++r) {
                         const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
                         const bool valid = decltype(all_valid)::value || row < nvalid;
-                        if (!(A[r] > T) && valid) {
-                            if (count < S.cap) mine[count] = t * 32 + row;
-                            ++count;
+                        if constexpr (!SEED) {
+                            if (!(A[r] > T) && valid) {
+                                if (count < S.cap) mine[count] = t * 32 + row;
+                                ++count;
+                            }
                         }
                         if (A[r] < Ithr && valid) {
                             float key = (A[r] + Et) * S.P.up;
@@ -248,7 +260,12 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
             acc = acc_next;
         }
     }
-    if (active) S.cnt[list] = count;
+    if constexpr (SEED) {
+        // tau >= 0 (upper bounds of distances) or +inf: its bits order as floats.
+        if (active && half == 0) atomicMin(&S.seed[qi], __float_as_uint(tau));
+    } else {
+        if (active) S.cnt[list] = count;
+    }
 }
 
 // Exact top-K of one candidate list l of query qi (array-mode K-buffer),
