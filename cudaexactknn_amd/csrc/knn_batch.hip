@@ -589,9 +589,7 @@ static void plan(Batch& b) {
         if (span > I.node_depth) span = I.node_depth;
         A.stop_level = I.node_depth - span;
         A.stack_depth = A.stop_level + 1;
-        uint32_t pl = 0;
-        while (pl < A.stop_level && ((uint64_t)I.n >> (pl + 1)) >= 2ull * it.k) ++pl;
-        A.prime_level = pl;
+        A.prime_level = packet_prime_level(I.n, it.k, A.stop_level, h.kind == KNN_KIND_BALL);
         A.dims = I.dims;
         A.qs = it.d_q;
         A.perm = b.perm.get() + sorted_off[t];

@@ -92,14 +92,22 @@ struct RegArray {
         for (int i = 0; i < KCAP; ++i) s[i] = (i >= KCAP - (int)K) ? empty : 0ull;
         return empty;
     }
+    // Shift insertion: with c_i = (s[i] < key), which holds for a prefix of
+    // the slots, the new slot i is s[i] where c_i, key where c_{i-1} and not
+    // c_i, else s[i-1]; the last slot drops out (c_{KCAP-1} is false).
+    // Walking i downwards reads each s[i-1] before it is rewritten and
+    // carries c_{i-1} to the next step, so one 64-bit compare and two 64-bit
+    // selects per slot suffice.
     __device__ __forceinline__ uint64_t consider(uint64_t key) {
+        bool ci = false;  // key < s[KCAP-1] is the caller's precondition
 #pragma unroll
-        for (int i = 0; i < KCAP; ++i) {
-            uint64_t e = s[i];
-            bool lt = e < key;
-            s[i] = lt ? e : key;
-            key = lt ? key : e;
+        for (int i = KCAP - 1; i >= 1; --i) {
+            const bool cp = s[i - 1] < key;
+            const uint64_t t = cp ? key : s[i - 1];
+            s[i] = ci ? s[i] : t;
+            ci = cp;
         }
+        s[0] = ci ? s[0] : key;
         return s[KCAP - 1];
     }
     // Raw keys, ascending (partial results of a split scan).

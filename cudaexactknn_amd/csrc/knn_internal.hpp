@@ -132,6 +132,21 @@ inline uint32_t log2_ceil_host(uint32_t x) {
     while (c < x) { c *= 2; r += 1; }
     return r;
 }
+// Priming level of the wave-cooperative traversals: the deepest level whose
+// subtrees all hold >= F * K points (a subtree at level l holds floor or
+// ceil of n / 2^l), capped at the scan level.  Each lane scans its own
+// subtree at that level before the shared traversal, with all lanes
+// inserting in parallel; a larger subtree moves insertions out of the shared
+// traversal, where they run for the whole wave on behalf of a few lanes.
+// Measured on the 16.7M-point 3-D mixture at K = 16 (F = 2 / 4 / 8 / 16):
+// kd 17.3 / 16.7 / 15.7 / 16.2 ms, ball 19.06 / 19.08 / 19.30 / 19.91 ms.
+inline uint32_t packet_prime_level(uint32_t n, uint32_t K, uint32_t stop_level, bool ball) {
+    const uint64_t F = ball ? 2 : 8;
+    uint32_t pl = 0;
+    while (pl < stop_level && ((uint64_t)n >> (pl + 1)) >= F * K) ++pl;
+    return pl;
+}
+
 inline void tree_sizing(knn_tree_info& t, uint32_t n, uint32_t D, uint32_t SD, uint32_t skips, bool ball) {
     uint32_t L = log2_ceil_host(n);
     uint32_t s = skips;

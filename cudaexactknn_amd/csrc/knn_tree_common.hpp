@@ -109,48 +109,50 @@ __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uin
 }
 
 // Wave-cooperative scan of leaf slots [slot0, slot0 + count) (count, slot0
-// wave-uniform).  Every lane loads the same slot address (one cache-line
-// request per wave), four slots are in flight per step, and readfirstlane
-// moves each slot into SGPRs; every lane then tests the slot against its own
-// query.  (Scalar loads through the constant address space were measured
-// slower here: ~20 waves per CU stream different leaves through a small
-// scalar cache.)  Slots inside the lane's primed range [land0, land0 + landN)
-// were already considered and are skipped.
-__device__ __forceinline__ float bcast(float v) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
+// wave-uniform).  Slot addresses are wave-uniform, so slots are read with
+// scalar loads: one load per slot for the index and the coordinates together,
+// issued one slot ahead, so a slot's test overlaps the next slot's load
+// (reading the index first and the coordinates after its -1 check costs two
+// dependent load latencies per slot).  Every lane tests each slot against its
+// own query.  Slots inside the lane's primed range [land0, land0 + landN) were
+// already considered and are skipped: per group when no lane's group is
+// larger than that range (the group then lies wholly inside or outside it),
+// else slot by slot.
 template <int D, class Q, class B>
 __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
                                             bool stop_at_empty, const Q& q, bool active, uint32_t land0,
                                             uint32_t landN, B& buf, uint64_t& worst, uint32_t& whi,
                                             uint32_t& nslots) {
     constexpr uint32_t stride = 1 + D;
-    // A group lies wholly inside or wholly outside the lane's primed range
-    // when it is no larger than that range; otherwise test slot by slot.
-    const bool group_primed = slot0 - land0 < landN && count <= landN;
-    const bool per_slot = count > landN;
-    for (uint32_t o = 0; o < count; ++o) {
-        const uint32_t sl = slot0 + o;
-        const float* e = leaves + (size_t)sl * stride;
-        const int32_t pi = __float_as_int(bcast(e[0]));
-        ++nslots;
-        if (pi == -1) {
-            if (stop_at_empty) break;
-            continue;
-        }
-        float p[D];
+    const bool slotwise = __ballot(active && count > landN) != 0ull;
+    const bool mine_group = active && !(slot0 - land0 < landN);
+    float v[stride];
 #pragma unroll
-        for (int d = 0; d < D; ++d) p[d] = bcast(e[1 + d]);
-        const float d2 = q.dist_to(p);
-        const bool mine = active && (per_slot ? (sl - land0 >= landN) : !group_primed);
-        if (mine && f2u(d2) <= whi) {
-            const uint64_t key = make_key(d2, (uint32_t)pi);
-            if (key < worst) {
-                worst = buf.consider(key);
-                whi = key_hi(worst);
+    for (uint32_t t = 0; t < stride; ++t) v[t] = leaves[(size_t)slot0 * stride + t];
+    for (uint32_t o = 0; o < count; ++o) {
+        // The next slot's load is issued before this slot is tested (the
+        // last slot re-reads itself).
+        float nx[stride];
+        const float* e = leaves + (size_t)(slot0 + min(o + 1, count - 1)) * stride;
+#pragma unroll
+        for (uint32_t t = 0; t < stride; ++t) nx[t] = e[t];
+        ++nslots;
+        if (__float_as_int(v[0]) == -1) {
+            if (stop_at_empty) break;
+        } else {
+            const uint32_t sl = slot0 + o;
+            const float d2 = q.dist_to(v + 1);
+            const bool mine = slotwise ? (active && sl - land0 >= landN) : mine_group;
+            if (mine && f2u(d2) <= whi) {
+                const uint64_t key = make_key(d2, (uint32_t)__float_as_int(v[0]));
+                if (key < worst) {
+                    worst = buf.consider(key);
+                    whi = key_hi(worst);
+                }
             }
         }
+#pragma unroll
+        for (uint32_t t = 0; t < stride; ++t) v[t] = nx[t];
     }
 }
 

@@ -127,13 +127,7 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     const uint32_t span = a.leaf_span < I.node_depth ? a.leaf_span : I.node_depth;
     A.stop_level = I.node_depth - span;
     A.stack_depth = A.stop_level + 1;
-    {
-        // Deepest level whose subtrees all hold >= 2K points (each holds
-        // floor or ceil of N / 2^level), capped at the scan level.
-        uint32_t pl = 0;
-        while (pl < A.stop_level && ((uint64_t)I.n >> (pl + 1)) >= 2ull * a.K) ++pl;
-        A.prime_level = pl;
-    }
+    A.prime_level = packet_prime_level(I.n, a.K, A.stop_level, h.kind == KNN_KIND_BALL);
     A.dims = I.dims;
     A.qs = a.q;
     A.perm = d_perm;
