@@ -35,12 +35,13 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
     constexpr uint32_t nstride = 2 * D + 2;
     const uint32_t stride = 1 + D;
     const uint32_t stop = A.stop_level;
-    const bool single_bucket = stop == A.node_depth;
     // Priming: each lane descends to its own landing bucket (nearer centroid
     // at each level) and scans it; the shared traversal then skips it.
     uint32_t land0 = 0xFFFFFFFFu, landN = 0;
     if (active) {
-        const uint32_t pl = A.prime_level;
+        // Capped at the scan level, so every leaf group of the shared
+        // traversal lies wholly inside or wholly outside the primed range.
+        const uint32_t pl = min(A.prime_level, stop);
         uint32_t n = 0;
         for (uint32_t l = 0; l < pl; ++l) {
             const float* nd = A.nodes + (size_t)n * nstride;
@@ -95,8 +96,9 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
             } else {
                 uint32_t slot0, count;
                 leaf_range(A, node, level, slot0, count);
-                packet_scan<D>(A.leaves, uni_b(slot0), uni_b(count), single_bucket, q, active, land0, landN, buf,
-                               worst, whi, nslots);
+                const uint32_t s0 = uni_b(slot0);
+                packet_scan<D>(A.leaves, s0, uni_b(count), q, active && s0 - land0 >= landN, buf, worst, whi,
+                               nslots);
             }
             bool found = false;
             while (sp > 0) {

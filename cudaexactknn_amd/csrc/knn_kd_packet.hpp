@@ -28,19 +28,35 @@ namespace knn {
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float unif(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 
-// Sequential-order certified bound of q to the box [lo, hi].
+// The wave-uniform box faces: kept in SGPRs (readfirstlane) for small D,
+// where that frees VGPRs; for larger D there are too many faces for SGPRs.
+template <bool SCALAR>
+__device__ __forceinline__ float box_reg(float v) {
+    if constexpr (SCALAR) return unif(v);
+    else return v;
+}
+
+// Squared face gaps of q to the box [lo, hi], one term per dim (0 where q
+// lies inside the box in that dim); branch-free.
 template <int D>
-__device__ __forceinline__ float box_bound(const float (&q)[D], const float (&lo)[D], const float (&hi)[D]) {
+__device__ __forceinline__ void box_terms(const float (&q)[D], const float (&lo)[D], const float (&hi)[D],
+                                          float (&g)[D]) {
 #pragma clang fp contract(off)
-    float r = 0.0f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        float t = 0.0f;
-        if (q[d] < lo[d]) t = q[d] - lo[d];
-        else if (q[d] > hi[d]) t = q[d] - hi[d];
-        const float s = t * t;
-        r = d == 0 ? s : r + s;
+        const float a = q[d] - lo[d], b = q[d] - hi[d];
+        const float t = q[d] < lo[d] ? a : (q[d] > hi[d] ? b : 0.0f);
+        g[d] = t * t;
     }
+}
+
+// Sequential FP32 sum of the gap terms: the certified bound of q to the box.
+template <int D>
+__device__ __forceinline__ float seq_sum(const float (&g)[D]) {
+#pragma clang fp contract(off)
+    float r = g[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) r = r + g[d];
     return r;
 }
 
@@ -67,7 +83,6 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
 
     const uint32_t SD = A.split_dims, stop = A.stop_level;
     const uint32_t stride = 1 + D;
-    const bool single_bucket = stop == A.node_depth;
 
     float lo[D], hi[D];
 #pragma unroll
@@ -80,7 +95,9 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
     // shared traversal skips that bucket for this lane.
     uint32_t land0 = 0xFFFFFFFFu, landN = 0;
     if (active) {
-        const uint32_t pl = A.prime_level;
+        // Capped at the scan level, so every leaf group of the shared
+        // traversal lies wholly inside or wholly outside the primed range.
+        const uint32_t pl = min(A.prime_level, stop);
         uint32_t n = 0, dd = 0;
         for (uint32_t l = 0; l < pl; ++l) {
             const float sv = A.nodes[n];
@@ -104,29 +121,48 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
     const uint32_t prime_bytes = nbytes, prime_slots = nslots;
     nbytes = nslots = 0;
     uint32_t node = 0, level = 0, dim = 0, sp = 0;
-    float cur = 0.0f;  // this lane's bound for the current node's box
+    // This lane's squared face gaps to the current node's box, and their
+    // sequential sum (its certified bound to the box).  A child's box differs
+    // from its parent's only at the split face, and only the far child's gap
+    // in the split dim changes (to (q_dim - split)^2), so for small D each
+    // node costs one term and a D-term sum; pops recompute the terms from
+    // the stored box.  For larger D the D extra registers cost more occupancy
+    // than the terms save, and the far bound is recomputed from the box.
+    constexpr bool kIncremental = D <= 4;
+    float g[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) g[d] = 0.0f;
+    float cur = 0.0f;
 
     if (__ballot(active) != 0ull) {
         for (;;) {
             if (level < stop) {
-                const float split = unif(A.nodes[node]);
+                const float split = A.nodes[uni(node)];
                 nbytes += 4u;
-                // Far-side bound: the current box with the dim face moved to the
-                // split.  Every dim-indexed update is written as a select over all
-                // dims, so no array is ever indexed by the runtime dim (that would
-                // put it in scratch).
+                // Every dim-indexed access is a select over all dims, so no
+                // array is indexed by the runtime dim (that would put it in
+                // scratch).
                 float qd = q[0];
 #pragma unroll
                 for (int d = 1; d < D; ++d) qd = ((uint32_t)d == dim) ? q[d] : qd;
                 const bool left_side = qd < split;  // near child of this lane is the left one
-                float flo[D], fhi[D];
+                float fg[D];
+                if constexpr (kIncremental) {
+                    const float t = qd - split;
+                    const float fsq = t * t;
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const bool here = (uint32_t)d == dim;
-                    flo[d] = (here && left_side) ? split : lo[d];
-                    fhi[d] = (here && !left_side) ? split : hi[d];
+                    for (int d = 0; d < D; ++d) fg[d] = ((uint32_t)d == dim) ? fsq : g[d];
+                } else {
+                    float flo[D], fhi[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const bool here = (uint32_t)d == dim;
+                        flo[d] = (here && left_side) ? split : lo[d];
+                        fhi[d] = (here && !left_side) ? split : hi[d];
+                    }
+                    box_terms<D>(q, flo, fhi, fg);
                 }
-                const float far = box_bound<D>(q, flo, fhi);
+                const float far = seq_sum<D>(fg);
                 const float bl = left_side ? cur : far;
                 const float br = left_side ? far : cur;
                 const uint64_t needL = __ballot(active && f2u(bl) <= whi);
@@ -152,12 +188,14 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
                     __builtin_amdgcn_wave_barrier();
                     ++sp;
                     node = goLeft ? l : l + 1;
-                    cur = goLeft ? bl : br;
+                    const bool to_far = goLeft != left_side;
+                    cur = to_far ? far : cur;
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
                         const bool here = (uint32_t)d == dim;
-                        hi[d] = (here && goLeft) ? split : hi[d];
-                        lo[d] = (here && !goLeft) ? split : lo[d];
+                        if constexpr (kIncremental) g[d] = to_far ? fg[d] : g[d];
+                        hi[d] = box_reg<kIncremental>((here && goLeft) ? split : hi[d]);
+                        lo[d] = box_reg<kIncremental>((here && !goLeft) ? split : lo[d]);
                     }
                     ++level;
                     dim = (dim + 1 == SD) ? 0 : dim + 1;
@@ -166,12 +204,14 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
                 if (needL || needR) {
                     const bool goLeft = needL != 0ull;
                     node = goLeft ? l : l + 1;
-                    cur = goLeft ? bl : br;
+                    const bool to_far = goLeft != left_side;
+                    cur = to_far ? far : cur;
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
                         const bool here = (uint32_t)d == dim;
-                        hi[d] = (here && goLeft) ? split : hi[d];
-                        lo[d] = (here && !goLeft) ? split : lo[d];
+                        if constexpr (kIncremental) g[d] = to_far ? fg[d] : g[d];
+                        hi[d] = box_reg<kIncremental>((here && goLeft) ? split : hi[d]);
+                        lo[d] = box_reg<kIncremental>((here && !goLeft) ? split : lo[d]);
                     }
                     ++level;
                     dim = (dim + 1 == SD) ? 0 : dim + 1;
@@ -184,8 +224,9 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
                 QPoint<D> qp;
 #pragma unroll
                 for (int d = 0; d < D; ++d) qp.v[d] = q[d];
-                packet_scan<D>(A.leaves, uni(slot0), uni(count), single_bucket, qp, active, land0, landN, buf, worst,
-                               whi, nslots);
+                const uint32_t s0 = uni(slot0);
+                packet_scan<D>(A.leaves, s0, uni(count), qp, active && s0 - land0 >= landN, buf, worst, whi,
+                               nslots);
             }
             // pop until some lane needs the entry
             bool found = false;
@@ -198,12 +239,18 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
                     lo[d] = unif(__uint_as_float(e[1 + d]));
                     hi[d] = unif(__uint_as_float(e[1 + D + d]));
                 }
-                const float b = box_bound<D>(q, lo, hi);
+                float pg[D];
+                box_terms<D>(q, lo, hi, pg);
+                const float b = seq_sum<D>(pg);
                 if (__ballot(active && f2u(b) <= whi) == 0ull) continue;
                 node = n2;
                 level = 31u - __clz(node + 1);
                 dim = level_dim(level, SD, A.sd_inv);
                 cur = b;
+                if constexpr (kIncremental) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) g[d] = pg[d];
+                }
                 found = true;
                 break;
             }

@@ -108,52 +108,42 @@ __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uin
     }
 }
 
-// Wave-cooperative scan of leaf slots [slot0, slot0 + count) (count, slot0
-// wave-uniform).  Slot addresses are wave-uniform, so slots are read with
-// scalar loads: one load per slot for the index and the coordinates together,
-// issued one slot ahead, so a slot's test overlaps the next slot's load
-// (reading the index first and the coordinates after its -1 check costs two
-// dependent load latencies per slot).  Every lane tests each slot against its
-// own query.  Slots inside the lane's primed range [land0, land0 + landN) were
-// already considered and are skipped: per group when no lane's group is
-// larger than that range (the group then lies wholly inside or outside it),
-// else slot by slot.
+// Wave-cooperative scan of one leaf group [slot0, slot0 + count) (count,
+// slot0 wave-uniform).  Slot addresses are wave-uniform, so slots are read
+// with scalar loads: one load per slot for the index and the coordinates
+// together, issued one slot ahead, so a slot's test overlaps the next slot's
+// load (the leaves array has a spare slot at its end for the last look-ahead).
+// Every lane tests each slot against its own query; the loop body has one
+// branch, around the insertion, into which padding slots (index -1) fold.
+// `mine` is false for a lane whose primed range holds this group: groups are
+// subtrees at the scan level and the primed range is a subtree at or above it,
+// so a group lies wholly inside or wholly outside it.
 template <int D, class Q, class B>
 __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
-                                            bool stop_at_empty, const Q& q, bool active, uint32_t land0,
-                                            uint32_t landN, B& buf, uint64_t& worst, uint32_t& whi,
+                                            const Q& q, bool mine, B& buf, uint64_t& worst, uint32_t& whi,
                                             uint32_t& nslots) {
-    constexpr uint32_t stride = 1 + D;
-    const bool slotwise = __ballot(active && count > landN) != 0ull;
-    const bool mine_group = active && !(slot0 - land0 < landN);
-    float v[stride];
+    constexpr uint32_t S = 1 + D;
+    const float* e = leaves + (size_t)slot0 * S;
+    float v[S];
 #pragma unroll
-    for (uint32_t t = 0; t < stride; ++t) v[t] = leaves[(size_t)slot0 * stride + t];
+    for (uint32_t t = 0; t < S; ++t) v[t] = e[t];
     for (uint32_t o = 0; o < count; ++o) {
-        // The next slot's load is issued before this slot is tested (the
-        // last slot re-reads itself).
-        float nx[stride];
-        const float* e = leaves + (size_t)(slot0 + min(o + 1, count - 1)) * stride;
+        float nx[S];
 #pragma unroll
-        for (uint32_t t = 0; t < stride; ++t) nx[t] = e[t];
-        ++nslots;
-        if (__float_as_int(v[0]) == -1) {
-            if (stop_at_empty) break;
-        } else {
-            const uint32_t sl = slot0 + o;
-            const float d2 = q.dist_to(v + 1);
-            const bool mine = slotwise ? (active && sl - land0 >= landN) : mine_group;
-            if (mine && f2u(d2) <= whi) {
-                const uint64_t key = make_key(d2, (uint32_t)__float_as_int(v[0]));
-                if (key < worst) {
-                    worst = buf.consider(key);
-                    whi = key_hi(worst);
-                }
+        for (uint32_t t = 0; t < S; ++t) nx[t] = e[(size_t)(o + 1) * S + t];
+        const int32_t pi = __float_as_int(v[0]);
+        const float d2 = q.dist_to(v + 1);
+        if (mine && pi != -1 && f2u(d2) <= whi) {
+            const uint64_t key = make_key(d2, (uint32_t)pi);
+            if (key < worst) {
+                worst = buf.consider(key);
+                whi = key_hi(worst);
             }
         }
 #pragma unroll
-        for (uint32_t t = 0; t < stride; ++t) v[t] = nx[t];
+        for (uint32_t t = 0; t < S; ++t) v[t] = nx[t];
     }
+    nslots += count;
 }
 
 // Leaf-slot range under `node` (at `level`), scanning the full subtree when
