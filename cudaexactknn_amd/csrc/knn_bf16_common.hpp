@@ -285,7 +285,7 @@ __device__ __forceinline__ bool bf16_rerank_list(const Bf16Set& S, uint32_t qi, 
         const uint32_t* list = S.cand + li * S.cap;
         for (uint32_t j = 0; j < c; ++j) {
             const uint32_t pt = list[j];
-            const float d2 = dist2_rt(q, 1, S.pts + (size_t)pt * S.D, S.D);
+            const float d2 = dist2_rows(q, S.pts + (size_t)pt * S.D, S.D);
             if (f2u(d2) <= whi) {
                 const uint64_t key = make_key(d2, pt);
                 if (key < worst) {
@@ -316,7 +316,7 @@ __device__ __forceinline__ bool bf16_rerank_one(const Bf16Set& S, uint32_t qi) {
         const uint32_t c = cnt[l];
         for (uint32_t j = 0; j < c; ++j) {
             const uint32_t pt = list[j];
-            const float d2 = dist2_rt(q, 1, S.pts + (size_t)pt * S.D, S.D);
+            const float d2 = dist2_rows(q, S.pts + (size_t)pt * S.D, S.D);
             if (f2u(d2) <= whi) {
                 const uint64_t key = make_key(d2, pt);
                 if (key < worst) {

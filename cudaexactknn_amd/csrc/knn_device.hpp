@@ -59,6 +59,40 @@ __device__ __forceinline__ float dist2_rt(const float* __restrict__ q, uint32_t 
     return r;
 }
 
+// Runtime-D distance of two rows in global memory, 16-byte loads when both
+// rows are 16-byte aligned and D % 4 == 0 (same sequential order and
+// rounding as dist2_rt).
+__device__ __forceinline__ float dist2_rows(const float* __restrict__ q, const float* __restrict__ p, uint32_t D) {
+#pragma clang fp contract(off)
+    if (((D & 3u) | (((uintptr_t)q | (uintptr_t)p) & 15u)) == 0) {
+        const float4* __restrict__ q4 = reinterpret_cast<const float4*>(q);
+        const float4* __restrict__ p4 = reinterpret_cast<const float4*>(p);
+        float4 a = q4[0], b = p4[0];
+        float t = a.x - b.x;
+        float r = t * t;
+        t = a.y - b.y;
+        r = r + t * t;
+        t = a.z - b.z;
+        r = r + t * t;
+        t = a.w - b.w;
+        r = r + t * t;
+        for (uint32_t i = 1; i < (D >> 2); ++i) {
+            a = q4[i];
+            b = p4[i];
+            t = a.x - b.x;
+            r = r + t * t;
+            t = a.y - b.y;
+            r = r + t * t;
+            t = a.z - b.z;
+            r = r + t * t;
+            t = a.w - b.w;
+            r = r + t * t;
+        }
+        return r;
+    }
+    return dist2_rt(q, 1, p, D);
+}
+
 // Select q[dim] for a runtime dim without dynamic register indexing.
 template <int D>
 __device__ __forceinline__ float select_dim(const float (&q)[D], uint32_t dim) {
