@@ -46,7 +46,7 @@ def candidates(kind: str, dims: int, k: int) -> list[dict]:
         for bs, heap, lds in itertools.product((64, 128, 256), (False, True), (False, True)):
             out.append(dict(BLOCK_SIZE=bs, USE_HEAP_TRACKING=heap, USE_TRACKING_SHMEM=lds, form=FORM_PLAIN))
         if k <= 32 and dims <= 256:
-            for heap, tile in itertools.product((False, True), (1, 2, 4, 8)):
+            for heap, tile in itertools.product((False, True), (0, 1, 2, 4, 8, 16, 32)):
                 out.append(dict(BLOCK_SIZE=256, USE_HEAP_TRACKING=heap, USE_TRACKING_SHMEM=False, form=FORM_BF16,
                                 tile=tile))
         return out

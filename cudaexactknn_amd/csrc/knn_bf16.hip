@@ -58,9 +58,11 @@ __global__ void bf16_norms_kernel(const float* __restrict__ pts, uint32_t n, uin
 template <int D16, int KC, bool SEED>
 __global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t tiles_per_chunk, uint32_t stage,
                                                           uint32_t ts, uint32_t vtiles) {
-    __shared__ uint4 s_hi[bf16_stage_tiles<D16>() * D16 * 64];
-    __shared__ uint4 s_lo[bf16_stage_tiles<D16>() * D16 * 64];
-    __shared__ __align__(16) float s_np[bf16_stage_tiles<D16>() * 33];
+    // Dynamic LDS of bf16_stage_bytes(D16, stage): hi tiles, lo tiles, norms.
+    extern __shared__ uint4 lds_stage[];
+    uint4* s_hi = lds_stage;
+    uint4* s_lo = s_hi + (size_t)stage * D16 * 64;
+    float* s_np = reinterpret_cast<float*>(s_lo + (size_t)stage * D16 * 64);
     const uint32_t t0 = blockIdx.y * tiles_per_chunk;
     const uint32_t t1 = min(vtiles, t0 + tiles_per_chunk);
     bf16_screen_run<D16, KC, SEED>(S, blockIdx.x, blockIdx.y, t0, t1, stage, s_hi, s_lo, s_np, ts);
@@ -242,13 +244,19 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     ScreenFn fn = nullptr, sfn = nullptr;
     uint32_t max_stage = 1;
     switch (D16) {
-        case 1: fn = screen_for<1, false>(a.K); sfn = screen_for<1, true>(a.K); max_stage = bf16_stage_tiles<1>(); break;
-        case 2: fn = screen_for<2, false>(a.K); sfn = screen_for<2, true>(a.K); max_stage = bf16_stage_tiles<2>(); break;
-        case 4: fn = screen_for<4, false>(a.K); sfn = screen_for<4, true>(a.K); max_stage = bf16_stage_tiles<4>(); break;
-        case 8: fn = screen_for<8, false>(a.K); sfn = screen_for<8, true>(a.K); max_stage = bf16_stage_tiles<8>(); break;
-        default: fn = screen_for<16, false>(a.K); sfn = screen_for<16, true>(a.K); max_stage = bf16_stage_tiles<16>(); break;
+        case 1: fn = screen_for<1, false>(a.K); sfn = screen_for<1, true>(a.K); max_stage = bf16_stage_tiles_wide<1>(); break;
+        case 2: fn = screen_for<2, false>(a.K); sfn = screen_for<2, true>(a.K); max_stage = bf16_stage_tiles_wide<2>(); break;
+        case 4: fn = screen_for<4, false>(a.K); sfn = screen_for<4, true>(a.K); max_stage = bf16_stage_tiles_wide<4>(); break;
+        case 8: fn = screen_for<8, false>(a.K); sfn = screen_for<8, true>(a.K); max_stage = bf16_stage_tiles_wide<8>(); break;
+        default: fn = screen_for<16, false>(a.K); sfn = screen_for<16, true>(a.K); max_stage = bf16_stage_tiles_wide<16>(); break;
     }
-    const uint32_t stage = a.tile ? (a.tile < max_stage ? a.tile : max_stage) : max_stage;
+    // Requested (or widest) staging, halved until it fits the device's
+    // per-workgroup LDS.
+    uint32_t stage = a.tile ? (a.tile < max_stage ? a.tile : max_stage) : max_stage;
+    while (stage > 1 && bf16_stage_bytes(D16, stage) > device_limits().max_lds_per_block) stage >>= 1;
+    const size_t stage_lds = bf16_stage_bytes(D16, stage);
+    require(stage_lds <= device_limits().max_lds_per_block, "bf16 staging exceeds the per-workgroup LDS limit",
+            KNN_ERR_UNSUPPORTED);
     // Seeding pass over every 16th tile when the point set is large and
     // D > 16: it gives each query a starting bound near the K-th distance of
     // a 1/16 sample, so the full pass emits about 3x fewer candidates while
@@ -263,9 +271,9 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
         launch_fill_u32(seed, a.nq, 0x7F800000u, a.stream);  // +inf
         const uint32_t vt = (tiles + kSeedStride - 1) / kSeedStride;
         const uint32_t vpc = (vt + S_ - 1) / S_;
-        hipLaunchKernelGGL(sfn, dim3(qblocks, S_), dim3(256), 0, a.stream, S, vpc, stage, kSeedStride, vt);
+        hipLaunchKernelGGL(sfn, dim3(qblocks, S_), dim3(256), stage_lds, a.stream, S, vpc, stage, kSeedStride, vt);
     }
-    hipLaunchKernelGGL(fn, dim3(qblocks, S_), dim3(256), 0, a.stream, S, tpc, stage, 1u, tiles);
+    hipLaunchKernelGGL(fn, dim3(qblocks, S_), dim3(256), stage_lds, a.stream, S, tpc, stage, 1u, tiles);
     const size_t lanes = (size_t)a.nq * lists;
     hipLaunchKernelGGL(rerank_for(a.K), dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, a.stream, S, partial,
                        ovf_flag, ovf, novf);

@@ -84,16 +84,28 @@ struct Bf16Set {
     float* odist;
 };
 
-// Point tiles staged per barrier pair: 8 KiB per part for D16 <= 8.
+// Point tiles staged per barrier pair: 16 KiB per part (hi or lo) in the
+// persistent multi-dataset kernel, whose occupancy its LDS must not lower;
+// in the single-dataset screen, whose occupancy is set by its registers, the
+// depth measured fastest per D16 (1M x 65k, K = 32; ms for 1 / 2 / 4 / 8 /
+// 16 tiles): D = 128: 119 / 106 / 101; D = 64: - / 77 / 72 / 71; D = 32:
+// - / - / 55 / 52 / 54; D = 16: - / - / - / 42 / 41.
 template <int D16>
-constexpr uint32_t bf16_stage_tiles() { return D16 >= 8 ? 1u : 8u / D16; }
+constexpr uint32_t bf16_stage_tiles() { return D16 >= 16 ? 1u : 16u / D16; }
+template <int D16>
+constexpr uint32_t bf16_stage_tiles_wide() {
+    return D16 == 1 ? 16u : D16 == 2 ? 8u : D16 == 4 ? 8u : D16 == 8 ? 4u : 2u;
+}
+__host__ __device__ constexpr size_t bf16_stage_bytes(uint32_t D16, uint32_t stage) {
+    return 2 * (size_t)stage * D16 * 64 * 16 + (size_t)stage * 33 * 4;
+}
 template <int D16>
 constexpr size_t bf16_stage_lds() {
-    return 2 * (size_t)bf16_stage_tiles<D16>() * D16 * 64 * 16 + (size_t)bf16_stage_tiles<D16>() * 33 * 4;
+    return bf16_stage_bytes(D16, bf16_stage_tiles<D16>());
 }
 
-// Screen tiles [t0, t1) for the 128 queries of `qblock`, `stage` (<=
-// bf16_stage_tiles<D16>()) tiles per barrier pair.  Every thread of the
+// Screen tiles [t0, t1) for the 128 queries of `qblock`, `stage` tiles per
+// barrier pair (s_hi, s_lo, s_np sized for it: bf16_stage_bytes).  Every thread of the
 // 256-wide workgroup must call it (it synchronises the workgroup).
 // Tile v of the range is point tile v * ts.  SEED: a pass over every ts-th
 // tile that emits nothing and only lowers S.seed[q] to the query's running
@@ -106,7 +118,7 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                                                 uint32_t ts = 1) {
 #pragma clang fp contract(off)
     // s_np: stage x 32 norms, then stage per-tile maxima.
-    float* __restrict__ s_npmax = s_np + bf16_stage_tiles<D16>() * 32;
+    float* __restrict__ s_npmax = s_np + stage * 32;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t qcol = lane & 31, half = lane >> 5;
     const uint32_t qi = qblock * 128 + wave * 32 + qcol;
@@ -168,17 +180,19 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
             s_hi[i] = S.phi[src];
             s_lo[i] = S.plo[src];
         }
-        {
-            // nt * 32 <= 256 norms, one per thread; per-tile maxima by a
-            // 32-lane butterfly.  Norms are >= 0 or NaN; fmaxf drops NaN,
-            // which is safe: a pair with a NaN norm has A = NaN and is
-            // always emitted.
-            const uint32_t p = (tb + (tid >> 5)) * ts * 32 + (tid & 31);
-            float v = (tid < nt * 32 && p < S.n) ? S.pnorm[p] : 0.0f;
-            if (tid < nt * 32) s_np[tid] = v;
+        // nt * 32 norms, 256 per round (one per thread; the round count is
+        // uniform, so every thread takes part in the shuffles); per-tile
+        // maxima by a 32-lane butterfly.  Norms are >= 0 or NaN; fmaxf drops
+        // NaN, which is safe: a pair with a NaN norm has A = NaN and is
+        // always emitted.
+        for (uint32_t base = 0; base < nt * 32; base += 256) {
+            const uint32_t i = base + tid;
+            const uint32_t p = (tb + (i >> 5)) * ts * 32 + (i & 31);
+            float v = (i < nt * 32 && p < S.n) ? S.pnorm[p] : 0.0f;
+            if (i < nt * 32) s_np[i] = v;
 #pragma unroll
             for (int off = 16; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-            if (tid < nt * 32 && (tid & 31) == 0) s_npmax[tid >> 5] = v;
+            if (i < nt * 32 && (i & 31) == 0) s_npmax[i >> 5] = v;
         }
         __syncthreads();
         // Software-pipelined: the next tile's three MFMAs are issued before

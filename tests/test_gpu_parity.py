@@ -284,6 +284,20 @@ def test_bf16_form_equals_plain(oracle, cuda, d):
     assert 0.0 < st["bf16_survivor_fraction"] < 0.5
 
 
+@pytest.mark.parametrize("d", [16, 32, 128])
+def test_bf16_staging_depths_agree(oracle, cuda, d):
+    """Every point-tile staging depth (tiles per barrier, up to more tiles
+    than a workgroup has threads for their norms) gives the plain result."""
+    g = np.random.Generator(np.random.Philox(1300 + d))
+    pts = g.standard_normal((20000, d)).astype(np.float32)
+    qs = g.standard_normal((300, d)).astype(np.float32)
+    qs[0] = pts[11]
+    bf = knn.BruteForceGpu(dev(pts, cuda))
+    want = oracle.bf_knn(pts, qs, 16)
+    for tile in (0, 1, 2, 4, 8, 16, 32):
+        same(run(bf, qs, 16, cuda, form=knn.FORM_BF16, tile=tile), want, f"bf16 d={d} tile={tile}")
+
+
 def test_bf16_form_overflow_falls_back(oracle, cuda):
     """Every point equidistant from the query: all are candidates, the
     candidate buffer overflows and the plain kernel recomputes the query."""
