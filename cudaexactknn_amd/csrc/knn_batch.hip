@@ -134,15 +134,12 @@ template <int D16, int KMAX>
 __device__ __forceinline__ void batch_bf16_item(const Bf16Set& S, uint32_t qblock, uint32_t t0, uint32_t t1,
                                                 uint32_t stage, uint8_t* lds) {
     if constexpr (D16 > 0) {
-        uint4* s_hi = reinterpret_cast<uint4*>(lds);
-        uint4* s_lo = s_hi + bf16_stage_tiles<D16>() * D16 * 64;
-        float* s_np = reinterpret_cast<float*>(s_lo + bf16_stage_tiles<D16>() * D16 * 64);
         if (S.K <= 8)
-            bf16_screen_run<D16, 8>(S, qblock, 0, t0, t1, stage, s_hi, s_lo, s_np);
+            bf16_screen_run<D16, 8>(S, qblock, 0, t0, t1, stage, lds);
         else if (KMAX <= 16 || S.K <= 16)
-            bf16_screen_run<D16, 16>(S, qblock, 0, t0, t1, stage, s_hi, s_lo, s_np);
+            bf16_screen_run<D16, 16>(S, qblock, 0, t0, t1, stage, lds);
         else if constexpr (KMAX > 16)
-            bf16_screen_run<D16, 32>(S, qblock, 0, t0, t1, stage, s_hi, s_lo, s_np);
+            bf16_screen_run<D16, 32>(S, qblock, 0, t0, t1, stage, lds);
     }
 }
 
@@ -219,7 +216,15 @@ static size_t bf16_lds_of(uint32_t D16) {
         default: return bf16_stage_lds<16>();
     }
 }
-static uint32_t bf16_stage_max(uint32_t D16) { return D16 >= 8 ? 1u : 8u / D16; }
+static uint32_t bf16_stage_max(uint32_t D16) {
+    switch (D16) {
+        case 1: return bf16_stage_tiles<1>();
+        case 2: return bf16_stage_tiles<2>();
+        case 4: return bf16_stage_tiles<4>();
+        case 8: return bf16_stage_tiles<8>();
+        default: return bf16_stage_tiles<16>();
+    }
+}
 
 // Exact rerank of every bf16 dataset's candidates: one lane per (query,
 // candidate list), adjacent lanes holding a query's two lists; each writes
@@ -488,11 +493,12 @@ static void plan(Batch& b) {
         const uint32_t tiles = (it.n + 31) / 32;
         const size_t words = (size_t)tiles * D16 * 64;
         bb->tiles.alloc(2 * words);
-        bb->norms.alloc(it.n);
+        bb->norms.alloc(it.n + tiles);
         Bf16Set& S = bsets[s];
         S.phi = bb->tiles.get();
         S.plo = bb->tiles.get() + words;
         S.pnorm = bb->norms.get();
+        S.ptmax = bb->norms.get() + it.n;
         S.pts = it.d_pts;
         S.n = it.n;
         S.tiles = tiles;

@@ -40,17 +40,27 @@ __global__ void bf16_prep_points_kernel(const float* __restrict__ pts, uint32_t 
 }
 
 // |p|^2 in FP32, sequential (its error is covered by gamma_{D+1}).
-__global__ void bf16_norms_kernel(const float* __restrict__ pts, uint32_t n, uint32_t D, float* __restrict__ out) {
+// |p|^2 of every point into out[0, n), and per tile of 32 points the max
+// into out[n + tile] (a 32-lane butterfly; fmaxf drops NaN, which is safe: a
+// pair with a NaN norm has A = NaN and is always emitted; 0 stands for the
+// rows past n).  One thread per row of the tiles * 32 rows.
+__global__ void bf16_norms_kernel(const float* __restrict__ pts, uint32_t n, uint32_t D, uint32_t tiles,
+                                  float* __restrict__ out) {
 #pragma clang fp contract(off)
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* p = pts + i * D;
-    float r = p[0] * p[0];
-    for (uint32_t d = 1; d < D; ++d) {
-        const float s = p[d] * p[d];
-        r = r + s;
+    float r = 0.0f;
+    if (i < n) {
+        const float* p = pts + i * D;
+        r = p[0] * p[0];
+        for (uint32_t d = 1; d < D; ++d) {
+            const float s = p[d] * p[d];
+            r = r + s;
+        }
+        out[i] = r;
     }
-    out[i] = r;
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) r = fmaxf(r, __shfl_xor(r, off, 64));
+    if ((i & 31) == 0 && i / 32 < tiles) out[n + i / 32] = r;
 }
 
 // grid (query blocks of 128, point chunks); chunk y screens tiles
@@ -58,14 +68,11 @@ __global__ void bf16_norms_kernel(const float* __restrict__ pts, uint32_t n, uin
 template <int D16, int KC, bool SEED>
 __global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t tiles_per_chunk, uint32_t stage,
                                                           uint32_t ts, uint32_t vtiles) {
-    // Dynamic LDS of bf16_stage_bytes(D16, stage): hi tiles, lo tiles, norms.
-    extern __shared__ uint4 lds_stage[];
-    uint4* s_hi = lds_stage;
-    uint4* s_lo = s_hi + (size_t)stage * D16 * 64;
-    float* s_np = reinterpret_cast<float*>(s_lo + (size_t)stage * D16 * 64);
+    // Dynamic LDS of bf16_stage_bytes(D16, stage).
+    extern __shared__ __align__(16) uint8_t lds_stage[];
     const uint32_t t0 = blockIdx.y * tiles_per_chunk;
     const uint32_t t1 = min(vtiles, t0 + tiles_per_chunk);
-    bf16_screen_run<D16, KC, SEED>(S, blockIdx.x, blockIdx.y, t0, t1, stage, s_hi, s_lo, s_np, ts);
+    bf16_screen_run<D16, KC, SEED>(S, blockIdx.x, blockIdx.y, t0, t1, stage, lds_stage, ts);
 }
 
 // Exact rerank, one lane per (query, candidate list): each lane writes its
@@ -112,7 +119,8 @@ uint32_t bf16_d16_of(uint32_t D) {
     return 16;
 }
 
-// Tiled hi/lo operands and norms of n x D points (tiles of 32 points).
+// Tiled hi/lo operands, norms and per-tile max norms of n x D points (tiles
+// of 32 points); `norms` holds n + ceil(n / 32) floats.
 void launch_bf16_prep(const float* pts, uint32_t n, uint32_t D, uint4* phi, uint4* plo, float* norms,
                       hipStream_t st) {
     const uint32_t D16 = bf16_d16_of(D);
@@ -122,7 +130,9 @@ void launch_bf16_prep(const float* pts, uint32_t n, uint32_t D, uint4* phi, uint
         hipLaunchKernelGGL(bf16_prep_points_kernel, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, st, pts, n,
                            D, D16, tiles, phi, plo);
     }
-    if (n) hipLaunchKernelGGL(bf16_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, st, pts, n, D, norms);
+    if (n)
+        hipLaunchKernelGGL(bf16_norms_kernel, dim3((tiles * 32 + 255) / 256), dim3(256), 0, st, pts, n, D, tiles,
+                           norms);
     KNN_CHECK_LAUNCH();
 }
 
@@ -133,7 +143,7 @@ void prepare_bf16(Handle& h, hipStream_t st) {
     const uint32_t tiles = (n + 31) / 32;
     const size_t words = (size_t)tiles * D16 * 64;  // uint4 per part
     h.pts_bf16.alloc(words * 16);                   // hi then lo, 8 bf16 per uint4
-    h.norms.alloc(n ? n : 1);
+    h.norms.alloc(n ? n + tiles : 1);
     uint4* phi = reinterpret_cast<uint4*>(h.pts_bf16.get());
     launch_bf16_prep(h.pts.get(), n, D, phi, phi + words, h.norms.get(), st);
 }
@@ -226,6 +236,7 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     S.phi = phi;
     S.plo = plo;
     S.pnorm = h.norms.get();
+    S.ptmax = h.norms.get() + n;
     S.pts = h.pts.get();
     S.n = n;
     S.tiles = tiles;

@@ -65,6 +65,7 @@ struct Bf16Set {
     const uint4* phi;    // tiled hi parts: tile t, k-step kk, lane l -> uint4 (8 bf16)
     const uint4* plo;    // tiled lo parts, same layout
     const float* pnorm;  // |p|^2
+    const float* ptmax;  // per tile of 32 points: max |p|^2 (NaN dropped, 0 past n)
     const float* pts;    // FP32 points (rerank and fallback)
     uint32_t n, tiles, D, K;
     const float* qs;
@@ -84,24 +85,45 @@ struct Bf16Set {
     float* odist;
 };
 
-// Point tiles staged per barrier pair: 16 KiB per part (hi or lo) in the
-// persistent multi-dataset kernel, whose occupancy its LDS must not lower;
-// in the single-dataset screen, whose occupancy is set by its registers, the
-// depth measured fastest per D16 (1M x 65k, K = 32; ms for 1 / 2 / 4 / 8 /
-// 16 tiles): D = 128: 119 / 106 / 101; D = 64: - / 77 / 72 / 71; D = 32:
-// - / - / 55 / 52 / 54; D = 16: - / - / - / 42 / 41.
+// Point tiles per staging round.  Rounds are double-buffered: the tiles of
+// round r + 1 are copied global -> LDS directly (global_load_lds, no
+// registers) while round r is screened.  8 KiB per part (hi or lo) and
+// buffer in the persistent multi-dataset kernel, whose occupancy its LDS
+// must not lower; in the single-dataset screen, whose occupancy is set by
+// its registers, the depth measured fastest (1M x 65k, K = 32, ms for 1 / 2
+// / 4 / 8 / 16 tiles): D = 128: 94 / 88; D = 64: - / 69 / 64; D = 32:
+// - / - / 52 / 54; D = 16: - / - / - / 43 / 45.
 template <int D16>
-constexpr uint32_t bf16_stage_tiles() { return D16 >= 16 ? 1u : 16u / D16; }
+constexpr uint32_t bf16_stage_tiles() { return D16 >= 8 ? 1u : 8u / D16; }
 template <int D16>
 constexpr uint32_t bf16_stage_tiles_wide() {
-    return D16 == 1 ? 16u : D16 == 2 ? 8u : D16 == 4 ? 8u : D16 == 8 ? 4u : 2u;
+    return D16 == 1 ? 8u : D16 == 2 ? 4u : D16 == 4 ? 4u : D16 == 8 ? 2u : 1u;
 }
+// Floats per buffer of norms (stage x 32) and tile maxima (stage, padded to
+// a multiple of 4 so the next buffer's norms stay 16-byte aligned).
+__host__ __device__ constexpr uint32_t bf16_np_stride(uint32_t stage) { return stage * 32 + ((stage + 3u) & ~3u); }
+// LDS of one screen workgroup: two buffers of (hi tiles, lo tiles, norms
+// and tile maxima).
 __host__ __device__ constexpr size_t bf16_stage_bytes(uint32_t D16, uint32_t stage) {
-    return 2 * (size_t)stage * D16 * 64 * 16 + (size_t)stage * 33 * 4;
+    return 2 * (2 * (size_t)stage * D16 * 64 * 16 + (size_t)bf16_np_stride(stage) * 4);
 }
 template <int D16>
 constexpr size_t bf16_stage_lds() {
     return bf16_stage_bytes(D16, bf16_stage_tiles<D16>());
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void global_void_t;
+
+// Copies `bytes` (4 or 16) per lane from global memory into LDS, lane l's
+// data at lds + l * bytes (the instruction's layout; lds is wave-uniform).
+template <int BYTES>
+__device__ __forceinline__ void dma_to_lds(const void* g, void* lds) {
+    static_assert(BYTES == 4 || BYTES == 16, "4- or 16-byte copies");
+    if constexpr (BYTES == 16)
+        __builtin_amdgcn_global_load_lds((const global_void_t*)g, (lds_void_t*)lds, 16, 0, 0);
+    else
+        __builtin_amdgcn_global_load_lds((const global_void_t*)g, (lds_void_t*)lds, 4, 0, 0);
 }
 
 // Screen tiles [t0, t1) for the 128 queries of `qblock`, `stage` tiles per
@@ -113,12 +135,9 @@ constexpr size_t bf16_stage_lds() {
 // upper bound of the K-th distance, so the full pass may start from it.
 template <int D16, int KC, bool SEED = false>
 __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qblock, uint32_t chunk, uint32_t t0,
-                                                uint32_t t1, uint32_t stage, uint4* __restrict__ s_hi,
-                                                uint4* __restrict__ s_lo, float* __restrict__ s_np,
+                                                uint32_t t1, uint32_t stage, uint8_t* __restrict__ lds,
                                                 uint32_t ts = 1) {
 #pragma clang fp contract(off)
-    // s_np: stage x 32 norms, then stage per-tile maxima.
-    float* __restrict__ s_npmax = s_np + stage * 32;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t qcol = lane & 31, half = lane >> 5;
     const uint32_t qi = qblock * 128 + wave * 32 + qcol;
@@ -170,31 +189,54 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
     uint32_t count = 0;
     constexpr uint32_t W = D16 * 64;  // uint4 words per tile and part
 
-    for (uint32_t tb = t0; tb < t1; tb += stage) {
+    // LDS: buffer b holds hi tiles at hi0 + b * stage * W, lo tiles at
+    // lo0 + b * stage * W, norms at np0 + b * bf16_np_stride(stage) and the
+    // tile maxima after them.  The compute phase issues no vector memory
+    // reads (a wait for one would also wait for the next round's copies);
+    // the compiler still waits for the copies before a tile's first norm
+    // read, so they overlap the MFMAs of the round's first two tiles.
+    uint4* __restrict__ hi0 = reinterpret_cast<uint4*>(lds);
+    uint4* __restrict__ lo0 = hi0 + 2 * (size_t)stage * W;
+    float* __restrict__ np0 = reinterpret_cast<float*>(lo0 + 2 * (size_t)stage * W);
+    // Issue the copies of the round starting at tile tb into buffer b: wave w
+    // copies words w * 64 + 256 j (a round's word count is a multiple of 64,
+    // so a wave copies whole 64-word runs); norms past n are not read (those
+    // rows are never tested).
+    auto issue = [&](uint32_t tb, uint32_t b) {
         const uint32_t nt = min(stage, t1 - tb);
-        __syncthreads();
         const uint32_t words = nt * W;
-        for (uint32_t i = tid; i < words; i += 256) {
+        uint4* dh = hi0 + (size_t)b * stage * W;
+        uint4* dl = lo0 + (size_t)b * stage * W;
+        for (uint32_t i0 = wave * 64; i0 < words; i0 += 256) {
+            const uint32_t i = i0 + lane;
             const uint32_t u = i / W;
             const size_t src = (size_t)(tb + u) * ts * W + (i - u * W);
-            s_hi[i] = S.phi[src];
-            s_lo[i] = S.plo[src];
+            dma_to_lds<16>(S.phi + src, dh + i0);
+            dma_to_lds<16>(S.plo + src, dl + i0);
         }
-        // nt * 32 norms, 256 per round (one per thread; the round count is
-        // uniform, so every thread takes part in the shuffles); per-tile
-        // maxima by a 32-lane butterfly.  Norms are >= 0 or NaN; fmaxf drops
-        // NaN, which is safe: a pair with a NaN norm has A = NaN and is
-        // always emitted.
-        for (uint32_t base = 0; base < nt * 32; base += 256) {
-            const uint32_t i = base + tid;
+        float* dn = np0 + (size_t)b * bf16_np_stride(stage);
+        for (uint32_t i0 = wave * 64; i0 < nt * 32; i0 += 256) {
+            const uint32_t i = i0 + lane;
             const uint32_t p = (tb + (i >> 5)) * ts * 32 + (i & 31);
-            float v = (i < nt * 32 && p < S.n) ? S.pnorm[p] : 0.0f;
-            if (i < nt * 32) s_np[i] = v;
-#pragma unroll
-            for (int off = 16; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-            if (i < nt * 32 && (i & 31) == 0) s_npmax[i >> 5] = v;
+            if (i < nt * 32 && p < S.n) dma_to_lds<4>(S.pnorm + p, dn + i0);
         }
+        // Tile maxima: lane u of the last wave copies tile u's (nt <= 64).
+        if (wave == 3 && lane < nt) dma_to_lds<4>(S.ptmax + (size_t)(tb + lane) * ts, dn + stage * 32);
+    };
+    if (t0 < t1) issue(t0, 0);
+    uint32_t buf = 0;
+    for (uint32_t tb = t0; tb < t1; tb += stage, buf ^= 1u) {
+        const uint32_t nt = min(stage, t1 - tb);
+        // This round's copies (issued a round ago) have landed, for every
+        // wave; and every wave is done with the other buffer, which the next
+        // round's copies may now overwrite.
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+        if (tb + stage < t1) issue(tb + stage, buf ^ 1u);
+        const uint4* __restrict__ s_hi = hi0 + (size_t)buf * stage * W;
+        const uint4* __restrict__ s_lo = lo0 + (size_t)buf * stage * W;
+        const float* __restrict__ s_np = np0 + (size_t)buf * bf16_np_stride(stage);
+        const float* __restrict__ s_npmax = s_np + stage * 32;
         // Software-pipelined: the next tile's three MFMAs are issued before
         // this tile's epilogue, so their latency overlaps it.
         auto tile_dot = [&](uint32_t u) {
@@ -219,7 +261,7 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                 float4 n4[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    n4[j] = *reinterpret_cast<const float4*>(s_np + u * 32 + 8 * j + 4 * half);
+                    n4[j] = __builtin_bit_cast(float4, *reinterpret_cast<const uint4*>(s_np + u * 32 + 8 * j + 4 * half));
                 const float Et = S.P.cA * (qn + s_npmax[u]) + S.P.absE;
                 const float T = (tau * S.P.inv_dn + Et) * 0x1.000004p+0f;  // (1 + 4u)
                 float Ithr = lane_tau * S.P.inv_up - Et;
