@@ -62,7 +62,7 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
     if (__ballot(active) != 0ull) {
         for (;;) {
             if (level < stop) {
-                const float* nd = A.nodes + (size_t)node * nstride;
+                const float* nd = A.nodes + (size_t)uni_b(node) * nstride;
                 float cl[D], cr[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
