@@ -56,14 +56,38 @@ __global__ void __launch_bounds__(bf_max_threads(D)) bf_kernel(const float* __re
         }
         __syncthreads();
         if (active) {
+            if constexpr (DP <= 4) {
+                // Points of <= 4 dims: the next point's LDS read is issued before
+                // this point is tested, so its latency overlaps the test.
+                float cur[DP];
+#pragma unroll
+                for (int d = 0; d < DP; ++d) cur[d] = tile[d];
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const float* nx = tile + min(j + 1, cnt - 1) * DP;
+                    float nxt[DP];
+#pragma unroll
+                    for (int d = 0; d < DP; ++d) nxt[d] = nx[d];
+                    const float d2 = dist2<D>(q, cur);
+                    if (f2u(d2) <= whi) {
+                        const uint64_t key = make_key(d2, c0 + base + j);
+                        if (key < worst) {
+                            worst = buf.consider(key);
+                            whi = key_hi(worst);
+                        }
+                    }
+#pragma unroll
+                    for (int d = 0; d < DP; ++d) cur[d] = nxt[d];
+                }
+            } else {
 #pragma unroll 4
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const float d2 = dist2<D>(q, tile + j * DP);
-                if (f2u(d2) <= whi) {
-                    const uint64_t key = make_key(d2, c0 + base + j);
-                    if (key < worst) {
-                        worst = buf.consider(key);
-                        whi = key_hi(worst);
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const float d2 = dist2<D>(q, tile + j * DP);
+                    if (f2u(d2) <= whi) {
+                        const uint64_t key = make_key(d2, c0 + base + j);
+                        if (key < worst) {
+                            worst = buf.consider(key);
+                            whi = key_hi(worst);
+                        }
                     }
                 }
             }
