@@ -16,7 +16,9 @@ checked bit-identical): the ball tree and both builds on the same points;
 the small uniform brute force (65,536 x 16,384, K=8, heap); the
 BF16-filtered brute force (1,048,576 x 65,536 scaled normals, D=128, K=32);
 the learned radius on the 8-D anisotropic mixture (4,194,304 x 1,048,576,
-K=32); the batched path over 256 uneven datasets (seed 4).
+K=32), beside the wave-cooperative and the per-lane traversal of the same
+trees; the batched path over 256 uneven datasets (seed 4).  `--only LEG`
+runs one of the last three legs alone (profiler runs).
 
 Roofline: the dominant kernel is the kd traversal.  Its algorithmic bytes
 per launch are Q * (4D query + 8K results + 4 order index) plus the node
@@ -171,13 +173,21 @@ def leg_learned(dev):
         tr = tree.train_radius(torch.from_numpy(sample).to(dev), K)
         t_plain, _, r_plain = _timed_query(tree, q, nq, K, dev, reps=1)
         t_l, s_l, r_l = _timed_query(tree, q, nq, K, dev, reps=1, form=knn.FORM_LEARNED)
+        # The plain HIP path of this configuration: the reference's per-lane
+        # traversal on the same tree (one timed call; the kd one runs ~17 s).
+        t_lane, _, r_lane = _timed_query(tree, q, nq, K, dev, reps=1, per_lane=True)
         out[kind] = {"plain_qps": round(nq / t_plain * 1e3, 1), "learned_qps": round(nq / t_l * 1e3, 1),
                      "speedup": round(t_plain / t_l, 3), "identical": _same(r_plain, r_l),
+                     "per_lane_qps": round(nq / t_lane * 1e3, 1),
+                     "wave_coop_vs_per_lane": round(t_lane / t_plain, 3),
+                     "learned_vs_per_lane": round(t_lane / t_l, 3),
+                     "per_lane_identical": _same(r_plain, r_lane),
                      "miss_rate": round(s_l["rerun_queries"] / nq, 5),
                      "mean_predicted_over_exact_radius": round(s_l["radius_ratio"], 4),
                      "train_ms": round(tr["train_ms"], 1), "build_ms": round(tree.build_ms(), 2)}
-        log(f"learned {kind}: plain {t_plain:.1f} ms, learned {t_l:.1f} ms, misses {s_l['rerun_queries']:.0f}, "
-            f"identical {out[kind]['identical']}")
+        log(f"learned {kind}: wave-cooperative {t_plain:.1f} ms, learned {t_l:.1f} ms, misses "
+            f"{s_l['rerun_queries']:.0f}, per-lane {t_lane:.1f} ms, identical {out[kind]['identical']} "
+            f"{out[kind]['per_lane_identical']}")
         tree.close()
     return out
 
@@ -247,6 +257,8 @@ def main() -> int:
     ap.add_argument("--leaf-span", type=int, default=0)
     ap.add_argument("--stack-lds", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the bf16 / learned / batch legs")
+    ap.add_argument("--only", choices=("bf16", "learned", "batch"),
+                    help="run only this leg and print its JSON record (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -272,6 +284,10 @@ def main() -> int:
     from cudaexactknn_amd import datasets
 
     dev = torch.device("cuda:0")
+    if args.only:
+        leg = {"bf16": leg_bf16, "learned": leg_learned, "batch": leg_batch}[args.only]
+        print(json.dumps({"leg": args.only, **leg(dev)}), flush=True)
+        return 0
     log("generating the 3-D Gaussian-mixture workload")
     pts, qs = datasets.gaussian_mixture_3d(args.points, args.queries, 32, seed=1)
     p = torch.from_numpy(pts).to(dev)

@@ -5,7 +5,7 @@
 
 namespace knn {
 
-template <int D, int KIND>
+template <int D, int KIND, bool COUNT>
 __global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shrink_a, float shrink_b) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const uint32_t BS = blockDim.x, tid = threadIdx.x;
@@ -13,31 +13,31 @@ __global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shri
     const uint32_t nwaves = (BS + 63) >> 6;
     uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth;
     uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, 1));
-    ball_packet_run<D, KIND>(A, shrink_a, shrink_b, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
+    ball_packet_run<D, KIND, COUNT>(A, shrink_a, shrink_b, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
 }
 
 using BallFn = void (*)(TreeArgs, float, float);
 
-template <int D>
+template <int D, bool C>
 static BallFn bp_pick(int kind) {
     switch (kind) {
-        case BUF_REG8: return ball_packet_kernel<D, BUF_REG8>;
-        case BUF_REG16: return ball_packet_kernel<D, BUF_REG16>;
-        case BUF_REG32: return ball_packet_kernel<D, BUF_REG32>;
-        case BUF_LDS_ARRAY: return ball_packet_kernel<D, BUF_LDS_ARRAY>;
-        case BUF_LDS_HEAP: return ball_packet_kernel<D, BUF_LDS_HEAP>;
-        case BUF_REGH8: return ball_packet_kernel<D, BUF_REGH8>;
-        case BUF_REGH16: return ball_packet_kernel<D, BUF_REGH16>;
-        default: return ball_packet_kernel<D, BUF_REGH32>;
+        case BUF_REG8: return ball_packet_kernel<D, BUF_REG8, C>;
+        case BUF_REG16: return ball_packet_kernel<D, BUF_REG16, C>;
+        case BUF_REG32: return ball_packet_kernel<D, BUF_REG32, C>;
+        case BUF_LDS_ARRAY: return ball_packet_kernel<D, BUF_LDS_ARRAY, C>;
+        case BUF_LDS_HEAP: return ball_packet_kernel<D, BUF_LDS_HEAP, C>;
+        case BUF_REGH8: return ball_packet_kernel<D, BUF_REGH8, C>;
+        case BUF_REGH16: return ball_packet_kernel<D, BUF_REGH16, C>;
+        default: return ball_packet_kernel<D, BUF_REGH32, C>;
     }
 }
 
-BallFn ball_packet_kernel_for(uint32_t D, int kind) {
+BallFn ball_packet_kernel_for(uint32_t D, int kind, bool count) {
     switch (D) {
-        case 2: return bp_pick<2>(kind);
-        case 3: return bp_pick<3>(kind);
-        case 4: return bp_pick<4>(kind);
-        case 8: return bp_pick<8>(kind);
+        case 2: return count ? bp_pick<2, true>(kind) : bp_pick<2, false>(kind);
+        case 3: return count ? bp_pick<3, true>(kind) : bp_pick<3, false>(kind);
+        case 4: return count ? bp_pick<4, true>(kind) : bp_pick<4, false>(kind);
+        case 8: return count ? bp_pick<8, true>(kind) : bp_pick<8, false>(kind);
         default: return nullptr;
     }
 }

@@ -16,7 +16,7 @@ __device__ __forceinline__ float unif_b(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-template <int D, int KIND>
+template <int D, int KIND, bool COUNT = false>
 __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_a, float shrink_b, uint32_t i,
                                                 uint32_t* stack, uint64_t* kbuf, uint32_t BS) {
     const uint32_t tid = threadIdx.x;
@@ -125,7 +125,7 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
     const uint64_t act = __ballot(active);
     const bool lead = act != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)act) - 1);
     if (active)
-        finish_query(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
+        finish_query<COUNT>(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
                      prime_slots + (lead ? nslots : 0u));
 }
 

@@ -26,7 +26,11 @@
 //    an A that is NaN is emitted (as an L that is NaN was).  The upper bound
 //    inserted into the K smallest is (A + E_t) up >= (A + E) up >= f; whether
 //    to insert is decided on A against an approximate threshold, which only
-//    affects how tight tau gets, never correctness.
+//    affects how tight tau gets, never correctness.  The kernel forms only
+//    B = |p|^2 - 2 s per pair (one FMA) and moves |q|^2 into the per-tile
+//    thresholds, T_t - |q|^2 and E_t + |q|^2, each widened by a margin that
+//    covers |A - (B + |q|^2)| <= 5.01 u (|q|^2 + max_tile |p|^2) and the
+//    roundings of the thresholds themselves.
 // 4. Rerank: exact FP32 distance (src/KnnTrees.cuh:65-77) and the usual key
 //    insertion over the candidates only.  A query whose candidates overflow
 //    the buffer is recomputed by an exact FP32 scan.
@@ -262,15 +266,26 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                     n4[j] = __builtin_bit_cast(float4, *reinterpret_cast<const uint4*>(s_np + u * 32 + 8 * j + 4 * half));
-                const float Et = S.P.cA * (qn + s_npmax[u]) + S.P.absE;
+                const float cq = qn + s_npmax[u];
+                const float Et = S.P.cA * cq + S.P.absE;
                 const float T = (tau * S.P.inv_dn + Et) * 0x1.000004p+0f;  // (1 + 4u)
-                float Ithr = lane_tau * S.P.inv_up - Et;
-                float A[16];
+                // Per pair only B = |p|^2 - 2 s (one FMA) is formed; the
+                // tests move qn to the thresholds.  With A = fl(fl(qn + |p|^2)
+                // - 2 s), |A - (B + qn)| <= 5.01 u cq (|s| <= (qn + |p|^2) / 2
+                // up to the screen's own 1e-4 error), so the margins below
+                // (>= 14 u (cq + T) after their own roundings; 2^-100 where
+                // the scaled terms would be subnormal) keep every A <= T pair
+                // at B <= Tb and keep (B + EtB) up >= (A + Et) up.
+                const float mK = cq * 0x1.0p-20f + 0x1.0p-100f;
+                const float Tb = (T - qn) + ((cq + T) * 0x1.0p-20f + 0x1.0p-100f);
+                const float EtB = (Et + qn) + mK;
+                float Ithr = lane_tau * S.P.inv_up - EtB;
+                float B[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const float4 v = n4[r >> 2];
                     const float np = (r & 3) == 0 ? v.x : (r & 3) == 1 ? v.y : (r & 3) == 2 ? v.z : v.w;
-                    A[r] = __builtin_fmaf(-2.0f, acc[r], qn + np);  // == (qn + np) - 2 acc, rounded once
+                    B[r] = __builtin_fmaf(-2.0f, acc[r], np);  // |p|^2 - 2 s, rounded once
                 }
                 // Rows at or beyond n exist only in the last tile: the row
                 // test is compiled into a separate copy of the loop, chosen by
@@ -282,13 +297,13 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                         const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
                         const bool valid = decltype(all_valid)::value || row < nvalid;
                         if constexpr (!SEED) {
-                            if (!(A[r] > T) && valid) {
+                            if (!(B[r] > Tb) && valid) {
                                 if (count < S.cap) mine[count] = t * 32 + row;
                                 ++count;
                             }
                         }
-                        if (A[r] < Ithr && valid) {
-                            float key = (A[r] + Et) * S.P.up;
+                        if (B[r] < Ithr && valid) {
+                            float key = (B[r] + EtB) * S.P.up;
 #pragma unroll
                             for (int i = 0; i < KC; ++i) {
                                 const float e = ul[i];
@@ -296,7 +311,7 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                                 key = fmaxf(e, key);
                             }
                             lane_tau = ul[KC - 1];
-                            Ithr = lane_tau * S.P.inv_up - Et;
+                            Ithr = lane_tau * S.P.inv_up - EtB;
                         }
                     }
                 };

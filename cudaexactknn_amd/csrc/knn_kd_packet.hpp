@@ -63,7 +63,7 @@ __device__ __forceinline__ float seq_sum(const float (&g)[D]) {
 // One wave's traversal for the ordered queries i = (this lane's index).
 // `stack` is this wave's LDS stack, `kbuf` this lane's LDS K-buffer slot
 // (stride BS), used only by LDS buffer kinds.
-template <int D, int KIND>
+template <int D, int KIND, bool COUNT = false>
 __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uint32_t* stack, uint64_t* kbuf,
                                               uint32_t BS) {
     constexpr int ENT = 1 + 2 * D;  // words per stack entry: node, lo[D], hi[D]
@@ -260,7 +260,7 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
     const uint64_t act = __ballot(active);
     const bool lead = act != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)act) - 1);
     if (active)
-        finish_query(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
+        finish_query<COUNT>(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
                      prime_slots + (lead ? nslots : 0u));
 }
 

@@ -166,12 +166,13 @@ __device__ __forceinline__ uint32_t level_dim(uint32_t level, uint32_t sd, uint3
     return level - sd * ((level * sd_inv) >> 16);
 }
 
-// Learned-form bookkeeping and the canonical write.
-template <class B>
+// Learned-form bookkeeping and the canonical write.  COUNT = false compiles
+// the visit counters out (their arithmetic is then dead code).
+template <bool COUNT = true, class B>
 __device__ __forceinline__ void finish_query(const TreeArgs& A, uint32_t qi, B& buf, uint64_t worst, bool tightened,
                                              uint32_t nbytes, uint32_t nslots) {
     buf.write(A.oidx + (size_t)qi * A.K, A.odist + (size_t)qi * A.K, A.K);
-    if (A.counters) {
+    if (COUNT && A.counters) {
         atomicAdd(&A.counters[0], (unsigned long long)nbytes);
         atomicAdd(&A.counters[1], (unsigned long long)nslots);
     }

@@ -14,8 +14,8 @@ using TreeFn = void (*)(TreeArgs);
 using BallFn = void (*)(TreeArgs, float, float);
 TreeFn kd_kernel_for(uint32_t D, int kind, bool stack_lds, bool& generic);
 BallFn ball_kernel_for(uint32_t D, int kind, bool stack_lds, bool& generic);
-TreeFn kd_packet_kernel_for(uint32_t D, int kind);
-BallFn ball_packet_kernel_for(uint32_t D, int kind);
+TreeFn kd_packet_kernel_for(uint32_t D, int kind, bool count);
+BallFn ball_packet_kernel_for(uint32_t D, int kind, bool count);
 int pick_buf_kind(bool heap, bool lds, uint32_t K);
 
 __global__ void morton_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D, const float* __restrict__ bbox,
@@ -156,9 +156,9 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     if (packet) {
         if (BS > 256) BS = 256;
         if (h.kind == KNN_KIND_KD)
-            kfn = kd_packet_kernel_for(I.dims, kind);
+            kfn = kd_packet_kernel_for(I.dims, kind, A.counters != nullptr);
         else
-            bfn = ball_packet_kernel_for(I.dims, kind);
+            bfn = ball_packet_kernel_for(I.dims, kind, A.counters != nullptr);
     }
     // XCD-contiguous block order: measured 10 % faster for the per-lane
     // traversal on the 16.7M-point mixture; neutral (kd) or slower (ball) for
