@@ -124,13 +124,7 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
                                             uint32_t& nslots) {
     constexpr uint32_t S = 1 + D;
     const float* e = leaves + (size_t)slot0 * S;
-    float v[S];
-#pragma unroll
-    for (uint32_t t = 0; t < S; ++t) v[t] = e[t];
-    for (uint32_t o = 0; o < count; ++o) {
-        float nx[S];
-#pragma unroll
-        for (uint32_t t = 0; t < S; ++t) nx[t] = e[(size_t)(o + 1) * S + t];
+    auto test = [&](const float (&v)[S]) {
         const int32_t pi = __float_as_int(v[0]);
         const float d2 = q.dist_to(v + 1);
         if (mine && pi != -1 && f2u(d2) <= whi) {
@@ -140,6 +134,26 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
                 whi = key_hi(worst);
             }
         }
+    };
+    // Scalar loads may return out of order, so a wait for one is a wait for
+    // all of them.  The first slot is peeled so that no load is outstanding
+    // when the loop is entered: each iteration then issues the next slot's
+    // load, tests the current slot without waiting, and waits only at its
+    // end (without the peel the wait for the loop's first slot sat at the top
+    // of every iteration, behind the just-issued look-ahead: kd traversal
+    // 12.33 -> 11.59 ms on the headline workload, 8-D mixture 1226 -> 1007 ms).
+    float v[S], nx[S];
+#pragma unroll
+    for (uint32_t t = 0; t < S; ++t) v[t] = e[t];
+#pragma unroll
+    for (uint32_t t = 0; t < S; ++t) nx[t] = e[S + t];
+    test(v);
+#pragma unroll
+    for (uint32_t t = 0; t < S; ++t) v[t] = nx[t];
+    for (uint32_t o = 1; o < count; ++o) {
+#pragma unroll
+        for (uint32_t t = 0; t < S; ++t) nx[t] = e[(size_t)(o + 1) * S + t];
+        test(v);
 #pragma unroll
         for (uint32_t t = 0; t < S; ++t) v[t] = nx[t];
     }
