@@ -88,7 +88,14 @@ void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
     h.q_vals_alt.ensure(n);
     h.q_temp.ensure(radix_temp_words(n));
     int bits;
-    const uint32_t tree_levels = h.info.node_depth < 24 ? h.info.node_depth : 24;
+    // Order by the cell of about 4 queries: points and queries that share a
+    // distribution put about equal query counts in every cell of a level, so
+    // level log2(n) - 2 already groups each wave's 64 queries spatially, and
+    // the deeper levels (most of the node array) are not read at random.
+    uint32_t tree_levels = log2_ceil_host(n);
+    tree_levels = tree_levels > 3 ? tree_levels - 2 : 1;
+    if (tree_levels > h.info.node_depth) tree_levels = h.info.node_depth;
+    if (tree_levels > 24) tree_levels = 24;
     if (a.order_morton || h.info.n == 0) {
         hipLaunchKernelGGL(morton_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D, h.bbox.get(),
                            h.q_keys.get(), h.q_vals.get());
