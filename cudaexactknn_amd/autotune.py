@@ -61,7 +61,7 @@ def candidates(kind: str, dims: int, k: int) -> list[dict]:
 
 def _default_span(kind: str, info: dict, k: int, per_lane: bool) -> int:
     """The library's default leaf span (knn_capi.hip default_span)."""
-    target = k
+    target = k // 2 if (kind == "ball" and info["dims"] >= 8 and k >= 2) else k
     mc = max(1, info["max_children"])
     s = 0 if target <= mc else math.ceil(math.log2((target + mc - 1) // mc))
     return min(s, info["node_depth"])
