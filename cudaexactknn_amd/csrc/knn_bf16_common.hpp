@@ -181,11 +181,14 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
     if (!SEED && S.seed && active) maxcap = fminf(maxcap, __uint_as_float(S.seed[qi]));
 
     // This lane's K smallest upper bounds, ascending; KC-K leading -inf pads
-    // keep the K-th at a static slot.
+    // keep the K-th at a static slot.  The K slots start at the cap (the
+    // seeding pass's bound when there is one), itself an upper bound of the
+    // K-th distance: the running bound is then min(cap, K-th smallest bound
+    // seen), still an upper bound, and only pairs below the cap are inserted.
     float ul[KC];
 #pragma unroll
-    for (int i = 0; i < KC; ++i) ul[i] = (i >= KC - (int)S.K) ? INFINITY : -INFINITY;
-    float lane_tau = INFINITY;
+    for (int i = 0; i < KC; ++i) ul[i] = (i >= KC - (int)S.K) ? maxcap : -INFINITY;
+    float lane_tau = maxcap;
     float tau = maxcap;
     // This lane's candidate list and its length.
     const size_t list = SEED ? 0 : (size_t)qi * S.lists + chunk * 2 + half;
