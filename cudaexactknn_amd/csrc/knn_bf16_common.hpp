@@ -294,27 +294,34 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                 // test is compiled into a separate copy of the loop, chosen by
                 // a tile-uniform branch.
                 const uint32_t nvalid = S.n - t * 32;
+                // One gate per pair, against the larger of the two thresholds
+                // (NaN when Tb is: a NaN B or a NaN Tb passes it, as each
+                // passes the emit test); the two tests run only behind it.
+                float gate = SEED ? Ithr : (!(Tb <= Ithr) ? Tb : Ithr);
                 auto pairs = [&](auto all_valid) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
                         const bool valid = decltype(all_valid)::value || row < nvalid;
-                        if constexpr (!SEED) {
-                            if (!(B[r] > Tb) && valid) {
-                                if (count < S.cap) mine[count] = t * 32 + row;
-                                ++count;
+                        if (!(B[r] > gate) && valid) {
+                            if constexpr (!SEED) {
+                                if (!(B[r] > Tb)) {
+                                    if (count < S.cap) mine[count] = t * 32 + row;
+                                    ++count;
+                                }
                             }
-                        }
-                        if (B[r] < Ithr && valid) {
-                            float key = (B[r] + EtB) * S.P.up;
+                            if (B[r] < Ithr) {
+                                float key = (B[r] + EtB) * S.P.up;
 #pragma unroll
-                            for (int i = 0; i < KC; ++i) {
-                                const float e = ul[i];
-                                ul[i] = fminf(e, key);
-                                key = fmaxf(e, key);
+                                for (int i = 0; i < KC; ++i) {
+                                    const float e = ul[i];
+                                    ul[i] = fminf(e, key);
+                                    key = fmaxf(e, key);
+                                }
+                                lane_tau = ul[KC - 1];
+                                Ithr = lane_tau * S.P.inv_up - EtB;
+                                gate = SEED ? Ithr : (!(Tb <= Ithr) ? Tb : Ithr);
                             }
-                            lane_tau = ul[KC - 1];
-                            Ithr = lane_tau * S.P.inv_up - EtB;
                         }
                     }
                 };
