@@ -185,6 +185,25 @@ def test_tree_queries_all_variants(oracle, cuda, kind, d):
 
 
 @pytest.mark.parametrize("kind", ["kd", "ball"])
+@pytest.mark.parametrize("d", [3, 8])
+def test_visit_counting_kernels_are_exact(oracle, cuda, kind, d):
+    """The visit-counting traversals are compiled apart from the timed ones;
+    they return the same rows, for every buffer kind, and count visits."""
+    g = np.random.Generator(np.random.Philox(31 + d))
+    pts = g.random((20000, d), dtype=np.float32)
+    qs = g.random((3000, d), dtype=np.float32)
+    t = knn.KdTreeGpu(dev(pts, cuda)) if kind == "kd" else knn.BallTreeGpu(dev(pts, cuda))
+    for k in (8, 16, 32):
+        want = {h: oracle.bf_knn(pts, qs, k, heap=h) for h in (False, True)}
+        for v in TREE_VARIANTS:
+            for per_lane in (False, True):
+                got = run(t, qs, k, cuda, count_visits=True, per_lane=per_lane, **v)
+                same(got, want[v["USE_HEAP_TRACKING"]], f"{kind} d={d} k={k} counting lane={per_lane} {v}")
+                st = t.last_query_stats()
+                assert st["node_bytes"] > 0 and st["leaf_slots"] >= len(qs), st
+
+
+@pytest.mark.parametrize("kind", ["kd", "ball"])
 def test_tree_radius_edges_and_ties(oracle, cuda, kind):
     pts, qs = pins.lattice_case(77, 900, 120, 3, side=7)
     p = dev(pts, cuda)
