@@ -124,14 +124,21 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
                                             uint32_t& nslots) {
     constexpr uint32_t S = 1 + D;
     const float* e = leaves + (size_t)slot0 * S;
+    // One compare per slot: the lane's threshold gets its sign bit set when
+    // the group is in its primed range (`mine` false) or the slot is padding
+    // (index -1), and distance bits compared as signed integers (all
+    // non-negative) never pass a negative threshold.
+    uint32_t whim = whi | (mine ? 0u : 0x80000000u);
     auto test = [&](const float (&v)[S]) {
         const int32_t pi = __float_as_int(v[0]);
         const float d2 = q.dist_to(v + 1);
-        if (mine && pi != -1 && f2u(d2) <= whi) {
+        const uint32_t padm = pi == -1 ? 0x80000000u : 0u;
+        if ((int32_t)f2u(d2) <= (int32_t)(whim | padm)) {
             const uint64_t key = make_key(d2, (uint32_t)pi);
             if (key < worst) {
                 worst = buf.consider(key);
                 whi = key_hi(worst);
+                whim = whi;
             }
         }
     };
