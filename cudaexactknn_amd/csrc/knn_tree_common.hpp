@@ -80,13 +80,18 @@ template <int D, class Q, class B>
 __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uint32_t stride, uint32_t slot0,
                                            uint32_t count, bool stop_at_empty, const Q& q, B& buf,
                                            uint64_t& worst, uint32_t& whi, uint32_t& nslots) {
+    // D = 3: 16-byte slots, each loaded one slot ahead (the last look-ahead
+    // reads the leaves array's spare slot).
+    float4 nxt;
+    if constexpr (D == 3) nxt = *reinterpret_cast<const float4*>(leaves + (size_t)slot0 * 4);
     for (uint32_t o = 0; o < count; ++o) {
         ++nslots;
         const float* e = leaves + (size_t)(slot0 + o) * stride;
         float d2;
         int32_t pi;
         if constexpr (D == 3) {
-            const float4 v = *reinterpret_cast<const float4*>(e);
+            const float4 v = nxt;
+            nxt = *reinterpret_cast<const float4*>(leaves + (size_t)(slot0 + o + 1) * 4);
             pi = __float_as_int(v.x);
             const float p[3] = {v.y, v.z, v.w};
             d2 = q.dist_to(p);
