@@ -199,8 +199,30 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     const uint4* phi = reinterpret_cast<const uint4*>(h.pts_bf16.get());
     const uint4* plo = phi + words;
 
+    ScreenFn fn = nullptr, sfn = nullptr;
+    uint32_t max_stage = 1;
+    switch (D16) {
+        case 1: fn = screen_for<1, false>(a.K); sfn = screen_for<1, true>(a.K); max_stage = bf16_stage_tiles_wide<1>(); break;
+        case 2: fn = screen_for<2, false>(a.K); sfn = screen_for<2, true>(a.K); max_stage = bf16_stage_tiles_wide<2>(); break;
+        case 4: fn = screen_for<4, false>(a.K); sfn = screen_for<4, true>(a.K); max_stage = bf16_stage_tiles_wide<4>(); break;
+        case 8: fn = screen_for<8, false>(a.K); sfn = screen_for<8, true>(a.K); max_stage = bf16_stage_tiles_wide<8>(); break;
+        default: fn = screen_for<16, false>(a.K); sfn = screen_for<16, true>(a.K); max_stage = bf16_stage_tiles_wide<16>(); break;
+    }
+    // Requested (or widest) staging, halved until it fits the device's
+    // per-workgroup LDS.
+    uint32_t stage = a.tile ? (a.tile < max_stage ? a.tile : max_stage) : max_stage;
+    while (stage > 1 && bf16_stage_bytes(D16, stage) > device_limits().max_lds_per_block) stage >>= 1;
+    const size_t stage_lds = bf16_stage_bytes(D16, stage);
+    require(stage_lds <= device_limits().max_lds_per_block, "bf16 staging exceeds the per-workgroup LDS limit",
+            KNN_ERR_UNSUPPORTED);
+    // Point chunks: enough workgroups for two full rounds of the screen's
+    // resident workgroups per CU, so no partial last round leaves CUs at a
+    // fraction of their occupancy (3 resident at D <= 64 gave a 4-workgroup
+    // CU a second round at a third of its waves).
     const uint32_t qblocks = (a.nq + 127) / 128;
-    const uint32_t want = 4 * device_limits().cu_count;
+    int per_cu = 0;
+    KNN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), 256, stage_lds));
+    const uint32_t want = 2 * device_limits().cu_count * (uint32_t)(per_cu > 0 ? per_cu : 2);
     uint32_t S_ = 1;
     if (qblocks < want) {
         S_ = (want + qblocks - 1) / qblocks;
@@ -252,22 +274,6 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     S.lists = lists;
     S.oidx = a.oidx;
     S.odist = a.odist;
-    ScreenFn fn = nullptr, sfn = nullptr;
-    uint32_t max_stage = 1;
-    switch (D16) {
-        case 1: fn = screen_for<1, false>(a.K); sfn = screen_for<1, true>(a.K); max_stage = bf16_stage_tiles_wide<1>(); break;
-        case 2: fn = screen_for<2, false>(a.K); sfn = screen_for<2, true>(a.K); max_stage = bf16_stage_tiles_wide<2>(); break;
-        case 4: fn = screen_for<4, false>(a.K); sfn = screen_for<4, true>(a.K); max_stage = bf16_stage_tiles_wide<4>(); break;
-        case 8: fn = screen_for<8, false>(a.K); sfn = screen_for<8, true>(a.K); max_stage = bf16_stage_tiles_wide<8>(); break;
-        default: fn = screen_for<16, false>(a.K); sfn = screen_for<16, true>(a.K); max_stage = bf16_stage_tiles_wide<16>(); break;
-    }
-    // Requested (or widest) staging, halved until it fits the device's
-    // per-workgroup LDS.
-    uint32_t stage = a.tile ? (a.tile < max_stage ? a.tile : max_stage) : max_stage;
-    while (stage > 1 && bf16_stage_bytes(D16, stage) > device_limits().max_lds_per_block) stage >>= 1;
-    const size_t stage_lds = bf16_stage_bytes(D16, stage);
-    require(stage_lds <= device_limits().max_lds_per_block, "bf16 staging exceeds the per-workgroup LDS limit",
-            KNN_ERR_UNSUPPORTED);
     // Seeding pass over every 16th tile when the point set is large and
     // D > 16: it gives each query a starting bound near the K-th distance of
     // a 1/16 sample, so the full pass emits about 3x fewer candidates while
