@@ -132,8 +132,12 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
     // One compare per slot: the lane's threshold gets its sign bit set when
     // the group is in its primed range (`mine` false) or the slot is padding
     // (index -1), and distance bits compared as signed integers (all
-    // non-negative) never pass a negative threshold.
-    uint32_t whim = whi | (mine ? 0u : 0x80000000u);
+    // non-negative) never pass a negative threshold.  A threshold whose own
+    // sign bit is set (maxD2 negative, -0 or a negative NaN) passes every
+    // distance under the unsigned key order, so it is clamped to the largest
+    // positive value first.
+    auto clamp_hi = [](uint32_t w) { return w > 0x7FFFFFFFu ? 0x7FFFFFFFu : w; };
+    uint32_t whim = clamp_hi(whi) | (mine ? 0u : 0x80000000u);
     auto test = [&](const float (&v)[S]) {
         const int32_t pi = __float_as_int(v[0]);
         const float d2 = q.dist_to(v + 1);
@@ -143,7 +147,7 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
             if (key < worst) {
                 worst = buf.consider(key);
                 whi = key_hi(worst);
-                whim = whi;
+                whim = clamp_hi(whi);
             }
         }
     };
