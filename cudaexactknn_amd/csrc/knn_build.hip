@@ -409,6 +409,158 @@ __global__ void leaf_fill_kernel(const float* __restrict__ pts, uint32_t D, cons
     }
 }
 
+// --- kd local finish: the deep levels of one segment inside one workgroup ---
+// Once every segment of a level holds at most LF_THREADS * IT points, one
+// workgroup per segment runs all remaining levels in LDS with the same rule
+// as the level kernels above (split at rank ceil(n/2)-1 of the split list,
+// stable partition of the other lists), so the tree is identical; the global
+// levels cost three launches and a pass over every list each.  Points get
+// local ids (their position in list 0 of the segment) through a point-id ->
+// local-id scratch map, so the lists fit in 16 bits.  Per level and list:
+// left flag per local id from the split list's positions, a block scan of
+// the flags in position order, and a scatter to the stable position.
+constexpr uint32_t LF_THREADS = 256;
+
+// Start and size of segment q (path bits, most significant = first split)
+// after l levels below a segment of n0 points.
+__device__ __forceinline__ void lf_walk(uint32_t q, uint32_t l, uint32_t n0, uint32_t& st, uint32_t& n) {
+    st = 0;
+    n = n0;
+    for (int b = (int)l - 1; b >= 0; --b) {
+        const uint32_t L = (n + 1) >> 1;
+        if ((q >> b) & 1u) { st += L; n -= L; } else { n = L; }
+    }
+}
+
+__host__ __device__ constexpr size_t lf_lds_bytes(uint32_t M, uint32_t SD) {
+    return (size_t)M * 4 + (size_t)M * 2 + M + 32 + (size_t)SD * M * 2;
+}
+
+// Largest segment size (4096, 2048 or 1024 points) whose workgroup fits in
+// 64 KiB of LDS with SD lists; 0 when none does (then every level is global).
+inline uint32_t kd_local_finish_points(uint32_t SD) {
+    for (uint32_t M = 4096; M >= 1024; M >>= 1)
+        if (lf_lds_bytes(M, SD) <= 65536) return M;
+    return 0;
+}
+
+template <int IT>
+__global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
+    const float* __restrict__ pts, uint32_t D, uint32_t SD, uint32_t N, uint32_t* __restrict__ P,
+    uint32_t* __restrict__ loc, const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
+    uint32_t d0, uint32_t nl, float* __restrict__ nodes, uint32_t* __restrict__ leafStart,
+    uint32_t* __restrict__ leafN) {
+    constexpr uint32_t M = LF_THREADS * IT;
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    uint32_t* gidOf = reinterpret_cast<uint32_t*>(lds_raw);        // M: point id per local id
+    uint16_t* E = reinterpret_cast<uint16_t*>(gidOf + M);          // M: left flags before a position
+    uint8_t* flag = reinterpret_cast<uint8_t*>(E + M);             // M: left flag per local id
+    uint32_t* scan_w = reinterpret_cast<uint32_t*>(flag + M);      // 8 words
+    uint16_t* lists = reinterpret_cast<uint16_t*>(scan_w + 8);     // SD lists of M local ids
+    const uint32_t s0 = blockIdx.x, tid = threadIdx.x;
+    const uint32_t st0 = segStart[s0], n0 = segN[s0];
+
+    // Local ids: position in list 0.  The map is written and read back by
+    // this workgroup only (its own point ids); the reads go to L2.
+    for (uint32_t p = tid; p < n0; p += LF_THREADS) {
+        const uint32_t g = P[st0 + p];
+        gidOf[p] = g;
+        lists[p] = (uint16_t)p;
+        loc[g] = p;
+    }
+    __threadfence();
+    __syncthreads();
+    for (uint32_t j = 1; j < SD; ++j)
+        for (uint32_t p = tid; p < n0; p += LF_THREADS)
+            lists[j * M + p] = (uint16_t)__hip_atomic_load(&loc[P[(size_t)j * N + st0 + p]], __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+
+    // Thread tid owns positions tid * IT + i; ist / inn = start and size of
+    // the segment holding that position at the current level.
+    uint32_t ist[IT], inn[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) { ist[i] = 0; inn[i] = n0; }
+    const uint32_t p0 = tid * IT;
+    for (uint32_t l = 0; l < nl; ++l) {
+        const uint32_t depth = d0 + l, dim = depth % SD;
+        const uint16_t* Ld = lists + dim * M;
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t p = p0 + i;
+            if (p < n0) flag[Ld[p]] = (p - ist[i]) < ((inn[i] + 1) >> 1) ? 1 : 0;
+        }
+        const uint32_t nseg = 1u << l, S = 1u << depth;
+        for (uint32_t q = tid; q < nseg; q += LF_THREADS) {
+            uint32_t st, n;
+            lf_walk(q, l, n0, st, n);
+            const uint32_t L = (n + 1) >> 1;
+            nodes[S - 1 + (s0 << l) + q] = L ? pts[(size_t)gidOf[Ld[st + L - 1]] * D + dim] : 0.0f;
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < SD; ++j) {
+            if (j == dim) continue;
+            uint16_t* Lj = lists + j * M;
+            uint32_t v[IT];
+            uint32_t fl = 0, cnt = 0;
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                const uint32_t p = p0 + i;
+                v[i] = p < n0 ? Lj[p] : 0u;
+                const uint32_t f = p < n0 ? flag[v[i]] : 0u;
+                fl |= f << i;
+                cnt += f;
+            }
+            uint32_t total;
+            uint32_t run = block_excl_scan(cnt, scan_w, &total);
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                const uint32_t p = p0 + i;
+                if (p < n0) E[p] = (uint16_t)run;
+                run += (fl >> i) & 1u;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                const uint32_t p = p0 + i;
+                if (p < n0) {
+                    const uint32_t L = (inn[i] + 1) >> 1;
+                    const uint32_t lb = (uint32_t)E[p] - (uint32_t)E[ist[i]];
+                    const uint32_t pos = ((fl >> i) & 1u) ? ist[i] + lb : ist[i] + L + (p - ist[i] - lb);
+                    v[i] |= pos << 16;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < IT; ++i)
+                if (p0 + i < n0) Lj[v[i] >> 16] = (uint16_t)(v[i] & 0xFFFFu);
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t L = (inn[i] + 1) >> 1;
+            if (p0 + i - ist[i] < L) {
+                inn[i] = L;
+            } else {
+                ist[i] += L;
+                inn[i] -= L;
+            }
+        }
+        __syncthreads();
+    }
+
+    // Leaf segment tables and the final order of the last split list.
+    const uint32_t nleaf = 1u << nl;
+    for (uint32_t q = tid; q < nleaf; q += LF_THREADS) {
+        uint32_t st, n;
+        lf_walk(q, nl, n0, st, n);
+        leafStart[(s0 << nl) + q] = st0 + st;
+        leafN[(s0 << nl) + q] = n;
+    }
+    const uint32_t lastDim = (d0 + nl - 1) % SD;
+    const uint16_t* Lf = lists + lastDim * M;
+    for (uint32_t p = tid; p < n0; p += LF_THREADS) P[(size_t)lastDim * N + st0 + p] = gidOf[Lf[p]];
+}
+
 __global__ void fill_f32_kernel(float* __restrict__ a, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
@@ -625,7 +777,15 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     launch_fill_u32(B.n.get(), 1, N, st);
     uint32_t *start = B.start.get(), *n = B.n.get(), *start2 = B.start2.get(), *n2 = B.n2.get();
     uint32_t *segOf = B.segOf.get(), *segOf2 = B.segOf2.get();
-    for (uint32_t depth = 0; depth < I.node_depth; ++depth) {
+    // Global levels until every segment fits one workgroup's LDS, then the
+    // local finish for the rest.
+    const uint32_t lfM = kd_local_finish_points(SD);
+    uint32_t d0 = I.node_depth;
+    if (lfM) {
+        d0 = 0;
+        while (d0 < I.node_depth && ((uint64_t)N + (1ull << d0) - 1) >> d0 > lfM) ++d0;
+    }
+    for (uint32_t depth = 0; depth < d0; ++depth) {
         const uint32_t S = 1u << depth;
         const uint32_t dim = depth % SD;
         hipLaunchKernelGGL(kd_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, pts, D, B.P.get() + (size_t)dim * N,
@@ -633,6 +793,16 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         level_common(B, N, S, dim, false, start, n, segOf, segOf2, st);
         partition_lists(B, N, SD, dim, S, start, n, segOf, st);
         std::swap(segOf, segOf2);
+        std::swap(start, start2);
+        std::swap(n, n2);
+    }
+    if (d0 < I.node_depth) {
+        const uint32_t nl = I.node_depth - d0;
+        const size_t lds = lf_lds_bytes(lfM, SD);
+        auto* fn = lfM == 4096 ? kd_local_finish_kernel<16> : lfM == 2048 ? kd_local_finish_kernel<8>
+                                                                          : kd_local_finish_kernel<4>;
+        hipLaunchKernelGGL(fn, dim3(1u << d0), dim3(LF_THREADS), lds, st, pts, D, SD, N, B.P.get(), B.keys.get(),
+                           start, n, d0, nl, h.nodes.get(), start2, n2);
         std::swap(start, start2);
         std::swap(n, n2);
     }

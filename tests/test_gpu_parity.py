@@ -143,6 +143,27 @@ def test_kd_structure_matches_oracle(oracle, cuda, d, sd, skips):
     assert np.array_equal(leaf_pts[m], pts[leaf_idx[m]])
 
 
+@pytest.mark.parametrize("n,d,sd,skips", [(4096, 3, 3, 0), (4097, 3, 3, 0), (50000, 3, 3, 0), (50000, 4, 3, 2),
+                                          (50000, 8, 8, 1), (40000, 16, 16, 0), (20000, 30, 30, 0)])
+def test_kd_structure_global_then_local_levels(oracle, cuda, n, d, sd, skips):
+    """Sizes where the build runs some levels over the whole point set and
+    finishes each segment inside one workgroup (segments of at most 4096,
+    2048 or 1024 points by the number of split lists; 30 lists: every level
+    global).  Ties on the first coordinate and duplicate points included."""
+    g = np.random.Generator(np.random.Philox(n + d * 7 + skips))
+    pts = g.random((n, d), dtype=np.float32)
+    pts[: n // 8, 0] = 0.5
+    pts[n // 2 : n // 2 + 64] = pts[n // 2 + 64]
+    t = knn.KdTreeGpu(dev(pts, cuda), depth_skips=skips, split_dimensions=sd)
+    o = oracle.KdTree(pts, skips, sd)
+    nodes, leaf_idx, leaf_pts, _ = t.export()
+    onodes, oleaf = o.export()
+    assert np.array_equal(nodes.view(np.uint32), onodes.view(np.uint32))
+    assert np.array_equal(leaf_idx, oleaf)
+    m = leaf_idx >= 0
+    assert np.array_equal(leaf_pts[m], pts[leaf_idx[m]])
+
+
 @pytest.mark.parametrize("d,skips", [(3, 1), (3, 0), (4, 2), (8, 1), (5, 3)])
 def test_ball_structure_matches_oracle(oracle, cuda, d, skips):
     g = np.random.Generator(np.random.Philox(500 + d * 10 + skips))
