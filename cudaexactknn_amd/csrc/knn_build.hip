@@ -94,12 +94,20 @@ void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hip
 __global__ void kd_node_kernel(const float* __restrict__ pts, uint32_t D, const uint32_t* __restrict__ Pdim,
                                uint32_t dim, const uint32_t* __restrict__ segStart,
                                const uint32_t* __restrict__ segN, uint32_t S, float* __restrict__ nodes,
-                               uint32_t* __restrict__ nStart, uint32_t* __restrict__ nN, uint32_t* __restrict__ Lcount) {
+                               uint32_t* __restrict__ nStart, uint32_t* __restrict__ nN, uint32_t* __restrict__ Lcount,
+                               uint32_t* __restrict__ pivKey, uint32_t* __restrict__ pivId) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     const uint32_t st = segStart[s], n = segN[s];
     const uint32_t L = (n + 1) / 2;
-    nodes[S - 1 + s] = L ? pts[(size_t)Pdim[st + L - 1] * D + dim] : 0.0f;
+    float v = 0.0f;
+    if (L) {
+        const uint32_t pid = Pdim[st + L - 1];
+        v = pts[(size_t)pid * D + dim];
+        pivKey[s] = ordered_bits(v);
+        pivId[s] = pid;
+    }
+    nodes[S - 1 + s] = v;
     nStart[2 * s] = st;
     nN[2 * s] = L;
     nStart[2 * s + 1] = st + L;
@@ -111,7 +119,9 @@ __global__ void kd_node_kernel(const float* __restrict__ pts, uint32_t D, const 
 __global__ void ball_node_kernel(const double* __restrict__ var, uint32_t D, const uint32_t* __restrict__ segStart,
                                  const uint32_t* __restrict__ segN, uint32_t S, uint32_t* __restrict__ nodeDim,
                                  uint32_t* __restrict__ segDim, uint32_t* __restrict__ nStart,
-                                 uint32_t* __restrict__ nN, uint32_t* __restrict__ Lcount) {
+                                 uint32_t* __restrict__ nN, uint32_t* __restrict__ Lcount,
+                                 const float* __restrict__ pts, const uint32_t* __restrict__ P, uint32_t N,
+                                 uint32_t* __restrict__ pivKey, uint32_t* __restrict__ pivId) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     uint32_t best = 0;
@@ -124,6 +134,11 @@ __global__ void ball_node_kernel(const double* __restrict__ var, uint32_t D, con
     segDim[s] = best;
     const uint32_t st = segStart[s], n = segN[s];
     const uint32_t L = (n + 1) / 2;
+    if (L) {
+        const uint32_t pid = P[(size_t)best * N + st + L - 1];
+        pivKey[s] = ordered_bits(pts[(size_t)pid * D + best]);
+        pivId[s] = pid;
+    }
     nStart[2 * s] = st;
     nN[2 * s] = L;
     nStart[2 * s + 1] = st + L;
@@ -131,27 +146,43 @@ __global__ void ball_node_kernel(const double* __restrict__ var, uint32_t D, con
     Lcount[s] = L;
 }
 
-// Left/right bit per point id (bit id of a zeroed bitmap, set for left) and
+// Left/right bit per point id (bit id of the bitmap, set for left) and
 // next-level segment per position.  The bitmap (N / 8 bytes: 2 MiB at 2^24
 // points) stays in L2 for the partition kernels' random gathers.
 __device__ __forceinline__ uint32_t left_bit(const uint32_t* __restrict__ bits, uint32_t id) {
     return (bits[id >> 5] >> (id & 31u)) & 1u;
 }
 
-__global__ void flag_kernel(const uint32_t* __restrict__ P, uint32_t N, uint32_t uniformDim,
+// Thread i handles position i and point id i.  A point is left iff its
+// (ordered key, id) in the segment's split dimension is at most the pivot's
+// (the point at rank ceil(n/2)-1 of the split list): the lists are sorted by
+// exactly that pair, so this is the split list's first ceil(n/2) positions.
+// Points are visited in id order, so each wave's ballot is two whole bitmap
+// words (no atomics, no clearing); segById carries each point's segment.
+__global__ void flag_kernel(const float* __restrict__ pts, uint32_t D, uint32_t N, uint32_t uniformDim,
                             const uint32_t* __restrict__ segDim, const uint32_t* __restrict__ segOf,
                             const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
-                            uint32_t* __restrict__ bits, uint32_t* __restrict__ segOfNext) {
-    size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    const uint32_t s = segOf[j];
-    const uint32_t st = segStart[s];
-    const uint32_t L = (segN[s] + 1) / 2;
-    const uint32_t d = segDim ? segDim[s] : uniformDim;
-    const uint32_t id = P[(size_t)d * N + j];
-    const bool left = (uint32_t)j - st < L;
-    if (left) atomicOr(&bits[id >> 5], 1u << (id & 31u));
-    segOfNext[j] = 2 * s + (left ? 0 : 1);
+                            uint32_t* __restrict__ segOfNext, uint32_t* __restrict__ segById,
+                            const uint32_t* __restrict__ pivKey, const uint32_t* __restrict__ pivId,
+                            uint32_t* __restrict__ bits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool left = false;
+    if (i < N) {
+        const uint32_t sp = segOf[i];
+        segOfNext[i] = 2 * sp + (i - segStart[sp] < (segN[sp] + 1) / 2 ? 0u : 1u);
+        const uint32_t s = segById[i];
+        const uint32_t d = segDim ? segDim[s] : uniformDim;
+        const uint32_t key = ordered_bits(pts[(size_t)i * D + d]);
+        const uint32_t pk = pivKey[s];
+        left = key < pk || (key == pk && i <= pivId[s]);
+        segById[i] = 2 * s + (left ? 0u : 1u);
+    }
+    const uint64_t m = __ballot(left);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t w = (i - lane) >> 5;  // first of the wave's two words
+    const uint32_t nwords = (N + 31) / 32;
+    if (lane == 0 && w < nwords) bits[w] = (uint32_t)m;
+    if (lane == 32 && w + 1 < nwords) bits[w + 1] = (uint32_t)(m >> 32);
 }
 
 // --- stable segmented partition of the lists ---------------------------------
@@ -578,6 +609,8 @@ namespace {
 struct LevelBufs {
     DevBuf<uint32_t> P, P2;              // D (or SD) lists of N ids, double-buffered
     DevBuf<uint32_t> segOf, segOf2;      // segment per position
+    DevBuf<uint32_t> segId;              // segment per point id
+    DevBuf<uint32_t> pivKey, pivId;      // split pivot (ordered key, id) per segment
     DevBuf<uint8_t> flag;                // left bit per point id (bitmap of 32-bit words)
     DevBuf<uint32_t> start, n, start2, n2;  // segment tables (cur / next level)
     DevBuf<uint32_t> Lcount, leftBefore, segDim;
@@ -614,6 +647,9 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
     B.P2.ensure((size_t)nl * N);
     B.segOf.ensure(N);
     B.segOf2.ensure(N);
+    B.segId.ensure(N);
+    B.pivKey.ensure(maxSeg);
+    B.pivId.ensure(maxSeg);
     B.flag.ensure(((size_t)N + 31) / 32 * 4);
     B.start.ensure(maxSeg);
     B.n.ensure(maxSeg);
@@ -671,13 +707,14 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
     if (coordD) std::swap(B.X.p, B.X2.p);
 }
 
-void level_common(LevelBufs& B, uint32_t N, uint32_t S, uint32_t uniformDim, bool perSegDim, const uint32_t* start,
-                  const uint32_t* n, const uint32_t* segOf, uint32_t* segOf2, hipStream_t st) {
+void level_common(LevelBufs& B, const float* pts, uint32_t D, uint32_t N, uint32_t S, uint32_t uniformDim,
+                  bool perSegDim, const uint32_t* start, const uint32_t* n, const uint32_t* segOf, uint32_t* segOf2,
+                  hipStream_t st) {
     exclusive_scan_u32(B.Lcount.get(), B.leftBefore.get(), S, B.temp.get(), st);
     uint32_t* bits = reinterpret_cast<uint32_t*>(B.flag.get());
-    launch_fill_u32(bits, (N + 31) / 32, 0u, st);
-    hipLaunchKernelGGL(flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, B.P.get(), N, uniformDim,
-                       perSegDim ? B.segDim.get() : nullptr, segOf, start, n, bits, segOf2);
+    hipLaunchKernelGGL(flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, D, N, uniformDim,
+                       perSegDim ? B.segDim.get() : nullptr, segOf, start, n, segOf2, B.segId.get(), B.pivKey.get(),
+                       B.pivId.get(), bits);
 }
 
 struct Timer {
@@ -773,6 +810,7 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     }
     presort(B, pts, N, D, SD, st);
     launch_fill_u32(B.segOf.get(), N, 0u, st);
+    launch_fill_u32(B.segId.get(), N, 0u, st);
     launch_fill_u32(B.start.get(), 1, 0u, st);
     launch_fill_u32(B.n.get(), 1, N, st);
     uint32_t *start = B.start.get(), *n = B.n.get(), *start2 = B.start2.get(), *n2 = B.n2.get();
@@ -789,8 +827,9 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         const uint32_t S = 1u << depth;
         const uint32_t dim = depth % SD;
         hipLaunchKernelGGL(kd_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, pts, D, B.P.get() + (size_t)dim * N,
-                           dim, start, n, S, h.nodes.get(), start2, n2, B.Lcount.get());
-        level_common(B, N, S, dim, false, start, n, segOf, segOf2, st);
+                           dim, start, n, S, h.nodes.get(), start2, n2, B.Lcount.get(), B.pivKey.get(),
+                           B.pivId.get());
+        level_common(B, pts, D, N, S, dim, false, start, n, segOf, segOf2, st);
         partition_lists(B, N, SD, dim, S, start, n, segOf, st);
         std::swap(segOf, segOf2);
         std::swap(start, start2);
@@ -837,6 +876,7 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     presort(B, pts, N, D, D, st);
     hipLaunchKernelGGL(soa_gather_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, D, B.P.get(), N, B.X.get());
     launch_fill_u32(B.segOf.get(), N, 0u, st);
+    launch_fill_u32(B.segId.get(), N, 0u, st);
     launch_fill_u32(B.start.get(), 1, 0u, st);
     launch_fill_u32(B.n.get(), 1, N, st);
     uint32_t *start = B.start.get(), *n = B.n.get(), *start2 = B.start2.get(), *n2 = B.n2.get();
@@ -849,8 +889,9 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     for (uint32_t depth = 0; depth < I.node_depth; ++depth) {
         const uint32_t S = 1u << depth;
         hipLaunchKernelGGL(ball_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, var, D, start, n, S, h.dims.get(),
-                           B.segDim.get(), start2, n2, B.Lcount.get());
-        level_common(B, N, S, 0, true, start, n, segOf, segOf2, st);
+                           B.segDim.get(), start2, n2, B.Lcount.get(), pts, B.P.get(), N, B.pivKey.get(),
+                           B.pivId.get());
+        level_common(B, pts, D, N, S, 0, true, start, n, segOf, segOf2, st);
         partition_lists(B, N, D, 0xFFFFFFFFu, S, start, n, segOf, st, D);
         std::swap(segOf, segOf2);
         std::swap(start, start2);
