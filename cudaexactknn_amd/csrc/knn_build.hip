@@ -475,6 +475,73 @@ inline uint32_t kd_local_finish_points(uint32_t SD) {
     return 0;
 }
 
+// Stable partition of one LDS list inside every segment by the left flags of
+// its local ids.  Thread tid owns positions p0 + i (p0 = tid * IT), whose
+// segment starts at ist[i] with inn[i] points.  Contains barriers.
+template <int IT>
+__device__ __forceinline__ void lf_partition(uint16_t* Lj, const uint8_t* flag, uint16_t* E, uint32_t* scan_w,
+                                             const uint32_t (&ist)[IT], const uint32_t (&inn)[IT], uint32_t p0,
+                                             uint32_t n0) {
+    uint32_t v[IT];
+    uint32_t fl = 0, cnt = 0;
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const uint32_t p = p0 + i;
+        v[i] = p < n0 ? Lj[p] : 0u;
+        const uint32_t f = p < n0 ? flag[v[i]] : 0u;
+        fl |= f << i;
+        cnt += f;
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan(cnt, scan_w, &total);
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const uint32_t p = p0 + i;
+        if (p < n0) E[p] = (uint16_t)run;
+        run += (fl >> i) & 1u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const uint32_t p = p0 + i;
+        if (p < n0) {
+            const uint32_t L = (inn[i] + 1) >> 1;
+            const uint32_t lb = (uint32_t)E[p] - (uint32_t)E[ist[i]];
+            const uint32_t pos = ((fl >> i) & 1u) ? ist[i] + lb : ist[i] + L + (p - ist[i] - lb);
+            v[i] |= pos << 16;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < IT; ++i)
+        if (p0 + i < n0) Lj[v[i] >> 16] = (uint16_t)(v[i] & 0xFFFFu);
+}
+
+// Local ids of a segment's points: gidOf[p] = list 0 entry p, and every list
+// rewritten as local ids through the point-id -> local-id map `loc` (written
+// and read back by this workgroup only, for its own point ids; the reads go
+// to L2).
+template <int IT>
+__device__ __forceinline__ void lf_load_lists(const uint32_t* __restrict__ P, uint32_t N, uint32_t nlists,
+                                              uint32_t* __restrict__ loc, uint32_t st0, uint32_t n0,
+                                              uint32_t* gidOf, uint16_t* lists) {
+    constexpr uint32_t M = LF_THREADS * IT;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t p = tid; p < n0; p += LF_THREADS) {
+        const uint32_t g = P[st0 + p];
+        gidOf[p] = g;
+        lists[p] = (uint16_t)p;
+        loc[g] = p;
+    }
+    __threadfence();
+    __syncthreads();
+    for (uint32_t j = 1; j < nlists; ++j)
+        for (uint32_t p = tid; p < n0; p += LF_THREADS)
+            lists[j * M + p] = (uint16_t)__hip_atomic_load(&loc[P[(size_t)j * N + st0 + p]], __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+}
+
 template <int IT>
 __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
     const float* __restrict__ pts, uint32_t D, uint32_t SD, uint32_t N, uint32_t* __restrict__ P,
@@ -491,21 +558,7 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
     const uint32_t s0 = blockIdx.x, tid = threadIdx.x;
     const uint32_t st0 = segStart[s0], n0 = segN[s0];
 
-    // Local ids: position in list 0.  The map is written and read back by
-    // this workgroup only (its own point ids); the reads go to L2.
-    for (uint32_t p = tid; p < n0; p += LF_THREADS) {
-        const uint32_t g = P[st0 + p];
-        gidOf[p] = g;
-        lists[p] = (uint16_t)p;
-        loc[g] = p;
-    }
-    __threadfence();
-    __syncthreads();
-    for (uint32_t j = 1; j < SD; ++j)
-        for (uint32_t p = tid; p < n0; p += LF_THREADS)
-            lists[j * M + p] = (uint16_t)__hip_atomic_load(&loc[P[(size_t)j * N + st0 + p]], __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
+    lf_load_lists<IT>(P, N, SD, loc, st0, n0, gidOf, lists);
 
     // Thread tid owns positions tid * IT + i; ist / inn = start and size of
     // the segment holding that position at the current level.
@@ -529,43 +582,8 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
             nodes[S - 1 + (s0 << l) + q] = L ? pts[(size_t)gidOf[Ld[st + L - 1]] * D + dim] : 0.0f;
         }
         __syncthreads();
-        for (uint32_t j = 0; j < SD; ++j) {
-            if (j == dim) continue;
-            uint16_t* Lj = lists + j * M;
-            uint32_t v[IT];
-            uint32_t fl = 0, cnt = 0;
-#pragma unroll
-            for (int i = 0; i < IT; ++i) {
-                const uint32_t p = p0 + i;
-                v[i] = p < n0 ? Lj[p] : 0u;
-                const uint32_t f = p < n0 ? flag[v[i]] : 0u;
-                fl |= f << i;
-                cnt += f;
-            }
-            uint32_t total;
-            uint32_t run = block_excl_scan(cnt, scan_w, &total);
-#pragma unroll
-            for (int i = 0; i < IT; ++i) {
-                const uint32_t p = p0 + i;
-                if (p < n0) E[p] = (uint16_t)run;
-                run += (fl >> i) & 1u;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < IT; ++i) {
-                const uint32_t p = p0 + i;
-                if (p < n0) {
-                    const uint32_t L = (inn[i] + 1) >> 1;
-                    const uint32_t lb = (uint32_t)E[p] - (uint32_t)E[ist[i]];
-                    const uint32_t pos = ((fl >> i) & 1u) ? ist[i] + lb : ist[i] + L + (p - ist[i] - lb);
-                    v[i] |= pos << 16;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < IT; ++i)
-                if (p0 + i < n0) Lj[v[i] >> 16] = (uint16_t)(v[i] & 0xFFFFu);
-        }
+        for (uint32_t j = 0; j < SD; ++j)
+            if (j != dim) lf_partition<IT>(lists + j * M, flag, E, scan_w, ist, inn, p0, n0);
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
             const uint32_t L = (inn[i] + 1) >> 1;
@@ -590,6 +608,197 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
     const uint32_t lastDim = (d0 + nl - 1) % SD;
     const uint16_t* Lf = lists + lastDim * M;
     for (uint32_t p = tid; p < n0; p += LF_THREADS) P[(size_t)lastDim * N + st0 + p] = gidOf[Lf[p]];
+}
+
+// --- ball local finish --------------------------------------------------------
+// The ball tree's deep levels in one workgroup per segment, as for kd, plus
+// the per-child FP64 statistics.  The level kernels compute those with
+// seg_reduce_kernel / seg_finalize_kernel; once every child holds fewer than
+// 4096 points on average they use one chunk per child and G lanes per child
+// (G set by the level's average child size), each lane summing positions
+// r, r + G, ... of list 0, then an xor butterfly whose lane-0 value is kept.
+// This kernel runs exactly that order of operations on the same values
+// (coordinates of the same points in the same list-0 order, read from an LDS
+// copy indexed by local id), so centroids, deviation sums, split dims and
+// radii are bit-identical.
+constexpr uint32_t BLF_MAXD = 8;
+
+__host__ __device__ constexpr size_t blf_lds_bytes(uint32_t M, uint32_t D) {
+    return (size_t)D * M * 4 + (size_t)M * 4 + (size_t)M * 2 + (size_t)M * 3 + 32 + (size_t)D * M * 2;
+}
+
+inline uint32_t ball_local_finish_points(uint32_t D) {
+    if (D > BLF_MAXD) return 0;
+    for (uint32_t M = 4096; M >= 1024; M >>= 1)
+        if (blf_lds_bytes(M, D) <= 65536) return M;
+    return 0;
+}
+
+// Lanes per child and the butterfly of seg_reduce_kernel for one chunk.
+__device__ __forceinline__ uint32_t blf_lanes(uint32_t N, uint32_t S2) {
+    const uint64_t avg = ((uint64_t)N + S2 - 1) / S2;
+    uint32_t g = 1;
+    while (g < 64 && (uint64_t)g * 4 < avg) g <<= 1;
+    return g;
+}
+
+template <int IT, int DM>
+__global__ void __launch_bounds__(LF_THREADS) ball_local_finish_kernel(
+    const float* __restrict__ pts, uint32_t D, uint32_t N, uint32_t* __restrict__ P, uint32_t* __restrict__ loc,
+    const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN, const double* __restrict__ var0,
+    uint32_t d0, uint32_t nl, uint32_t* __restrict__ nodeDim, float* __restrict__ ballNodes, float inflate,
+    uint32_t* __restrict__ leafStart, uint32_t* __restrict__ leafN) {
+#pragma clang fp contract(off)
+    constexpr uint32_t M = LF_THREADS * IT;
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    float* xs = reinterpret_cast<float*>(lds_raw);                 // D x M: coordinates per local id
+    uint32_t* gidOf = reinterpret_cast<uint32_t*>(xs + D * M);     // M: point id per local id
+    uint16_t* E = reinterpret_cast<uint16_t*>(gidOf + M);          // M: left flags before a position
+    uint8_t* flag = reinterpret_cast<uint8_t*>(E + M);             // M: left flag per local id
+    uint8_t* dimCur = flag + M;                                    // M: split dim per segment, this level
+    uint8_t* dimNext = dimCur + M;                                 // M: split dim per segment, next level
+    uint32_t* scan_w = reinterpret_cast<uint32_t*>(dimNext + M);   // 8 words
+    uint16_t* lists = reinterpret_cast<uint16_t*>(scan_w + 8);     // D lists of M local ids
+    const uint32_t s0 = blockIdx.x, tid = threadIdx.x;
+    const uint32_t st0 = segStart[s0], n0 = segN[s0];
+
+    lf_load_lists<IT>(P, N, D, loc, st0, n0, gidOf, lists);
+    for (uint32_t p = tid; p < n0; p += LF_THREADS)
+        for (uint32_t d = 0; d < D; ++d) xs[d * M + p] = pts[(size_t)gidOf[p] * D + d];
+    if (tid == 0) {
+        // first maximum of the segment's deviation sums (strict >, from 0)
+        uint32_t best = 0;
+        double bv = 0.0;
+        for (uint32_t d = 0; d < D; ++d) {
+            const double v = var0[(size_t)s0 * D + d];
+            if (v > bv) { bv = v; best = d; }
+        }
+        dimCur[0] = (uint8_t)best;
+    }
+    __syncthreads();
+
+    uint32_t ist[IT], inn[IT], iseg[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) { ist[i] = 0; inn[i] = n0; iseg[i] = 0; }
+    const uint32_t p0 = tid * IT;
+    const uint32_t lane = tid & 63u;
+    for (uint32_t l = 0; l < nl; ++l) {
+        const uint32_t depth = d0 + l, S = 1u << depth, nseg = 1u << l;
+        for (uint32_t q = tid; q < nseg; q += LF_THREADS) nodeDim[S - 1 + (s0 << l) + q] = dimCur[q];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t p = p0 + i;
+            if (p < n0) {
+                const uint16_t* Ld = lists + (uint32_t)dimCur[iseg[i]] * M;
+                flag[Ld[p]] = (p - ist[i]) < ((inn[i] + 1) >> 1) ? 1 : 0;
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < D; ++j) lf_partition<IT>(lists + j * M, flag, E, scan_w, ist, inn, p0, n0);
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t L = (inn[i] + 1) >> 1;
+            if (p0 + i - ist[i] < L) {
+                inn[i] = L;
+                iseg[i] = 2 * iseg[i];
+            } else {
+                ist[i] += L;
+                inn[i] -= L;
+                iseg[i] = 2 * iseg[i] + 1;
+            }
+        }
+        __syncthreads();
+
+        // Children (level depth + 1): centroids, deviation sums, radii and
+        // the children's split dims.
+        const uint32_t S2 = 2 * S, nch = 2 * nseg;
+        const uint32_t G = blf_lanes(N, S2), groups = LF_THREADS / G;
+        const uint32_t gi = tid / G, r = tid % G, base = lane & ~(G - 1);
+        for (uint32_t cb = 0; cb < nch; cb += groups) {
+            const uint32_t c = cb + gi;
+            const bool act = c < nch;
+            uint32_t st = 0, n = 0;
+            if (act) lf_walk(c, l + 1, n0, st, n);
+            double acc[DM];
+#pragma unroll
+            for (int d = 0; d < DM; ++d) acc[d] = 0.0;
+            for (uint32_t j = st + r; j < st + n; j += G) {
+                const float* x = xs + lists[j];  // x[d * M] is dimension d
+#pragma unroll
+                for (int d = 0; d < DM; ++d)
+                    if ((uint32_t)d < D) acc[d] += (double)x[d * M];
+            }
+            for (uint32_t o = G >> 1; o >= 1; o >>= 1) {
+#pragma unroll
+                for (int d = 0; d < DM; ++d) acc[d] += __shfl_xor(acc[d], (int)o, 64);
+            }
+            float cent[DM];
+#pragma unroll
+            for (int d = 0; d < DM; ++d) {
+                const double v = __shfl(acc[d], (int)base, 64);
+                cent[d] = n ? (float)(v / (double)n) : 0.0f;
+            }
+            const uint32_t g = S2 - 1 + (s0 << (l + 1)) + c;
+            float* nodeBase = ballNodes + (size_t)((g - 1) / 2) * (2 * D + 2) + ((g & 1u) ? 0 : D + 1);
+            if (act && r == 0)
+                for (uint32_t d = 0; d < D; ++d) nodeBase[d] = cent[d];
+            double dev[DM];
+#pragma unroll
+            for (int d = 0; d < DM; ++d) dev[d] = 0.0;
+            float mx = 0.0f;
+            for (uint32_t j = st + r; j < st + n; j += G) {
+                const float* x = xs + lists[j];
+#pragma unroll
+                for (int d = 0; d < DM; ++d)
+                    if ((uint32_t)d < D) {
+                        const double u = (double)x[d * M] - (double)cent[d];
+                        dev[d] += u * u;
+                    }
+                // calcDistance2(centroid, point), src/BallTree.cuh:81
+                float tt = cent[0] - x[0];
+                float d2 = tt * tt;
+#pragma unroll
+                for (int d = 1; d < DM; ++d)
+                    if ((uint32_t)d < D) {
+                        const float u = cent[d] - x[d * M];
+                        const float sq = u * u;
+                        d2 = d2 + sq;
+                    }
+                mx = d2 > mx ? d2 : mx;
+            }
+            for (uint32_t o = G >> 1; o >= 1; o >>= 1) {
+#pragma unroll
+                for (int d = 0; d < DM; ++d) dev[d] += __shfl_xor(dev[d], (int)o, 64);
+                const float om = __shfl_xor(mx, (int)o, 64);
+                mx = om > mx ? om : mx;
+            }
+            if (act && r == 0) {
+                uint32_t best = 0;
+                double bv = 0.0;
+#pragma unroll
+                for (int d = 0; d < DM; ++d)
+                    if ((uint32_t)d < D && dev[d] > bv) { bv = dev[d]; best = d; }
+                dimNext[c] = (uint8_t)best;
+                float m = 0.0f;
+                m = mx > m ? mx : m;
+                nodeBase[D] = n ? sqrtf(m) * inflate + 0x1.0p-100f : -INFINITY;
+            }
+        }
+        uint8_t* t = dimCur;
+        dimCur = dimNext;
+        dimNext = t;
+        __syncthreads();
+    }
+
+    const uint32_t nleaf = 1u << nl;
+    for (uint32_t q = tid; q < nleaf; q += LF_THREADS) {
+        uint32_t st, n;
+        lf_walk(q, nl, n0, st, n);
+        leafStart[(s0 << nl) + q] = st0 + st;
+        leafN[(s0 << nl) + q] = n;
+    }
+    for (uint32_t j = 0; j < D; ++j)
+        for (uint32_t p = tid; p < n0; p += LF_THREADS) P[(size_t)j * N + st0 + p] = gidOf[lists[j * M + p]];
 }
 
 __global__ void fill_f32_kernel(float* __restrict__ a, size_t n, float v) {
@@ -886,7 +1095,15 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     // Root centroid and deviation sums (no BallNode entry for the root).
     seg_reduce(B.X.get(), N, D, start, n, 1, 0, cent, nullptr, nullptr, B.partial, B.pmax, st);
     seg_reduce(B.X.get(), N, D, start, n, 1, 1, cent, var, nullptr, B.partial, B.pmax, st);
-    for (uint32_t depth = 0; depth < I.node_depth; ++depth) {
+    // Global levels until every segment fits one workgroup's LDS, then the
+    // local finish for the rest (D <= 8).
+    const uint32_t lfM = ball_local_finish_points(D);
+    uint32_t d0 = I.node_depth;
+    if (lfM) {
+        d0 = 0;
+        while (d0 < I.node_depth && ((uint64_t)N + (1ull << d0) - 1) >> d0 > lfM) ++d0;
+    }
+    for (uint32_t depth = 0; depth < d0; ++depth) {
         const uint32_t S = 1u << depth;
         hipLaunchKernelGGL(ball_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, var, D, start, n, S, h.dims.get(),
                            B.segDim.get(), start2, n2, B.Lcount.get(), pts, B.P.get(), N, B.pivKey.get(),
@@ -902,6 +1119,20 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         seg_reduce(B.X.get(), N, D, start, n, S2, 1, cent2, var2, h.nodes.get(), B.partial, B.pmax, st);
         std::swap(cent, cent2);
         std::swap(var, var2);
+    }
+    if (d0 < I.node_depth) {
+        const uint32_t nl = I.node_depth - d0;
+        const float inflate = 1.0f + (float)(D + 8) * 0x1.0p-23f;
+        // ball_local_finish_points gives 4096 points for D = 1, 2048 for
+        // D = 2, 3 and 1024 for D = 4 .. 8.
+        auto* fn = D > 4           ? ball_local_finish_kernel<4, 8>
+                   : lfM == 4096 ? ball_local_finish_kernel<16, 4>
+                   : lfM == 2048 ? ball_local_finish_kernel<8, 4>
+                                 : ball_local_finish_kernel<4, 4>;
+        hipLaunchKernelGGL(fn, dim3(1u << d0), dim3(LF_THREADS), blf_lds_bytes(lfM, D), st, pts, D, N, B.P.get(),
+                           B.keys.get(), start, n, var, d0, nl, h.dims.get(), h.nodes.get(), inflate, start2, n2);
+        std::swap(start, start2);
+        std::swap(n, n2);
     }
     hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N, 0u,
                        h.dims.get(), I.node_size, start, n, I.depth_skips, I.index_size, h.leaves.get());

@@ -181,6 +181,28 @@ def test_ball_structure_matches_oracle(oracle, cuda, d, skips):
     assert ok, msg
 
 
+@pytest.mark.parametrize("n,d,skips", [(4096, 3, 1), (4097, 3, 1), (50000, 3, 1), (50000, 4, 2), (30000, 8, 1),
+                                        (20000, 9, 1)])
+def test_ball_structure_global_then_local_levels(oracle, cuda, n, d, skips):
+    """Sizes where the ball build runs some levels over the whole point set
+    and finishes each segment inside one workgroup (segments of at most 2048
+    points for D = 2, 3 and 1024 for D = 4 .. 8; D = 9: every level global)."""
+    g = np.random.Generator(np.random.Philox(700 + n + d * 10 + skips))
+    pts = (g.standard_normal((n, d)) * np.exp(g.uniform(-1, 1, d))).astype(np.float32)
+    pts[n // 2 : n // 2 + 40] = pts[n // 2 + 40]
+    t = knn.BallTreeGpu(dev(pts, cuda), depth_skips=skips)
+    o = oracle.BallTree(pts, skips)
+    nodes, leaf_idx, _, dims = t.export()
+    onodes, odims, oleaf = o.export()
+    assert np.array_equal(dims, odims)
+    assert np.array_equal(leaf_idx, oleaf)
+    a, b = nodes.reshape(onodes.shape), onodes
+    ulp = np.spacing(np.abs(b)).astype(np.float64)
+    assert np.all(np.abs(a.astype(np.float64) - b) <= 2 * ulp)
+    ok, msg = pins.check_ball_structure(pts, t.info(), nodes, dims, leaf_idx)
+    assert ok, msg
+
+
 @pytest.mark.parametrize("kind", ["kd", "ball"])
 @pytest.mark.parametrize("d", [3, 4, 8, 6])
 def test_tree_queries_all_variants(oracle, cuda, kind, d):
