@@ -192,16 +192,27 @@ constexpr uint32_t PT_TILE = PT_THREADS * PT_ITEMS;
 
 __device__ __forceinline__ uint32_t list_of(uint32_t y, uint32_t skip) { return y + (y >= skip ? 1u : 0u); }
 
+// Left bits per list position: the count pass gathers them by point id and
+// stores them in position order (pbits: one bitmap of nwords words per list
+// of the launch), so the scatter pass reads them coalesced.
 __global__ void __launch_bounds__(PT_THREADS) part_count_kernel(const uint32_t* __restrict__ P, uint32_t N, uint32_t skip,
                                                                 const uint32_t* __restrict__ bits,
-                                                                uint32_t* __restrict__ cnt, uint32_t nb) {
+                                                                uint32_t* __restrict__ cnt, uint32_t nb,
+                                                                uint32_t* __restrict__ pbits, uint32_t nwords) {
     __shared__ uint32_t red[PT_THREADS / 64];
     const uint32_t* L = P + (size_t)list_of(blockIdx.y, skip) * N;
+    uint32_t* pb = pbits + (size_t)blockIdx.y * nwords;
     const size_t base = (size_t)blockIdx.x * PT_TILE;
+    const uint32_t lane = threadIdx.x & 63u;
     uint32_t c = 0;
     for (uint32_t i = threadIdx.x; i < PT_TILE; i += PT_THREADS) {
-        size_t j = base + i;
-        if (j < N) c += left_bit(bits, L[j]);
+        const size_t j = base + i;
+        const uint32_t f = j < N ? left_bit(bits, L[j]) : 0u;
+        c += f;
+        const uint64_t m = __ballot(f != 0);
+        const size_t w = (j - lane) >> 5;  // first of the wave's two words
+        if (lane == 0 && w < nwords) pb[w] = (uint32_t)m;
+        if (lane == 32 && w + 1 < nwords) pb[w + 1] = (uint32_t)(m >> 32);
     }
     for (int off = 32; off >= 1; off >>= 1) c += (uint32_t)__shfl_xor((int)c, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
@@ -220,7 +231,7 @@ __global__ void __launch_bounds__(PT_THREADS) part_count_kernel(const uint32_t* 
 // inside its wave's ballot.
 __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t* __restrict__ P, uint32_t* __restrict__ Pout,
                                                                   uint32_t N, uint32_t skip,
-                                                                  const uint32_t* __restrict__ bits,
+                                                                  const uint32_t* __restrict__ pbits, uint32_t nwords,
                                                                   const uint32_t* __restrict__ off, uint32_t nb,
                                                                   const uint32_t* __restrict__ segOf,
                                                                   const uint32_t* __restrict__ segStart,
@@ -235,6 +246,7 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
     const size_t lo = (size_t)list_of(y, skip) * N;
     const uint32_t* L = P + lo;
     uint32_t* O = Pout + lo;
+    const uint32_t* pb = pbits + (size_t)y * nwords;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Every list holds the same number of left points, so list y's block
     // offsets in the concatenated scan start at y * totalLeft.
@@ -251,7 +263,7 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
 #pragma unroll
     for (int i = 0; i < PT_ITEMS; ++i) {
         const size_t j = base + (size_t)i * PT_THREADS + tid;
-        const uint32_t f = j < N ? left_bit(bits, id[i]) : 0u;
+        const uint32_t f = j < N ? (pb[j >> 5] >> (j & 31u)) & 1u : 0u;
         fl |= f << i;
         const uint64_t m = __ballot(f != 0);
         rank[i] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -821,6 +833,7 @@ struct LevelBufs {
     DevBuf<uint32_t> segId;              // segment per point id
     DevBuf<uint32_t> pivKey, pivId;      // split pivot (ordered key, id) per segment
     DevBuf<uint8_t> flag;                // left bit per point id (bitmap of 32-bit words)
+    DevBuf<uint32_t> posFlag;            // left bit per list position, one bitmap per partitioned list
     DevBuf<uint32_t> start, n, start2, n2;  // segment tables (cur / next level)
     DevBuf<uint32_t> Lcount, leftBefore, segDim;
     DevBuf<uint32_t> cnt, temp;          // partition block counts, scan temp
@@ -860,6 +873,7 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
     B.pivKey.ensure(maxSeg);
     B.pivId.ensure(maxSeg);
     B.flag.ensure(((size_t)N + 31) / 32 * 4);
+    B.posFlag.ensure((size_t)nl * (((size_t)N + 31) / 32));
     B.start.ensure(maxSeg);
     B.n.ensure(maxSeg);
     B.start2.ensure(maxSeg);
@@ -900,11 +914,12 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
     const uint32_t nlists = (skip < nl) ? nl - 1 : nl;
     if (nlists == 0) return;
     const uint32_t nb = nblk(N, PT_TILE);
+    const uint32_t nwords = (N + 31) / 32;
     hipLaunchKernelGGL(part_count_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), N, skip,
-                       reinterpret_cast<const uint32_t*>(B.flag.get()), B.cnt.get(), nb);
+                       reinterpret_cast<const uint32_t*>(B.flag.get()), B.cnt.get(), nb, B.posFlag.get(), nwords);
     exclusive_scan_u32(B.cnt.get(), B.cnt.get(), nb * nlists, B.temp.get(), st);
     hipLaunchKernelGGL(part_scatter_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), B.P2.get(), N, skip,
-                       reinterpret_cast<const uint32_t*>(B.flag.get()), B.cnt.get(), nb, segOf, start, n,
+                       B.posFlag.get(), nwords, B.cnt.get(), nb, segOf, start, n,
                        B.leftBefore.get(), B.Lcount.get(), S, coordD ? B.X.get() : nullptr,
                        coordD ? B.X2.get() : nullptr, coordD);
     // The partitioned lists live in P2; the skipped list (already in order)
