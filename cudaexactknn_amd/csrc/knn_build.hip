@@ -39,22 +39,34 @@ __device__ __forceinline__ float from_ordered(uint32_t o) {
     return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
 }
 
-__global__ void bbox_kernel(const float* __restrict__ pts, uint32_t n, uint32_t D, uint32_t* __restrict__ omin,
-                            uint32_t* __restrict__ omax) {
-    const uint32_t d = blockIdx.y;
-    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t o = ordered_bits(pts[i * D + d]);
-        lo = min(lo, o);
-        hi = max(hi, o);
+// One coalesced pass over the flattened n x D array: the grid stride T is a
+// multiple of D, so thread t always sees dimension t % D.  Per-workgroup
+// LDS extrema, then one global atomic per dimension and workgroup.
+__global__ void bbox_kernel(const float* __restrict__ pts, size_t total, uint32_t D, uint32_t T,
+                            uint32_t* __restrict__ omin, uint32_t* __restrict__ omax) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    uint32_t* lo_s = reinterpret_cast<uint32_t*>(lds_raw);  // D
+    uint32_t* hi_s = lo_s + D;                              // D
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) {
+        lo_s[d] = 0xFFFFFFFFu;
+        hi_s[d] = 0u;
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        lo = min(lo, (uint32_t)__shfl_xor((int)lo, off, 64));
-        hi = max(hi, (uint32_t)__shfl_xor((int)hi, off, 64));
+    __syncthreads();
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < T) {
+        uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+        for (size_t i = t; i < total; i += T) {
+            const uint32_t o = ordered_bits(pts[i]);
+            lo = min(lo, o);
+            hi = max(hi, o);
+        }
+        atomicMin(&lo_s[t % D], lo);
+        atomicMax(&hi_s[t % D], hi);
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(&omin[d], lo);
-        atomicMax(&omax[d], hi);
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) {
+        atomicMin(&omin[d], lo_s[d]);
+        atomicMax(&omax[d], hi_s[d]);
     }
 }
 
@@ -73,9 +85,12 @@ void compute_bbox_ws(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, 
     launch_fill_u32(ext, D, 0xFFFFFFFFu, st);
     launch_fill_u32(ext + D, D, 0u, st);
     if (n) {
-        uint32_t gx = nblk(n, 256);
-        if (gx > 1024) gx = 1024;
-        hipLaunchKernelGGL(bbox_kernel, dim3(gx, D), dim3(256), 0, st, d_pts, n, D, ext, ext + D);
+        const size_t total = (size_t)n * D;
+        uint32_t gx = nblk(total, 256);
+        if (gx > 2048) gx = 2048;
+        const uint32_t T = gx * 256 / D * D;  // D <= 256 * gx whenever total >= D
+        hipLaunchKernelGGL(bbox_kernel, dim3(gx), dim3(256), 2 * D * sizeof(uint32_t), st, d_pts, total, D, T, ext,
+                           ext + D);
     }
     hipLaunchKernelGGL(bbox_finish_kernel, dim3(nblk(D, 64)), dim3(64), 0, st, ext, ext + D, D, d_bbox);
     KNN_CHECK_LAUNCH();

@@ -84,3 +84,20 @@ def test_round_trip_bit_identical(oracle, cuda, tmp_path, kind, skips):
     same(run(t, qs, 16, cuda), run(t2, qs, 16, cuda, per_lane=True), f"{kind} original vs restored")
     t.close()
     t2.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,d,kind", [(1, 3, "kd"), (5, 7, "kd"), (1000, 3, "ball"), (300001, 3, "kd"),
+                                      (70000, 5, "ball"), (20000, 40, "kd")])
+def test_bounding_box_is_exact(cuda, n, d, kind):
+    """The tree's bounding box (last 2 D floats of the serialized tree: minima,
+    then maxima) equals the per-dimension extrema of the points."""
+    from tests.test_gpu_parity import dev
+    g = np.random.Generator(np.random.Philox(n + d))
+    pts = (g.standard_normal((n, d)) * np.exp(g.uniform(-3, 3, d))).astype(np.float32)
+    p = dev(pts, cuda)
+    t = knn.KdTreeGpu(p, depth_skips=0) if kind == "kd" else knn.BallTreeGpu(p, depth_skips=1)
+    bbox = np.frombuffer(t.serialize()[-8 * d:], dtype=np.float32)
+    assert np.array_equal(bbox[:d], pts.min(axis=0)), "minima"
+    assert np.array_equal(bbox[d:], pts.max(axis=0)), "maxima"
+    t.close()
