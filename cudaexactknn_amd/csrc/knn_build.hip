@@ -573,8 +573,7 @@ template <int IT>
 __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
     const float* __restrict__ pts, uint32_t D, uint32_t SD, uint32_t N, uint32_t* __restrict__ P,
     uint32_t* __restrict__ loc, const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
-    uint32_t d0, uint32_t nl, float* __restrict__ nodes, uint32_t* __restrict__ leafStart,
-    uint32_t* __restrict__ leafN) {
+    uint32_t d0, uint32_t nl, float* __restrict__ nodes, uint32_t skips, float* __restrict__ leaves) {
     constexpr uint32_t M = LF_THREADS * IT;
     extern __shared__ __align__(16) uint8_t lds_raw[];
     uint32_t* gidOf = reinterpret_cast<uint32_t*>(lds_raw);        // M: point id per local id
@@ -624,17 +623,25 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
         __syncthreads();
     }
 
-    // Leaf segment tables and the final order of the last split list.
-    const uint32_t nleaf = 1u << nl;
-    for (uint32_t q = tid; q < nleaf; q += LF_THREADS) {
+    // The segment's leaf buckets (2^skips slots per leaf, padded with id -1),
+    // each in the order of the last split list.
+    const uint16_t* Lf = lists + ((d0 + nl - 1) % SD) * M;
+    const uint32_t nslot = 1u << (nl + skips), kmask = (1u << skips) - 1;
+    float* out = leaves + ((size_t)s0 << (nl + skips)) * (1 + D);
+    for (uint32_t z = tid; z < nslot; z += LF_THREADS) {
         uint32_t st, n;
-        lf_walk(q, nl, n0, st, n);
-        leafStart[(s0 << nl) + q] = st0 + st;
-        leafN[(s0 << nl) + q] = n;
+        lf_walk(z >> skips, nl, n0, st, n);
+        const uint32_t k = z & kmask;
+        float* e = out + (size_t)z * (1 + D);
+        if (k < n) {
+            const uint32_t id = gidOf[Lf[st + k]];
+            e[0] = __int_as_float((int32_t)id);
+            for (uint32_t d = 0; d < D; ++d) e[1 + d] = pts[(size_t)id * D + d];
+        } else {
+            e[0] = __int_as_float(-1);
+            for (uint32_t d = 0; d < D; ++d) e[1 + d] = 0.0f;
+        }
     }
-    const uint32_t lastDim = (d0 + nl - 1) % SD;
-    const uint16_t* Lf = lists + lastDim * M;
-    for (uint32_t p = tid; p < n0; p += LF_THREADS) P[(size_t)lastDim * N + st0 + p] = gidOf[Lf[p]];
 }
 
 // --- ball local finish --------------------------------------------------------
@@ -674,7 +681,7 @@ __global__ void __launch_bounds__(LF_THREADS) ball_local_finish_kernel(
     const float* __restrict__ pts, uint32_t D, uint32_t N, uint32_t* __restrict__ P, uint32_t* __restrict__ loc,
     const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN, const double* __restrict__ var0,
     uint32_t d0, uint32_t nl, uint32_t* __restrict__ nodeDim, float* __restrict__ ballNodes, float inflate,
-    uint32_t* __restrict__ leafStart, uint32_t* __restrict__ leafN) {
+    uint32_t skips, float* __restrict__ leaves) {
 #pragma clang fp contract(off)
     constexpr uint32_t M = LF_THREADS * IT;
     extern __shared__ __align__(16) uint8_t lds_raw[];
@@ -817,15 +824,25 @@ __global__ void __launch_bounds__(LF_THREADS) ball_local_finish_kernel(
         __syncthreads();
     }
 
-    const uint32_t nleaf = 1u << nl;
-    for (uint32_t q = tid; q < nleaf; q += LF_THREADS) {
+    // The segment's leaf buckets (2^skips slots per leaf, padded with id -1),
+    // each in the order of its parent's split list.  After the last level's
+    // swap, dimNext holds the split dims of the leaves' parents.
+    const uint32_t nslot = 1u << (nl + skips), kmask = (1u << skips) - 1;
+    float* out = leaves + ((size_t)s0 << (nl + skips)) * (1 + D);
+    for (uint32_t z = tid; z < nslot; z += LF_THREADS) {
+        const uint32_t q = z >> skips, k = z & kmask;
         uint32_t st, n;
         lf_walk(q, nl, n0, st, n);
-        leafStart[(s0 << nl) + q] = st0 + st;
-        leafN[(s0 << nl) + q] = n;
+        float* e = out + (size_t)z * (1 + D);
+        if (k < n) {
+            const uint32_t lid = lists[(uint32_t)dimNext[q >> 1] * M + st + k];
+            e[0] = __int_as_float((int32_t)gidOf[lid]);
+            for (uint32_t d = 0; d < D; ++d) e[1 + d] = xs[d * M + lid];
+        } else {
+            e[0] = __int_as_float(-1);
+            for (uint32_t d = 0; d < D; ++d) e[1 + d] = 0.0f;
+        }
     }
-    for (uint32_t j = 0; j < D; ++j)
-        for (uint32_t p = tid; p < n0; p += LF_THREADS) P[(size_t)j * N + st0 + p] = gidOf[lists[j * M + p]];
 }
 
 __global__ void fill_f32_kernel(float* __restrict__ a, size_t n, float v) {
@@ -1080,13 +1097,12 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         auto* fn = lfM == 4096 ? kd_local_finish_kernel<16> : lfM == 2048 ? kd_local_finish_kernel<8>
                                                                           : kd_local_finish_kernel<4>;
         hipLaunchKernelGGL(fn, dim3(1u << d0), dim3(LF_THREADS), lds, st, pts, D, SD, N, B.P.get(), B.keys.get(),
-                           start, n, d0, nl, h.nodes.get(), start2, n2);
-        std::swap(start, start2);
-        std::swap(n, n2);
+                           start, n, d0, nl, h.nodes.get(), I.depth_skips, h.leaves.get());
+    } else {
+        const uint32_t lastDim = (I.node_depth - 1) % SD;
+        hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N,
+                           lastDim, nullptr, I.node_size, start, n, I.depth_skips, I.index_size, h.leaves.get());
     }
-    const uint32_t lastDim = (I.node_depth - 1) % SD;
-    hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N,
-                       lastDim, nullptr, I.node_size, start, n, I.depth_skips, I.index_size, h.leaves.get());
     KNN_CHECK_LAUNCH();
     compute_bbox_ws(pts, N, D, h.bbox.get(), B.ext.get(), st);
 }
@@ -1160,12 +1176,12 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
                    : lfM == 2048 ? ball_local_finish_kernel<8, 4>
                                  : ball_local_finish_kernel<4, 4>;
         hipLaunchKernelGGL(fn, dim3(1u << d0), dim3(LF_THREADS), blf_lds_bytes(lfM, D), st, pts, D, N, B.P.get(),
-                           B.keys.get(), start, n, var, d0, nl, h.dims.get(), h.nodes.get(), inflate, start2, n2);
-        std::swap(start, start2);
-        std::swap(n, n2);
+                           B.keys.get(), start, n, var, d0, nl, h.dims.get(), h.nodes.get(), inflate, I.depth_skips,
+                           h.leaves.get());
+    } else {
+        hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N,
+                           0u, h.dims.get(), I.node_size, start, n, I.depth_skips, I.index_size, h.leaves.get());
     }
-    hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N, 0u,
-                       h.dims.get(), I.node_size, start, n, I.depth_skips, I.index_size, h.leaves.get());
     KNN_CHECK_LAUNCH();
     compute_bbox_ws(pts, N, D, h.bbox.get(), B.ext.get(), st);
 }
