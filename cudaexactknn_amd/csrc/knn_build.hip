@@ -21,6 +21,10 @@
 //  - Leaves are IndexPoint<D> buckets of 2^depthSkips slots padded with -1,
 //    in leaf order; inside a bucket points are in key order of the last
 //    split dimension.
+//  - Levels run over the whole point set until every segment fits one
+//    workgroup's LDS; then one workgroup per segment runs the remaining
+//    levels with the same rules (kd_local_finish_kernel,
+//    ball_local_finish_kernel) and writes the segment's leaf buckets.
 #include "knn_device.hpp"
 #include "knn_internal.hpp"
 
