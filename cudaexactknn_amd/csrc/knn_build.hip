@@ -776,8 +776,10 @@ __global__ void __launch_bounds__(LF_THREADS) ball_local_finish_kernel(
                 const double v = __shfl(acc[d], (int)base, 64);
                 cent[d] = n ? (float)(v / (double)n) : 0.0f;
             }
+            // BallNode slot of child c (BFS index g) in its parent's entry.
             const uint32_t g = S2 - 1 + (s0 << (l + 1)) + c;
-            float* nodeBase = ballNodes + (size_t)((g - 1) / 2) * (2 * D + 2) + ((g & 1u) ? 0 : D + 1);
+            float* nodeBase =
+                act ? ballNodes + (size_t)((g - 1) / 2) * (2 * D + 2) + ((g & 1u) ? 0 : D + 1) : nullptr;
             if (act && r == 0)
                 for (uint32_t d = 0; d < D; ++d) nodeBase[d] = cent[d];
             double dev[DM];
