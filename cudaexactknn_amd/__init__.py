@@ -206,12 +206,14 @@ class _Tree(_Structure):
         return float(_lib.load().knn_last_build_ms(self._h))
 
     def export(self):
-        """Copy the tree to host numpy arrays: (nodes, leaf_index, leaf_points[, dims])."""
+        """Copy the tree to host numpy arrays: (nodes, leaf_index, leaf_points, dims).
+        `dims` holds each node's split dimension for ball trees and zeros for kd
+        trees (whose split dimension is the level modulo SPLIT_DIMENSIONS)."""
         import numpy as np
         i = self.info()
         nodes = np.empty(i["node_size"] * i["node_bytes"] // 4, np.float32)
         leaves = np.empty(i["index_size"] * (1 + i["dims"]), np.float32)
-        dims = np.empty(max(1, i["node_size"]), np.uint32)
+        dims = np.zeros(max(1, i["node_size"]), np.uint32)
         _lib.check(_lib.load().knn_tree_export(self._h, nodes.ctypes.data, leaves.ctypes.data, dims.ctypes.data))
         leaves = leaves.reshape(i["index_size"], 1 + i["dims"])
         leaf_index = leaves[:, 0].copy().view(np.int32)
