@@ -42,19 +42,45 @@ def version() -> int:
     return int(_lib.load().knn_version())
 
 
-def _ptr(x: Any) -> int:
-    """Device pointer of a torch tensor (checked contiguous float32/int32 on a GPU) or an int."""
+def _ptr(x: Any, dtype: Any = None, what: str = "tensor", device: Any = None) -> int:
+    """Device pointer of a torch tensor, or an int passed through unchecked.
+    A tensor must live on the GPU, be contiguous, have `dtype` when one is
+    given (torch.float32 for points, queries and distances, torch.int32 for
+    indices) and sit on `device` when one is given."""
     if x is None:
         return 0
     if isinstance(x, int):
         return x
     if hasattr(x, "data_ptr"):
+        _dtype(x, dtype, what)
         if not x.is_cuda:
-            raise KnnError(_lib.KNN_ERR_INVALID_ARG, "tensor must live on the GPU")
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{what} must live on the GPU")
         if not x.is_contiguous():
-            raise KnnError(_lib.KNN_ERR_INVALID_ARG, "tensor must be contiguous")
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{what} must be contiguous")
+        if device is not None and x.device != device:
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{what} is on {x.device}, the structure on {device}")
         return int(x.data_ptr())
     raise TypeError(f"expected a device tensor or pointer, got {type(x)!r}")
+
+
+def _dtype(x: Any, dtype: Any, what: str) -> None:
+    """A tensor (not a raw pointer) must have torch.<dtype>."""
+    if dtype is not None and hasattr(x, "dtype") and hasattr(x, "data_ptr"):
+        import torch
+        if x.dtype != getattr(torch, dtype):
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{what} must be torch.{dtype}, got {x.dtype}")
+
+
+def _rows(x: Any, count: int, what: str) -> None:
+    """A tensor (not a raw pointer) must hold at least `count` rows."""
+    if hasattr(x, "shape") and int(x.shape[0]) < count:
+        raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{what} has {int(x.shape[0])} rows, {count} are read")
+
+
+def _result_size(x: Any, q: int, K: int, what: str) -> None:
+    """A result tensor must hold q*K elements of 4 bytes."""
+    if hasattr(x, "numel") and x.numel() * x.element_size() < 4 * q * K:
+        raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{what} holds fewer than count*K 4-byte elements")
 
 
 def _shape2(x: Any, what: str):
@@ -121,14 +147,22 @@ class _Structure:
     def _query(self, query: Query, K: int, opts: QueryOpts) -> None:
         q = query.n()
         if hasattr(query.points, "shape") and q:
-            _, d = _shape2(query.points, "query points")
+            rows, d = _shape2(query.points, "query points")
             if d != self.dims:
                 raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"query dims {d} != structure dims {self.dims}")
-        for name, t in (("r_indexes", query.r_indexes), ("r_distances", query.r_distances)):
-            if hasattr(t, "numel") and t.numel() < q * K:
-                raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"{name} holds fewer than count*K elements")
-        _lib.check(_lib.load().knn_query(self._h, _ptr(query.points), q, float(query.max_distance2),
-                                         _ptr(query.r_indexes), _ptr(query.r_distances), int(K),
+            _rows(query.points, q, "query points")
+        if q < 0 or K < 1:
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, "count must be >= 0 and K >= 1")
+        _dtype(query.points, "float32", "query points")
+        _dtype(query.r_indexes, "int32", "r_indexes")
+        _dtype(query.r_distances, "float32", "r_distances")
+        _result_size(query.r_indexes, q, K, "r_indexes")
+        _result_size(query.r_distances, q, K, "r_distances")
+        dv = getattr(self, "device", None)
+        _lib.check(_lib.load().knn_query(self._h, _ptr(query.points, "float32", "query points", dv), q,
+                                         float(query.max_distance2),
+                                         _ptr(query.r_indexes, "int32", "r_indexes", dv),
+                                         _ptr(query.r_distances, "float32", "r_distances", dv), int(K),
                                          ctypes.byref(opts)))
 
     def last_query_stats(self) -> dict:
@@ -136,7 +170,7 @@ class _Structure:
         _lib.check(_lib.load().knn_last_query_stats(self._h, buf))
         return {"rerun_queries": buf[0], "bf16_survivor_fraction": buf[1], "radius_ratio": buf[2],
                 "query_ms": buf[3], "kernel_ms": buf[4], "node_bytes": buf[5], "leaf_slots": buf[6],
-                "order_ms": buf[7]}
+                "order_ms": buf[7], "bf16_seeded": buf[5]}
 
     def close(self) -> None:
         if self._h and self._h.value:
@@ -181,8 +215,9 @@ class BruteForceGpu(_Structure):
         super().__init__()
         n, d = _shape2(points, "points")
         n = n if size is None else int(size)
-        self.dims, self.size = d, n
-        _lib.check(_lib.load().knn_bf_create(_ptr(points), n, d, ctypes.byref(self._h)))
+        _rows(points, n, "points")
+        self.dims, self.size, self.device = d, n, getattr(points, "device", None)
+        _lib.check(_lib.load().knn_bf_create(_ptr(points, "float32", "points"), n, d, ctypes.byref(self._h)))
 
     def batchKnn(self, query: Query, K: int, BLOCK_SIZE: int = 0, USE_HEAP_TRACKING: bool = False,
                  USE_TRACKING_SHMEM: bool = False, form: int = FORM_PLAIN, stream=None, tile: int = 0) -> None:
@@ -257,7 +292,10 @@ class _Tree(_Structure):
         """Train the learned-radius predictor (KNN_FORM_LEARNED) on sample queries."""
         n, d = _shape2(sample, "sample")
         stats = (ctypes.c_float * 2)()
-        _lib.check(_lib.load().knn_learned_train(self._h, _ptr(sample), n, int(K), int(epochs), float(lr), stats))
+        if d != self.dims:
+            raise KnnError(_lib.KNN_ERR_INVALID_ARG, f"sample dims {d} != structure dims {self.dims}")
+        _lib.check(_lib.load().knn_learned_train(self._h, _ptr(sample, "float32", "sample", getattr(self, "device", None)),
+                                                 n, int(K), int(epochs), float(lr), stats))
         return {"loss": stats[0], "train_ms": stats[1]}
 
 
@@ -271,8 +309,10 @@ class KdTreeGpu(_Tree):
         n, d = _shape2(points, "points")
         n = n if size is None else int(size)
         sd = d if split_dimensions is None else int(split_dimensions)
-        self.dims, self.size = d, n
-        _lib.check(_lib.load().knn_kd_build(_ptr(points), n, d, sd, int(depth_skips), int(block_size),
+        _rows(points, n, "points")
+        self.dims, self.size, self.device = d, n, getattr(points, "device", None)
+        _lib.check(_lib.load().knn_kd_build(_ptr(points, "float32", "points"), n, d, sd, int(depth_skips),
+                                            int(block_size),
                                             ctypes.byref(self._h)))
 
 
@@ -284,8 +324,10 @@ class BallTreeGpu(_Tree):
         super().__init__()
         n, d = _shape2(points, "points")
         n = n if size is None else int(size)
-        self.dims, self.size = d, n
-        _lib.check(_lib.load().knn_ball_build(_ptr(points), n, d, int(depth_skips), int(block_size),
+        _rows(points, n, "points")
+        self.dims, self.size, self.device = d, n, getattr(points, "device", None)
+        _lib.check(_lib.load().knn_ball_build(_ptr(points, "float32", "points"), n, d, int(depth_skips),
+                                              int(block_size),
                                               ctypes.byref(self._h)))
 
 
@@ -366,8 +408,16 @@ class Batch:
             a = arr[j]
             a.kind, a.dims, a.n, a.nq, a.k = int(it.kind), d, n, nq, int(it.k)
             a.split_dims, a.depth_skips, a.max_d2 = int(it.split_dimensions), int(it.depth_skips), float(it.max_distance2)
-            a.d_pts, a.d_q = _ptr(it.points), _ptr(it.queries)
-            a.d_idx, a.d_dist = _ptr(it.r_indexes), _ptr(it.r_distances)
+            dv = getattr(it.points, "device", None)
+            for t, dt, nm in ((it.points, "float32", "points"), (it.queries, "float32", "queries"),
+                              (it.r_indexes, "int32", "r_indexes"), (it.r_distances, "float32", "r_distances")):
+                _dtype(t, dt, f"batch item {j}: {nm}")
+            _result_size(it.r_indexes, nq, int(it.k), f"batch item {j}: r_indexes")
+            _result_size(it.r_distances, nq, int(it.k), f"batch item {j}: r_distances")
+            a.d_pts = _ptr(it.points, "float32", f"batch item {j}: points", dv)
+            a.d_q = _ptr(it.queries, "float32", f"batch item {j}: queries", dv)
+            a.d_idx = _ptr(it.r_indexes, "int32", f"batch item {j}: r_indexes", dv)
+            a.d_dist = _ptr(it.r_distances, "float32", f"batch item {j}: r_distances", dv)
         o = _lib.BatchOpts(int(build_streams), int(blocks_per_cu), int(tile), int(leaf_span))
         _lib.check(_lib.load().knn_batch_create(arr, len(self.items), ctypes.byref(o), ctypes.byref(self._h)))
 

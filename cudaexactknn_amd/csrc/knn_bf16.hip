@@ -283,6 +283,7 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     constexpr uint32_t kSeedStride = 16;
     const bool seeded = tiles >= kSeedStride * 64 && D16 >= 2;
     EventTimer timer(a.timed, a.stream);
+    h.stats[5] = seeded ? 1.0f : 0.0f;
     if (seeded) {
         S.seed = seed;
         launch_fill_u32(seed, a.nq, 0x7F800000u, a.stream);  // +inf

@@ -184,7 +184,9 @@ knn_status knn_learned_train(void* handle, const float* d_sample, uint32_t n_sam
  * [1] BF16 survivor fraction (bf16 form), [2] mean predicted / exact radius
  * ratio (learned form), [3] ms of the whole query, [4] ms of the main
  * kernel (traversal or scan), [5] node bytes read and [6] leaf slots read
- * (with KNN_FLAG_COUNT_VISITS), [7] ms spent ordering queries.
+ * (trees, with KNN_FLAG_COUNT_VISITS) or, for the bf16 form, [5] = 1 when
+ * the seeding pass ran (D > 16 and at least 32,768 points), [7] ms spent
+ * ordering queries.
  * All timings are hipEvent-measured on the query's stream. */
 knn_status knn_last_query_stats(const void* handle, float* out8);
 

@@ -207,8 +207,9 @@ typedef struct {
 
 static void tree_sizing(oracle_tree_info* t, uint32_t n, uint32_t D, uint32_t SD, uint32_t skips, int ball) {
     uint32_t L = log2_ceil(n);
+    if (g_mutation == 9 && n > 1 && (n & (n - 1)) == 0) L += 1; /* planted: log2Ceil rounds exact powers up */
     uint32_t s = skips;
-    if (ball && s < 1) s = 1;
+    if (ball && s < 1 && g_mutation != 8) s = 1; /* planted (8): ball clamp to >= 1 dropped */
     if (s > L - 1) s = L - 1;
     t->n = n; t->dims = D; t->split_dims = SD; t->skips = s;
     t->node_depth = L - s;

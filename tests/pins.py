@@ -258,3 +258,57 @@ def check_ball_structure(pts, info, nodes, dims, leaf):
             if r < rmax:
                 return False, f"node {node}: radius {r} below the exact radius {rmax}"
     return True, ""
+
+
+# Tree sizing written out by hand from the reference's constructors
+# (src/KdTree.cuh:56-62, src/BallTree.cuh:123-128): with L = log2Ceil(n)
+# (src/KnnTrees.cuh:84-93: 1 for n <= 2, else ceil(log2 n)), kd trees clamp
+# depthSkips to L - 1 and ball trees to [1, L - 1] (the upper clamp wins when
+# L = 1).  Each row is (n, requested skips) -> (kd tuple, ball tuple) with
+# tuple = (effective skips, node_depth, max_children, node_size, index_size).
+# The values are literals, not evaluations of the formula, so a sizing rule
+# shared by the oracle and the library (a dropped clamp, an off-by-one
+# log2Ceil) disagrees with at least one row.
+SIZING_TABLE = {
+    (1, 0): ((0, 1, 1, 1, 2), (0, 1, 1, 1, 2)),
+    (1, 1): ((0, 1, 1, 1, 2), (0, 1, 1, 1, 2)),
+    (1, 3): ((0, 1, 1, 1, 2), (0, 1, 1, 1, 2)),
+    (1, 40): ((0, 1, 1, 1, 2), (0, 1, 1, 1, 2)),
+    (2, 0): ((0, 1, 1, 1, 2), (0, 1, 1, 1, 2)),
+    (2, 1): ((0, 1, 1, 1, 2), (0, 1, 1, 1, 2)),
+    (2, 3): ((0, 1, 1, 1, 2), (0, 1, 1, 1, 2)),
+    (2, 40): ((0, 1, 1, 1, 2), (0, 1, 1, 1, 2)),
+    (3, 0): ((0, 2, 1, 3, 4), (1, 1, 2, 1, 4)),
+    (3, 1): ((1, 1, 2, 1, 4), (1, 1, 2, 1, 4)),
+    (3, 3): ((1, 1, 2, 1, 4), (1, 1, 2, 1, 4)),
+    (3, 40): ((1, 1, 2, 1, 4), (1, 1, 2, 1, 4)),
+    (4, 0): ((0, 2, 1, 3, 4), (1, 1, 2, 1, 4)),
+    (4, 1): ((1, 1, 2, 1, 4), (1, 1, 2, 1, 4)),
+    (4, 3): ((1, 1, 2, 1, 4), (1, 1, 2, 1, 4)),
+    (4, 40): ((1, 1, 2, 1, 4), (1, 1, 2, 1, 4)),
+    (5, 0): ((0, 3, 1, 7, 8), (1, 2, 2, 3, 8)),
+    (5, 1): ((1, 2, 2, 3, 8), (1, 2, 2, 3, 8)),
+    (5, 3): ((2, 1, 4, 1, 8), (2, 1, 4, 1, 8)),
+    (5, 40): ((2, 1, 4, 1, 8), (2, 1, 4, 1, 8)),
+    (65536, 0): ((0, 16, 1, 65535, 65536), (1, 15, 2, 32767, 65536)),
+    (65536, 1): ((1, 15, 2, 32767, 65536), (1, 15, 2, 32767, 65536)),
+    (65536, 3): ((3, 13, 8, 8191, 65536), (3, 13, 8, 8191, 65536)),
+    (65536, 40): ((15, 1, 32768, 1, 65536), (15, 1, 32768, 1, 65536)),
+    (65537, 0): ((0, 17, 1, 131071, 131072), (1, 16, 2, 65535, 131072)),
+    (65537, 1): ((1, 16, 2, 65535, 131072), (1, 16, 2, 65535, 131072)),
+    (65537, 3): ((3, 14, 8, 16383, 131072), (3, 14, 8, 16383, 131072)),
+    (65537, 40): ((16, 1, 65536, 1, 131072), (16, 1, 65536, 1, 131072)),
+    # the real-size 3-D workload: kd with depthSkips 0, ball with the default 1
+    (16777216, 0): ((0, 24, 1, 16777215, 16777216), (1, 23, 2, 8388607, 16777216)),
+    (16777216, 1): ((1, 23, 2, 8388607, 16777216), (1, 23, 2, 8388607, 16777216)),
+}
+SIZING_FIELDS = ("skips", "node_depth", "max_children", "node_size", "index_size")
+
+
+def check_sizing(info: dict, n: int, skips: int, ball: bool):
+    """Compares a tree's sizing (oracle names: skips; library: depth_skips)
+    with the hand-written row for (n, skips)."""
+    want = SIZING_TABLE[(n, skips)][1 if ball else 0]
+    got = (info.get("skips", info.get("depth_skips")), info["node_depth"], info["max_children"],
+           info["node_size"], info["index_size"])
+    return got == want, f"n={n} skips={skips} ball={ball}: got {got}, want {want}"

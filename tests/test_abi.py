@@ -4,10 +4,12 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import cudaexactknn_amd as knn
 from cudaexactknn_amd import _lib
+from tests import pins
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -49,14 +51,77 @@ def test_null_handle_and_bad_args_fail_cleanly():
         _lib.check(L.knn_query(None, None, 1, 1.0, None, None, 4, None))
 
 
-def test_tree_sizing_matches_reference_formula(oracle):
-    """tree sizing (src/KdTree.cuh:56-62, src/BallTree.cuh:123-128) as exported
-    by the oracle for edge sizes; the library uses the same rule."""
-    for n in (1, 2, 3, 4, 5, 1000, 1 << 16, (1 << 16) + 1):
-        for skips in (0, 1, 3, 40):
-            L = oracle.log2_ceil(n)
-            s_kd = min(skips, L - 1)
-            s_ball = min(max(skips, 1), L - 1)
-            for s in (s_kd, s_ball):
-                assert (1 << (L - s)) - 1 >= 1
-                assert (1 << L) >= n
+def _sizing_failures(oracle):
+    g = np.random.Generator(np.random.Philox(11))
+    fails = []
+    for (n, skips) in pins.SIZING_TABLE:
+        if n > 65537:
+            continue
+        pts = g.random((n, 3), dtype=np.float32)
+        for ball in (False, True):
+            t = oracle.BallTree(pts, skips) if ball else oracle.KdTree(pts, skips)
+            ok, msg = pins.check_sizing(t.info(), n, skips, ball)
+            if not ok:
+                fails.append(msg)
+    return fails
+
+
+def test_tree_sizing_matches_hand_written_table(oracle):
+    """Oracle tree sizing (src/KdTree.cuh:56-62, src/BallTree.cuh:123-128)
+    against literal values for n in {1..5, 2^16, 2^16+1} and depthSkips in
+    {0, 1, 3, 40}."""
+    assert _sizing_failures(oracle) == []
+
+
+@pytest.mark.parametrize("mutation", [8, 9])
+def test_planted_sizing_mistakes_are_caught(oracle, mutation):
+    """A dropped ball clamp (8) or a log2Ceil that rounds exact powers of two
+    up (9), planted in the oracle, disagrees with the table."""
+    oracle.set_mutation(mutation)
+    try:
+        fails = _sizing_failures(oracle)
+    finally:
+        oracle.set_mutation(0)
+    assert fails, f"mutation {mutation} passed the sizing table"
+
+
+def _fake_structure(dims=3):
+    """A structure object with no handle: the Python checks run before any
+    call into the library, so they can be exercised without a GPU."""
+    s = knn.BruteForceGpu.__new__(knn.BruteForceGpu)
+    knn._Structure.__init__(s)
+    s.dims, s.size, s.device = dims, 10, None
+    return s
+
+
+def test_python_layer_rejects_wrong_dtypes_and_sizes():
+    import torch
+    with pytest.raises(knn.KnnError, match="float32"):
+        knn.BruteForceGpu(torch.zeros((10, 3), dtype=torch.float64))
+    with pytest.raises(knn.KnnError, match="float32"):
+        knn.KdTreeGpu(torch.zeros((10, 3), dtype=torch.float16))
+    with pytest.raises(knn.KnnError, match="rows"):
+        knn.BallTreeGpu(torch.zeros((10, 3), dtype=torch.float32), size=11)
+    s = _fake_structure()
+    q = torch.zeros((4, 3), dtype=torch.float32)
+    ri = torch.zeros((4, 2), dtype=torch.int32)
+    rd = torch.zeros((4, 2), dtype=torch.float32)
+    cases = [
+        (knn.Query(q.double(), 4, 1.0, ri, rd), "float32"),                    # float64 queries
+        (knn.Query(q, 4, 1.0, ri.long(), rd), "int32"),                        # int64 indices
+        (knn.Query(q, 4, 1.0, ri, rd.half()), "float32"),                      # half-size distances
+        (knn.Query(q, 4, 1.0, ri, rd.to(torch.bfloat16)), "float32"),
+        (knn.Query(q, 4, 1.0, ri, rd[:3]), "count\\*K"),
+        (knn.Query(q, 4, 1.0, ri[:3], rd), "count\\*K"),                       # too few result rows
+        (knn.Query(q, 5, 1.0, ri, rd), "rows"),                                # count beyond the query rows
+        (knn.Query(torch.zeros((4, 2)), 4, 1.0, ri, rd), "dims"),              # wrong D
+    ]
+    for query, msg in cases:
+        with pytest.raises(knn.KnnError, match=msg):
+            s._query(query, 2, knn.QueryOpts())
+    with pytest.raises(knn.KnnError, match="GPU"):                             # right types, host memory
+        s._query(knn.Query(q, 4, 1.0, ri, rd), 2, knn.QueryOpts())
+    it = knn.BatchItem(knn.KIND_KD, torch.zeros((8, 3)), torch.zeros((2, 3)), 4, torch.zeros((2, 4), dtype=torch.int64),
+                       torch.zeros((2, 4)))
+    with pytest.raises(knn.KnnError, match="int32"):
+        knn.Batch([it])
