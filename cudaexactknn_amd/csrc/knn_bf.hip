@@ -238,7 +238,7 @@ int pick_buf_kind(bool heap, bool lds, uint32_t K) {
 
 static bool buf_in_lds(int kind) { return kind == BUF_LDS_ARRAY || kind == BUF_LDS_HEAP; }
 
-void query_bf(Handle& h, const QueryArgs& a) {
+void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial_ws, size_t partial_ws_words) {
     const uint32_t D = h.info.dims;
     const int kind = pick_buf_kind(a.heap, a.track_lds, a.K);
     // BLOCK_SIZE only changes the schedule, never the result: kernels built
@@ -283,10 +283,14 @@ void query_bf(Handle& h, const QueryArgs& a) {
         if (S > 64) S = 64;
         if (S < 1) S = 1;
         while (S > 1 && (size_t)a.nq * S * a.K * 8 > ((size_t)1 << 30)) S /= 2;
+        // A caller-provided buffer bounds the split (fewer chunks, same rows).
+        while (S > 1 && partial_ws && (size_t)a.nq * S * a.K > partial_ws_words) S /= 2;
     }
     const uint32_t chunk = S > 1 ? (n + S - 1) / S : (n ? n : 1);
     uint64_t* partial = nullptr;
-    if (S > 1) {
+    if (S > 1 && partial_ws) {
+        partial = partial_ws;
+    } else if (S > 1) {
         const size_t words = (size_t)a.nq * S * a.K;
         h.q_scratch.ensure(words * 8);
         partial = reinterpret_cast<uint64_t*>(h.q_scratch.get());

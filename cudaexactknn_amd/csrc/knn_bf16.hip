@@ -11,7 +11,6 @@
 
 namespace knn {
 
-void query_bf(Handle& h, const QueryArgs& a);
 int pick_buf_kind(bool heap, bool lds, uint32_t K);
 
 // Points -> tiled hi/lo bf16 operands: tile t (32 points), k-step kk (16 dims),

@@ -82,7 +82,7 @@ static uint32_t default_span(const Handle& h, uint32_t K, bool per_lane) {
     return s;
 }
 
-static void check_query(const Handle& h, uint32_t k, const knn_query_opts& o) {
+void check_query(const Handle& h, uint32_t k, const knn_query_opts& o) {
     require(k >= 1, "k must be >= 1");
     require(k <= 4096, "k above 4096 is not supported", KNN_ERR_UNSUPPORTED);
     require(o.form <= KNN_FORM_LEARNED, "unknown form");
@@ -91,8 +91,8 @@ static void check_query(const Handle& h, uint32_t k, const knn_query_opts& o) {
 }
 
 // Options -> resolved query parameters (tuned defaults for zero fields).
-static QueryArgs resolve_args(const Handle& h, const float* d_q, uint32_t q, float max_d2, int32_t* d_idx,
-                              float* d_dist, uint32_t k, const knn_query_opts& o) {
+QueryArgs resolve_args(const Handle& h, const float* d_q, uint32_t q, float max_d2, int32_t* d_idx, float* d_dist,
+                       uint32_t k, const knn_query_opts& o) {
     QueryArgs a{};
     a.q = d_q;
     a.nq = q;

@@ -182,6 +182,12 @@ struct QueryArgs {
     bool timed;  // synchronous call: time kernels with events into Handle::stats
 };
 
+// knn_capi.hip: argument checks and option resolution shared by the query
+// entry points.
+void check_query(const Handle& h, uint32_t k, const knn_query_opts& o);
+QueryArgs resolve_args(const Handle& h, const float* d_q, uint32_t q, float max_d2, int32_t* d_idx, float* d_dist,
+                       uint32_t k, const knn_query_opts& o);
+
 // Event pair timing a region on a stream when `on`.
 struct EventTimer {
     hipEvent_t a{}, b{};
@@ -225,12 +231,22 @@ void delete_level_bufs(LevelBufs* b);
 void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hipStream_t st);
 
 // Queries
-void query_bf(Handle& h, const QueryArgs& a);               // knn_bf.hip
+// knn_bf.hip.  Partial top-K keys of split scans go to `partial` (at most
+// `partial_words` u64) when it is given, else to the handle's scratch.
+void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial = nullptr, size_t partial_words = 0);
 void query_bf_bf16(Handle& h, const QueryArgs& a);          // knn_bf16.hip
 void prepare_bf16(Handle& h, hipStream_t st);               // knn_bf16.hip
 void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
                 uint32_t* d_short_flags);                    // knn_tree_query.hip
-void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm);  // knn_tree_query.hip
+void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm);  // knn_tree_host.hip
+// Query ordering on caller-provided scratch: keys, values, their sort
+// alternates (nq words each) and the radix sort's temporary words.
+struct OrderScratch {
+    uint32_t *keys, *vals, *keys_alt, *vals_alt, *temp;
+};
+size_t order_scratch_words(uint32_t nq);                      // u32 words for nq queries
+OrderScratch order_scratch_at(uint32_t* base, uint32_t nq);  // carve one block of that size
+void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& s, uint32_t** d_perm);
 
 // Learned radius (knn_learned.hip)
 void learned_train(Handle& h, const float* d_sample, uint32_t n_sample, uint32_t k, uint32_t epochs, float lr,

@@ -79,14 +79,26 @@ __global__ void ball_order_kernel(const float* __restrict__ qs, uint32_t nq, uin
     vals[i] = i;
 }
 
+size_t order_scratch_words(uint32_t nq) { return 4 * (size_t)nq + radix_temp_words(nq); }
+
 void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
-    const uint32_t n = a.nq, D = h.info.dims;
-    EventTimer timer(a.timed, a.stream);
+    const uint32_t n = a.nq;
     h.q_keys.ensure(n);
     h.q_vals.ensure(n);
     h.q_keys_alt.ensure(n);
     h.q_vals_alt.ensure(n);
     h.q_temp.ensure(radix_temp_words(n));
+    OrderScratch s{h.q_keys.get(), h.q_vals.get(), h.q_keys_alt.get(), h.q_vals_alt.get(), h.q_temp.get()};
+    order_queries_into(h, a, s, d_perm);
+}
+
+OrderScratch order_scratch_at(uint32_t* base, uint32_t nq) {
+    return OrderScratch{base, base + nq, base + 2 * (size_t)nq, base + 3 * (size_t)nq, base + 4 * (size_t)nq};
+}
+
+void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& S, uint32_t** d_perm) {
+    const uint32_t n = a.nq, D = h.info.dims;
+    EventTimer timer(a.timed, a.stream);
     int bits;
     // Order by the cell of about 4 queries: points and queries that share a
     // distribution put about equal query counts in every cell of a level, so
@@ -98,25 +110,24 @@ void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
     if (tree_levels > 24) tree_levels = 24;
     if (a.order_morton || h.info.n == 0) {
         hipLaunchKernelGGL(morton_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D, h.bbox.get(),
-                           h.q_keys.get(), h.q_vals.get());
+                           S.keys, S.vals);
         const uint32_t Dm = D < 30 ? D : 30;
         uint32_t bpd = 30 / Dm;
         if (bpd > 10) bpd = 10;
         bits = (int)(bpd * Dm);
     } else if (h.kind == KNN_KIND_KD) {
         hipLaunchKernelGGL(kd_order_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D, h.nodes.get(),
-                           tree_levels, h.info.split_dims, h.q_keys.get(), h.q_vals.get());
+                           tree_levels, h.info.split_dims, S.keys, S.vals);
         bits = (int)tree_levels;
     } else {
         hipLaunchKernelGGL(ball_order_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D,
-                           h.nodes.get(), tree_levels, h.q_keys.get(), h.q_vals.get());
+                           h.nodes.get(), tree_levels, S.keys, S.vals);
         bits = (int)tree_levels;
     }
     const int end = (bits + 7) / 8 * 8;
-    bool alt = radix_sort_pairs(h.q_keys.get(), h.q_vals.get(), h.q_keys_alt.get(), h.q_vals_alt.get(), n, 0, end,
-                                h.q_temp.get(), a.stream);
+    bool alt = radix_sort_pairs(S.keys, S.vals, S.keys_alt, S.vals_alt, n, 0, end, S.temp, a.stream);
     KNN_CHECK_LAUNCH();
-    *d_perm = alt ? h.q_vals_alt.get() : h.q_vals.get();
+    *d_perm = alt ? S.vals_alt : S.vals;
     h.stats[7] = timer.stop();
 }
 

@@ -120,6 +120,24 @@ knn_status knn_ball_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_
 knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max_d2,
                      int32_t* d_idx, float* d_dist, uint32_t k, const knn_query_opts* opts);
 
+/* knn_query without allocation (no counterpart in the reference: its
+ * batchKnn allocates nothing, but it also sorts no queries and splits no
+ * scans).  For callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, semantics and results as knn_query; the plain form only
+ * (opts->form must be KNN_FORM_PLAIN, KNN_FLAG_COUNT_VISITS is rejected).
+ * Scratch comes from a static 96 MiB device array that each GPU holds for
+ * the life of the process: tree queries need 16.25 bytes per query for
+ * their ordering, so a batch above about 6.19M queries is rejected with
+ * KNN_ERR_INVALID_ARG (split it, or pass KNN_FLAG_NO_QUERY_SORT); brute
+ * force narrows its split scan to fit and never fails for size.  Calls
+ * no hipMalloc/hipFree.  Synchronous when opts->stream is NULL, else
+ * enqueued on that stream.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two threads, or two streams in flight)
+ * race and must be serialised by the caller. */
+knn_status knn_query_noalloc(const void* handle, const float* d_q, uint32_t q, float max_d2,
+                             int32_t* d_idx, float* d_dist, uint32_t k, const knn_query_opts* opts);
+
 /* Host-resident queries (no reference counterpart; the reference's callers
  * copy queries in and results out around batchKnn, with pinnedMalloc,
  * src/KnnTrees.cuh:233-243, as the only host-memory hook).  h_q (q x dims),
