@@ -27,9 +27,12 @@ depth-first search reads, counted by the library's visit counters in an
 untimed per-lane run; divided by the wave-cooperative kernel's
 hipEvent-measured average duration.  Peak: 8 TB/s HBM.
 
-CPU baseline: the CPU oracle's kd tree (a port of the reference's KdTree,
-src/KdTree.cuh) on this host, on a bounded query sample, with its results
-checked against the GPU's rows for the same queries.
+CPU baseline: the CPU oracle (a port of the reference's CPU classes,
+src/KdTree.cuh, src/BallTree.cuh, src/BruteForce.cuh) on this host, on
+bounded query samples, on one core and on all cores, with its results
+checked against the GPU's rows for the same queries: the kd tree on the
+headline workload is `value`; the ball tree and both brute-force cases are
+reported beside it.
 """
 from __future__ import annotations
 
@@ -65,38 +68,83 @@ def traffic_from_profiles(kernel: str):
         return None
 
 
-def cpu_baseline(pts, qs, gpu_idx, gpu_dist, sample: int):
-    """Time the CPU oracle kd tree on a bounded sample (test infrastructure)."""
-    from oracle import oracle as O
-    O.build()
-    t0 = time.perf_counter()
-    tree = O.KdTree(pts, 0)
-    build_s = time.perf_counter() - t0
-    q1 = qs[: sample // 16]
-    t0 = time.perf_counter()
-    i1, d1 = tree.knn(q1, K, threads=1)
-    one_core = len(q1) / (time.perf_counter() - t0)
-    qa = qs[:sample]
-    ncores = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    ia, da = tree.knn(qa, K, threads=ncores)
-    all_core = len(qa) / (time.perf_counter() - t0)
-    exact = bool(np.array_equal(ia, gpu_idx[:sample]) and
-                 np.array_equal(da.view(np.uint32), gpu_dist[:sample].view(np.uint32)))
-    model = ""
+def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
+    return ""
+
+
+def _rows_equal(i1, d1, i2, d2) -> bool:
+    return bool(np.array_equal(i1, i2) and np.array_equal(d1.view(np.uint32), d2.view(np.uint32)))
+
+
+def cpu_baseline(pts, qs, gpu_idx, gpu_dist, sample: int, gpu_rows: dict):
+    """The CPU oracle (a port of the reference's CPU classes; test
+    infrastructure, timed here as the reported baseline) on this host, on
+    bounded samples, each checked against the GPU's rows for the same
+    queries: the kd tree on the headline workload (the `value`), the ball
+    tree on the same points, brute force on the 65,536 x 16,384 uniform case
+    and on the 1,048,576-point D = 128 case.  Brute force cost is linear in
+    the query count, so its per-query rate on a sample is the full set's."""
+    from oracle import oracle as O
+    O.build()
+    ncores = os.cpu_count() or 1
+
+    def rate(fn, q, threads):
+        t0 = time.perf_counter()
+        out = fn(q, threads)
+        return len(q) / (time.perf_counter() - t0), out
+
+    t0 = time.perf_counter()
+    tree = O.KdTree(pts, 0)
+    build_s = time.perf_counter() - t0
+    kd_fn = lambda q, t: tree.knn(q, K, threads=t)  # noqa: E731
+    one_core, _ = rate(kd_fn, qs[: sample // 16], 1)
+    all_core, (ia, da) = rate(kd_fn, qs[:sample], ncores)
+    exact = _rows_equal(ia, da, gpu_idx[:sample], gpu_dist[:sample])
+    del tree
+
+    others = {}
+    t0 = time.perf_counter()
+    ball = O.BallTree(pts, 1)
+    ball_build_s = time.perf_counter() - t0
+    ball_fn = lambda q, t: ball.knn(q, K, threads=t)  # noqa: E731
+    b1, _ = rate(ball_fn, qs[:4096], 1)
+    ba, (bi, bd) = rate(ball_fn, qs[:sample], ncores)
+    others["ball_tree_headline"] = {
+        "qps_1core": round(b1, 1), "qps_all_cores": round(ba, 1), "build_s": round(ball_build_s, 2),
+        "sample": f"BallTree (depthSkips 1) built on all 16,777,216 points; {sample} queries on {ncores} "
+                  "threads, 4,096 on 1", "sample_bit_exact_vs_gpu": _rows_equal(bi, bd, gpu_idx[:sample],
+                                                                                   gpu_dist[:sample])}
+    del ball
+
+    up, uq, ui, ud = gpu_rows["bf_uniform"]
+    bf_fn = lambda q, t: O.bf_knn(up, q, 8, heap=True, threads=t)  # noqa: E731
+    u1, _ = rate(bf_fn, uq[:1024], 1)
+    ua, (bi, bd) = rate(bf_fn, uq, ncores)
+    others["brute_force_uniform_65536x16384_k8_heap"] = {
+        "qps_1core": round(u1, 1), "qps_all_cores": round(ua, 1),
+        "sample": f"all 16,384 queries on {ncores} threads, 1,024 on 1", "bit_exact_vs_gpu": _rows_equal(bi, bd, ui, ud)}
+
+    if "bf_d128" in gpu_rows:
+        cp, cq, ci, cd = gpu_rows["bf_d128"]
+        c_fn = lambda q, t: O.bf_knn(cp, q, 32, threads=t)  # noqa: E731
+        c1, _ = rate(c_fn, cq[:32], 1)
+        ca, (bi, bd) = rate(c_fn, cq[:512], ncores)
+        others["brute_force_d128_1048576_points_k32"] = {
+            "qps_1core": round(c1, 2), "qps_all_cores": round(ca, 2), "extrapolated": True,
+            "sample": f"first 512 of 65,536 queries on {ncores} threads, 32 on 1 (cost is linear in queries)",
+            "sample_bit_exact_vs_gpu": _rows_equal(bi, bd, ci[:512], cd[:512])}
     return {
         "value": round(all_core, 1), "unit": "queries/s", "cores": ncores, "kind": "port",
         "sample": f"oracle KdTree (depthSkips 0) built on all 16,777,216 points in {build_s:.1f} s; "
-                  f"{len(qa)} queries timed on {ncores} threads, {len(q1)} on 1 thread",
-        "value_1core": round(one_core, 1), "build_s": round(build_s, 2), "cpu_model": model,
-        "sample_bit_exact_vs_gpu": exact,
+                  f"{sample} queries timed on {ncores} threads, {sample // 16} on 1 thread",
+        "value_1core": round(one_core, 1), "build_s": round(build_s, 2), "cpu_model": _cpu_model(),
+        "sample_bit_exact_vs_gpu": exact, "other_classes": others,
     }
 
 
@@ -144,6 +192,7 @@ def leg_bf16(dev):
     t_plain, s_plain, r_plain = _timed_query(bf, q, nq, K, dev, reps=1)
     t_bf16, s_bf16, r_bf16 = _timed_query(bf, q, nq, K, dev, reps=2, form=knn.FORM_BF16)
     bf.close()
+    rows = (pts, qs, r_plain[0].cpu().numpy(), r_plain[1].cpu().numpy())
     out = {"workload": "BruteForceGpu<128> batchKnn<K=32>, 1,048,576 x 65,536 scaled normals, seed 2",
            "plain_qps": round(nq / t_plain * 1e3, 1), "bf16_qps": round(nq / t_bf16 * 1e3, 1),
            "speedup": round(t_plain / t_bf16, 3), "identical": _same(r_plain, r_bf16),
@@ -152,7 +201,7 @@ def leg_bf16(dev):
            "plain_tops_3D_ops": round(3 * D * nq * (1 << 20) / (s_plain["kernel_ms"] * 1e-3) / 1e12, 2),
            "bf16_mfma_tflops_q_dot_p": round(2 * D * nq * (1 << 20) / (s_bf16["kernel_ms"] * 1e-3) / 1e12, 2)}
     log(f"bf16 D=128: plain {t_plain:.1f} ms, bf16 {t_bf16:.1f} ms ({out['speedup']}x), identical {out['identical']}")
-    return out
+    return out, rows
 
 
 def leg_learned(dev):
@@ -194,7 +243,11 @@ def leg_learned(dev):
 
 def leg_batch(dev):
     """256 uneven datasets (kd for D=3, bf16 brute force for D=16) in one batch
-    vs one plain call per dataset (per-lane kd, FP32 brute force)."""
+    against one call per dataset: with the same forms (kd wave-cooperative,
+    bf16 brute force), which isolates what batching gains, and with the plain
+    forms (per-lane kd, FP32 brute force).  Every timing is the best of three
+    warm passes that build and query each dataset; results are compared
+    after the timed passes."""
     import time as _t
 
     import torch
@@ -216,30 +269,46 @@ def leg_batch(dev):
         b.run()
         reps.append(b.stats())
     best = min(reps, key=lambda r: r["build_ms"] + r["query_ms"])
-    torch.cuda.synchronize()
-    t0 = _t.perf_counter()
-    identical = True
-    for st, it in zip(sets, items):
-        nq, k = len(st["queries"]), st["k"]
-        s = knn.KdTreeGpu(it.points, depth_skips=0) if it.kind == knn.KIND_KD else knn.BruteForceGpu(it.points)
-        r = _rows(nq, k, dev)
-        s.batchKnn(knn.Query(it.queries, nq, float("inf"), *r), k, per_lane=True) if it.kind == knn.KIND_KD \
-            else s.batchKnn(knn.Query(it.queries, nq, float("inf"), *r), k)
-        identical &= _same(r, (it.r_indexes, it.r_distances))
-        s.close()
-    torch.cuda.synchronize()
-    seq_ms = (_t.perf_counter() - t0) * 1e3
+    seq_rows = [_rows(len(st["queries"]), st["k"], dev) for st in sets]
+
+    def sequential(same_forms: bool) -> float:
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = _t.perf_counter()
+            for it, r in zip(items, seq_rows):
+                nq = it.queries.shape[0]
+                query = knn.Query(it.queries, nq, float("inf"), *r)
+                if it.kind == knn.KIND_KD:
+                    s = knn.KdTreeGpu(it.points, depth_skips=0)
+                    s.batchKnn(query, it.k, per_lane=not same_forms)
+                else:
+                    s = knn.BruteForceGpu(it.points)
+                    s.batchKnn(query, it.k, form=knn.FORM_BF16 if same_forms else knn.FORM_PLAIN)
+                s.close()
+            torch.cuda.synchronize()
+            times.append((_t.perf_counter() - t0) * 1e3)
+        return min(times)
+
+    same_ms = sequential(True)
+    identical_same = all(_same(r, (it.r_indexes, it.r_distances)) for it, r in zip(items, seq_rows))
+    plain_ms = sequential(False)
+    identical_plain = all(_same(r, (it.r_indexes, it.r_distances)) for it, r in zip(items, seq_rows))
     total = best["build_ms"] + best["query_ms"]
     out = {"workload": "256 datasets, N log-uniform in [1,024, 1,048,576] (40.1M points), Q = N/4 (10.0M "
                        "queries), U[0,1)^D, D in {3,16}, K in {4,8,16}, seed 4",
            "batch_build_ms": round(best["build_ms"], 2), "batch_query_ms": round(best["query_ms"], 2),
            "batch_qps_incl_build": round(nqt / total * 1e3, 1), "batch_qps_query_only": round(nqt / best["query_ms"] * 1e3, 1),
-           "sequential_plain_ms_incl_build": round(seq_ms, 1), "sequential_plain_qps": round(nqt / seq_ms * 1e3, 1),
-           "speedup_incl_build": round(seq_ms / total, 3), "identical": identical,
+           "sequential_same_forms_ms_incl_build": round(same_ms, 1),
+           "batching_speedup_same_forms": round(same_ms / total, 3),
+           "sequential_plain_ms_incl_build": round(plain_ms, 1), "sequential_plain_qps": round(nqt / plain_ms * 1e3, 1),
+           "speedup_incl_build": round(plain_ms / total, 3), "identical": bool(identical_same and identical_plain),
+           "timing": "best of 3 warm passes each; result checks outside the timed passes",
            "overflow_queries": best["overflow_queries"], "work_items": best["work_items"],
            "build_graph_nodes": best["build_graph_nodes"]}
-    log(f"batch: build {best['build_ms']:.1f} ms + query {best['query_ms']:.1f} ms vs sequential plain "
-        f"{seq_ms:.1f} ms ({out['speedup_incl_build']}x), identical {identical}")
+    log(f"batch: build {best['build_ms']:.1f} ms + query {best['query_ms']:.1f} ms vs sequential same forms "
+        f"{same_ms:.1f} ms ({out['batching_speedup_same_forms']}x) and plain {plain_ms:.1f} ms "
+        f"({out['speedup_incl_build']}x), identical {out['identical']}")
     b.close()
     return out
 
@@ -285,7 +354,7 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     if args.only:
-        leg = {"bf16": leg_bf16, "learned": leg_learned, "batch": leg_batch}[args.only]
+        leg = {"bf16": lambda d: leg_bf16(d)[0], "learned": leg_learned, "batch": leg_batch}[args.only]
         print(json.dumps({"leg": args.only, **leg(dev)}), flush=True)
         return 0
     log("generating the 3-D Gaussian-mixture workload")
@@ -396,18 +465,19 @@ def main() -> int:
     bf_ops = 3 * 3 * 65536 * 16384
     log(f"bf 65,536 x 16,384: kernel {bf_kernel_ms:.3f} ms, {bf_ops / bf_kernel_ms / 1e9:.1f} TFLOP/s (3D ops/pair)")
 
+    gpu_rows = {"bf_uniform": (up, uq, bri.cpu().numpy(), brd.cpu().numpy())}
     extra_legs = {}
     if not args.no_extra:
-        extra_legs["bf16_brute_force"] = leg_bf16(dev)
+        extra_legs["bf16_brute_force"], gpu_rows["bf_d128"] = leg_bf16(dev)
         extra_legs["learned_radius"] = leg_learned(dev)
         extra_legs["batched"] = leg_batch(dev)
 
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle kd tree on this host)")
-        cpu = cpu_baseline(pts, qs, kd_idx, kd_dist, args.cpu_sample)
+        cpu = cpu_baseline(pts, qs, kd_idx, kd_dist, args.cpu_sample, gpu_rows)
         log(f"CPU: {cpu['value']} q/s on {cpu['cores']} threads, {cpu['value_1core']} q/s on 1; "
-            f"sample bit-exact: {cpu['sample_bit_exact_vs_gpu']}")
+            f"sample bit-exact: {cpu['sample_bit_exact_vs_gpu']}; others {json.dumps(cpu['other_classes'])}")
 
     out = {
         "metric": "exact kNN queries/sec, chosen form vs plain HIP on same GPU (idx bit-exact)",

@@ -10,15 +10,19 @@ Shapes and template arguments follow the reference's drivers:
                 batchKnn<K,32,false,false>.
   construction  tests/ConstructionSpeedTests.cu: build times, D=3 with
                 100k..2.5M points, then D in {3,5,7,10,15,20} at 500k.
-  dim3          tests/Dim3SpeedTests.cu: D=3, K=10; 250k..10M queries on 1M
-                points, 250k..10M points with 1M queries.
+  dim3          tests/Dim3SpeedTests.cu: D=3, K=10, maxDistance2 5.9;
+                250k..10M queries on 1M points, 250k..10M points with 1M
+                queries.
   dimx          tests/DimXSpeedTests.cu: D=6..17, 500k points, 10k queries,
-                K=5.
+                K=5, maxDistance2 0.01 * D (DimXSpeedTests.cu:19).
 
 Inputs are uniform in [0, 1)^D from a seeded Philox stream (the reference
 drew them from std::random_device).  Timings are the library's hipEvent
 figures: query_ms per batchKnn (mean of 5 after one warm-up, as the
-reference averaged 5 of 6) and build_ms per construction.
+reference averaged 5 of 6) and build_ms per construction.  The reference
+also timed its CPU classes in the speedup sweep; their CPU counterparts are
+the header-only host classes of include/KnnTrees.hpp, outside this
+Python-driven GPU sweep.
 
     python -m cudaexactknn_amd.speed_suite speedup [--quick]
 """
@@ -38,7 +42,8 @@ def _points(n: int, d: int, seed: int) -> np.ndarray:
     return g.random((n, d), dtype=np.float32)
 
 
-def _time_query(struct, q, K, max_d2, reps, **opts) -> float:
+def _time_query(struct, q, K, max_d2, reps, **opts):
+    """Mean query ms over `reps` calls after one warm-up, and the result rows."""
     import torch
 
     import cudaexactknn_amd as knn
@@ -51,7 +56,7 @@ def _time_query(struct, q, K, max_d2, reps, **opts) -> float:
     for _ in range(reps):
         struct.batchKnn(Q, K, **opts)
         ms.append(struct.last_query_stats()["query_ms"])
-    return float(np.mean(ms))
+    return float(np.mean(ms)), (ri, rd)
 
 
 def _structures(p, d):
@@ -70,32 +75,40 @@ def _emit(rec: dict) -> None:
     print(json.dumps(rec), flush=True)
 
 
-def _query_sweep(suite, d, K, max_d2, n_pts_list, n_qry_list, vary, reps, seed, kinds=None):
+def _query_sweep(suite, d, K, max_d2, n_pts_list, n_qry_list, vary, reps, seed, kinds=None, on_result=None):
+    """Times each structure at each (points, queries) size.  `on_result`, if
+    given, receives (record, points, queries, (indices, distances)) after
+    each measurement, for result checks outside the timed calls."""
     import torch
     dev = torch.device("cuda:0")
     for n, nq in zip(n_pts_list, n_qry_list):
-        p = torch.from_numpy(_points(n, d, seed)).to(dev)
-        q = torch.from_numpy(_points(nq, d, seed + 1)).to(dev)
+        pn, qn = _points(n, d, seed), _points(nq, d, seed + 1)
+        p = torch.from_numpy(pn).to(dev)
+        q = torch.from_numpy(qn).to(dev)
         for name, make, opts in _structures(p, d):
             if kinds and name not in kinds:
                 continue
             s = make()
-            ms = _time_query(s, q, K, max_d2, reps, **opts)
-            _emit({"suite": suite, "structure": name, "dims": d, "points": n, "queries": nq, "k": K,
-                   "vary": vary, "query_ms": round(ms, 4), "queries_per_s": round(nq / ms * 1e3, 1)})
+            ms, rows = _time_query(s, q, K, max_d2, reps, **opts)
+            rec = {"suite": suite, "structure": name, "dims": d, "points": n, "queries": nq, "k": K,
+                   "max_distance2": max_d2, "vary": vary, "query_ms": round(ms, 4),
+                   "queries_per_s": round(nq / ms * 1e3, 1)}
+            _emit(rec)
+            if on_result:
+                on_result(rec, pn, qn, rows)
             s.close()
 
 
-def run_speedup(quick: bool, reps: int) -> None:
+def run_speedup(quick: bool, reps: int, on_result=None) -> None:
     q_counts = [1000, 5000, 10000, 25000, 50000, 100000]
     p_counts = [10000, 50000, 100000, 250000, 500000, 1000000]
     if quick:
         q_counts, p_counts = q_counts[:2], p_counts[:2]
-    _query_sweep("speedup", 5, 10, 5.9, [100000] * len(q_counts), q_counts, "queries", reps, 10)
-    _query_sweep("speedup", 5, 10, 5.9, p_counts, [10000] * len(p_counts), "points", reps, 20)
+    _query_sweep("speedup", 5, 10, 5.9, [100000] * len(q_counts), q_counts, "queries", reps, 10, None, on_result)
+    _query_sweep("speedup", 5, 10, 5.9, p_counts, [10000] * len(p_counts), "points", reps, 20, None, on_result)
 
 
-def run_construction(quick: bool, reps: int) -> None:
+def run_construction(quick: bool, reps: int, on_result=None) -> None:
     import torch
 
     import cudaexactknn_amd as knn
@@ -118,21 +131,22 @@ def run_construction(quick: bool, reps: int) -> None:
                    "build_ms": round(float(np.mean(ms)), 4)})
 
 
-def run_dim3(quick: bool, reps: int) -> None:
+def run_dim3(quick: bool, reps: int, on_result=None) -> None:
     counts = [250000, 500000, 1000000, 2500000, 5000000, 10000000]
     if quick:
         counts = counts[:2]
     kinds = ("KdTreeGpu", "BallTreeGpu")
-    _query_sweep("dim3", 3, 10, INF, [1000000] * len(counts), counts, "queries", reps, 40, kinds)
-    _query_sweep("dim3", 3, 10, INF, counts, [1000000] * len(counts), "points", reps, 50, kinds)
+    _query_sweep("dim3", 3, 10, 5.9, [1000000] * len(counts), counts, "queries", reps, 40, kinds, on_result)
+    _query_sweep("dim3", 3, 10, 5.9, counts, [1000000] * len(counts), "points", reps, 50, kinds, on_result)
 
 
-def run_dimx(quick: bool, reps: int) -> None:
+def run_dimx(quick: bool, reps: int, on_result=None) -> None:
     dims = list(range(6, 18))
     if quick:
         dims = dims[:2]
     for d in dims:
-        _query_sweep("dimx", d, 5, INF, [500000], [10000], "dims", reps, 60 + d)
+        max_d2 = float(np.float32(0.01 * d))
+        _query_sweep("dimx", d, 5, max_d2, [500000], [10000], "dims", reps, 60 + d, None, on_result)
 
 
 SUITES = {"speedup": run_speedup, "construction": run_construction, "dim3": run_dim3, "dimx": run_dimx}

@@ -1,7 +1,8 @@
 // C++ port of the reference's tests/FunctionalityTests.cu driving the drop-in
 // header include/KnnTrees.hpp: D=4, 100,000 points, 50,000 queries, K=10,
 // maxDistance2 = 5.9, every GPU variant the reference exercises
-// (FunctionalityTests.cu:70-106).  Inputs are seeded (the reference used
+// (FunctionalityTests.cu:70-106) and its host classes BruteForce, KdTree and
+// BallTree (FunctionalityTests.cu:41-67).  Inputs are seeded (the reference used
 // std::random_device).  Every variant must equal the array-mode brute force
 // bit for bit (heap rows reversed), which is stronger than the reference's
 // set comparison (tests/Tests.cuh:45-68).  Prints "All tests passed".
@@ -84,6 +85,33 @@ int main() {
         bfg.batchKnn<k, 16, false, false>(dQuery);
         fetch(correct);
 
+        // Host classes (FunctionalityTests.cu:41-67) on the same pinned host arrays.
+        {
+            Rows h;
+            h.idx.resize(qryCount * k);
+            h.dist.resize(qryCount * k);
+            KnnTrees::Query<dims, k> query = {queries, qryCount, 5.9f,
+                                              reinterpret_cast<KnnTrees::Array<int, k>*>(h.idx.data()),
+                                              reinterpret_cast<KnnTrees::Array<float, k>*>(h.dist.data())};
+            KnnTrees::BruteForce<dims> bf(points, pointCount);
+            bf.batchKnn<k, false>(query);
+            expect_same("BruteForce<4> array", h, correct, false);
+            bf.batchKnn<k, true>(query);
+            expect_same("BruteForce<4> heap", h, correct, true);
+            KnnTrees::KdTree<dims> kd(points, pointCount, 3);
+            kd.batchKnn<k, false>(query);
+            expect_same("KdTree<4> skips 3 array", h, correct, false);
+            KnnTrees::KdTree<dims> kd2(points, pointCount, 0);
+            kd2.batchKnn<k, true>(query);
+            expect_same("KdTree<4> skips 0 heap", h, correct, true);
+            KnnTrees::BallTree<dims> ball(points, pointCount, 3);
+            ball.batchKnn<k, false>(query);
+            expect_same("BallTree<4> skips 3 array", h, correct, false);
+            KnnTrees::BallTree<dims> ball2(points, pointCount, 0);
+            ball2.batchKnn<k, true>(query);
+            expect_same("BallTree<4> skips 0 heap", h, correct, true);
+        }
+
         bfg.batchKnn<k, 16, true, false>(dQuery);
         fetch(got);
         expect_same("BruteForceGpu<4> heap", got, correct, true);
@@ -117,7 +145,7 @@ int main() {
                                                reinterpret_cast<KnnTrees::Array<int, 4096>*>(dIdx),
                                                reinterpret_cast<KnnTrees::Array<float, 4096>*>(dDist)};
             bfg.batchKnn<4096, 1024, true, true>(big);
-        } catch (const std::runtime_error&) {
+        } catch (const KnnTrees::CudaException&) {  // the reference's exception name
             threw = true;
         }
         std::printf("%-44s %s\n", "oversized LDS request throws", threw ? "ok" : "FAILED");
