@@ -1052,10 +1052,10 @@ void alloc_tree(Handle& h) {
         h.nodes.alloc((size_t)I.node_size * (2 * D + 2));
         h.dims.alloc(I.node_size);
     }
-    // One spare slot past the last leaf: the wave-cooperative leaf scan
-    // loads one slot ahead of the one it tests and discards that load at
-    // the end of a group.
-    h.leaves.alloc((size_t)(I.index_size + 1) * (1 + D));
+    // Four spare slots past the last leaf: the wave-cooperative leaf scans
+    // load one slot (or one batch of four) ahead of the ones they test and
+    // discard that load at the end of a group.
+    h.leaves.alloc((size_t)(I.index_size + 4) * (1 + D));
     h.bbox.alloc(2 * (size_t)D);
 }
 

@@ -110,6 +110,36 @@ __device__ __forceinline__ float select_dim(const float (&q)[D], uint32_t dim) {
 // descending keys, i.e. a valid max-heap with the K-th neighbour at slot 0.
 // ---------------------------------------------------------------------------
 
+// Compare-exchange of two keys: a gets the smaller, b the larger.
+__device__ __forceinline__ void cmpx(uint64_t& a, uint64_t& b) {
+    const bool lt = a < b;
+    const uint64_t lo = lt ? a : b, hi = lt ? b : a;
+    a = lo;
+    b = hi;
+}
+
+// Bitonic sorting network: N (a power of two) keys in registers, ascending.
+// Every index is static, so the keys stay in VGPRs.
+template <int N>
+__device__ __forceinline__ void bitonic_sort(uint64_t (&c)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j >= 1; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    if ((i & k) == 0)
+                        cmpx(c[i], c[l]);
+                    else
+                        cmpx(c[l], c[i]);
+                }
+            }
+        }
+    }
+}
+
 // Sorted ascending array held in VGPRs, KCAP >= K static slots.  The first
 // KCAP-K slots hold key 0 (below every real key except (0.0, index 0), with
 // which it is interchangeable), so the K-th key always sits at slot KCAP-1
@@ -142,6 +172,29 @@ struct RegArray {
             ci = cp;
         }
         s[0] = ci ? s[0] : key;
+        return s[KCAP - 1];
+    }
+    // Merges C <= KCAP candidate keys sorted ascending (C a power of two;
+    // keys that belong nowhere may be ~0): the buffer becomes the KCAP
+    // smallest of buffer and candidates, ascending, which is what inserting
+    // the candidates one by one would leave.  With the candidates padded to
+    // KCAP by ~0 keys and reversed, slot i takes min(s[i], c[KCAP-1-i]): the
+    // KCAP smallest keys of both, as a bitonic sequence, which half-cleaner
+    // stages sort.  Returns the K-th key.
+    template <int C>
+    __device__ __forceinline__ uint64_t merge_sorted(const uint64_t (&c)[C]) {
+        static_assert(C <= KCAP, "more candidates than slots");
+#pragma unroll
+        for (int i = KCAP - C; i < KCAP; ++i) {
+            const uint64_t v = c[KCAP - 1 - i];
+            s[i] = v < s[i] ? v : s[i];
+        }
+#pragma unroll
+        for (int j = KCAP >> 1; j >= 1; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < KCAP; ++i)
+                if ((i & j) == 0) cmpx(s[i], s[i + j]);
+        }
         return s[KCAP - 1];
     }
     // Raw keys, ascending (partial results of a split scan).
@@ -274,6 +327,13 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t grid) {
     const uint32_t x = b % kXcds, j = b / kXcds;
     return x * per + (x < rem ? x : rem) + j;
 }
+
+// Register K-buffers take candidates in sorted batches (merge_sorted).
+template <class B> struct IsRegArray { static constexpr bool value = false; };
+template <int KCAP, bool DESC> struct IsRegArray<RegArray<KCAP, DESC>> {
+    static constexpr bool value = true;
+    static constexpr int cap = KCAP;
+};
 
 template <int KIND> struct BufOf;
 template <> struct BufOf<BUF_REG8> { using T = RegArray<8>; static constexpr bool lds = false; };

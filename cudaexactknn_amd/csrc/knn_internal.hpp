@@ -237,8 +237,8 @@ void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial = nullptr, size_t
 void query_bf_bf16(Handle& h, const QueryArgs& a);          // knn_bf16.hip
 void prepare_bf16(Handle& h, hipStream_t st);               // knn_bf16.hip
 void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
-                uint32_t* d_short_flags);                    // knn_tree_query.hip
-void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm);  // knn_tree_host.hip
+                uint32_t* d_short_flags);                                     // knn_tree_host.hip
+void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm);      // knn_tree_host.hip
 // Query ordering on caller-provided scratch: keys, values, their sort
 // alternates (nq words each) and the radix sort's temporary words.
 struct OrderScratch {

@@ -5,13 +5,14 @@
 
 namespace knn {
 
-// Minimum waves per SIMD asked of the compiler: 8 for the register K-buffers
-// of K <= 16 at D <= 4, whose natural allocation (~70 VGPRs) stops at 7; the
-// few values it then keeps in scratch cost less than the extra wave gains
-// (headline kd traversal 12.84 -> 12.42 ms).  Other forms keep the default.
+// Minimum waves per SIMD asked of the compiler: 6 for the register K-buffers
+// of K <= 16 at D <= 4.  The four-slot leaf scan needs more registers than
+// the one-slot scan did; asking for 8 waves made the compiler keep ~130
+// bytes per lane in scratch (headline kd traversal 16.0 ms at 8 waves, 11.4
+// at 7, 10.9 at 6, 10.9 at 5 and at 4).  Other forms keep the default.
 template <int D, int KIND>
 constexpr int kd_packet_min_waves() {
-    return (D <= 4 && (KIND == BUF_REG8 || KIND == BUF_REG16 || KIND == BUF_REGH8 || KIND == BUF_REGH16)) ? 8 : 1;
+    return (D <= 4 && (KIND == BUF_REG8 || KIND == BUF_REG16 || KIND == BUF_REGH8 || KIND == BUF_REGH16)) ? 6 : 1;
 }
 
 template <int D, int KIND, bool COUNT>

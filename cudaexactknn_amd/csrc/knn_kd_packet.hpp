@@ -94,10 +94,34 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
     // every lane enters the shared traversal with a tight K-th key.  The
     // shared traversal skips that bucket for this lane.
     uint32_t land0 = 0xFFFFFFFFu, landN = 0;
-    if (active) {
-        // Capped at the scan level, so every leaf group of the shared
-        // traversal lies wholly inside or wholly outside the primed range.
-        const uint32_t pl = min(A.prime_level, stop);
+    // Capped at the scan level, so every leaf group of the shared
+    // traversal lies wholly inside or wholly outside the primed range.
+    const uint32_t pl = min(A.prime_level, stop);
+    if constexpr (IsRegArray<decltype(buf)>::value) {
+        // Every lane's primed subtree has the same slot count (one level),
+        // and all lanes fill their buffers at once: sorted batches.
+        uint32_t n = 0, dd = 0;
+        for (uint32_t l = 0; l < pl; ++l) {
+            const float sv = A.nodes[active ? n : 0u];
+            float x = q[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) x = ((uint32_t)d == dd) ? q[d] : x;
+            n = 2 * n + 1 + (x < sv ? 0u : 1u);
+            dd = (dd + 1 == SD) ? 0 : dd + 1;
+        }
+        if (active) nbytes += 4u * pl;
+        uint32_t s0, cnt;
+        leaf_range(A, n, pl, s0, cnt);
+        if (active) {
+            land0 = s0;
+            landN = cnt;
+        }
+        QPoint<D> qp;
+#pragma unroll
+        for (int d = 0; d < D; ++d) qp.v[d] = q[d];
+        constexpr int C = IsRegArray<decltype(buf)>::cap < 8 ? IsRegArray<decltype(buf)>::cap : 8;
+        scan_slots_merged<D, C>(A.leaves, s0, uni(cnt), active, qp, buf, worst, whi, nslots);
+    } else if (active) {
         uint32_t n = 0, dd = 0;
         for (uint32_t l = 0; l < pl; ++l) {
             const float sv = A.nodes[n];
@@ -225,7 +249,11 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
 #pragma unroll
                 for (int d = 0; d < D; ++d) qp.v[d] = q[d];
                 const uint32_t s0 = uni(slot0);
-                packet_scan<D>(A.leaves, s0, uni(count), qp, active && s0 - land0 >= landN, buf, worst, whi,
+                const uint32_t cnt = uni(count);
+                if (D <= 4 && cnt >= 4)
+                    packet_scan4<D>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst, whi, nslots);
+                else
+                    packet_scan<D>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst, whi,
                                nslots);
             }
             // pop until some lane needs the entry

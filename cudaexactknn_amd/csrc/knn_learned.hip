@@ -280,9 +280,6 @@ __global__ void rows_kth_kernel(const float* __restrict__ dist, const int32_t* _
     full[i] = (idx[(size_t)i * K] >= 0 && idx[(size_t)i * K + K - 1] >= 0) ? 1u : 0u;
 }
 
-void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
-                uint32_t* d_short_flags);
-void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm);
 __global__ void gather_rows_kernel(const float* __restrict__ src, const uint32_t* __restrict__ ids, uint32_t m,
                                    uint32_t w, float* __restrict__ dst);
 __global__ void scatter_rows_kernel(const int32_t* __restrict__ si, const float* __restrict__ sd,

@@ -113,6 +113,57 @@ __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uin
     }
 }
 
+// Scan of a lane's own leaf slots [slot0, slot0 + count) into a register
+// K-buffer, C slots at a time: the C keys (padding slots give ~0) are sorted
+// by a bitonic network and merged into the buffer, unless no lane of the
+// wave has a key below its K-th.  For a wave whose lanes all fill their
+// buffers at once (the priming scan of the wave-cooperative traversals) this
+// replaces C insertions, each run for the whole wave, by one sort and one
+// merge.  `count` is wave-uniform; the result equals inserting the slots in
+// order.
+template <int D, int C, class Q, class B>
+__device__ __forceinline__ void scan_slots_merged(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
+                                                  bool active, const Q& q, B& buf, uint64_t& worst, uint32_t& whi,
+                                                  uint32_t& nslots) {
+    constexpr uint32_t S = 1 + D;
+    for (uint32_t o = 0; o < count; o += C) {
+        uint64_t c[C];
+        bool below = false;
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            const uint32_t slot = slot0 + o + j;
+            uint64_t key = ~0ull;
+            if (active && o + j < count) {
+                const float* e = leaves + (size_t)slot * S;
+                float v[S];
+                if constexpr (D == 3) {
+                    const float4 t = *reinterpret_cast<const float4*>(e);
+                    v[0] = t.x;
+                    v[1] = t.y;
+                    v[2] = t.z;
+                    v[3] = t.w;
+                } else {
+#pragma unroll
+                    for (uint32_t t = 0; t < S; ++t) v[t] = e[t];
+                }
+                const int32_t pi = __float_as_int(v[0]);
+                const float d2 = q.dist_to(v + 1);
+                if (pi != -1) key = make_key(d2, (uint32_t)pi);
+            }
+            c[j] = key;
+            below |= key < worst;
+        }
+        if (__ballot(below) != 0ull) {
+            bitonic_sort<C>(c);
+            if (active) {
+                worst = buf.merge_sorted(c);
+                whi = key_hi(worst);
+            }
+        }
+    }
+    if (active) nslots += count;
+}
+
 // Wave-cooperative scan of one leaf group [slot0, slot0 + count) (count,
 // slot0 wave-uniform).  Slot addresses are wave-uniform, so slots are read
 // with scalar loads: one load per slot for the index and the coordinates
@@ -172,6 +223,61 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
         test(v);
 #pragma unroll
         for (uint32_t t = 0; t < S; ++t) v[t] = nx[t];
+    }
+    nslots += count;
+}
+
+// packet_scan in batches of four slots, for D <= 4 and groups of a multiple
+// of four slots.  The four slots (4 (1 + D) words) arrive in one scalar
+// load, issued one batch ahead; the four tests run back to back and a single
+// branch skips the batch when no lane passes any of them, so the scalar loop
+// and branch work per slot is a quarter of the one-slot loop's.  A padding
+// slot (index -1, the only negative index) gets its threshold's sign bit set
+// from the index's own sign bit, one vector instruction with the threshold's
+// other bits.  Insertions inside a batch run in slot order, each re-tested
+// against the lane's K-th key of the moment, so the result equals the
+// one-slot loop's.
+template <int D, class Q, class B>
+__device__ __forceinline__ void packet_scan4(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
+                                             const Q& q, bool mine, B& buf, uint64_t& worst, uint32_t& whi,
+                                             uint32_t& nslots) {
+    constexpr uint32_t S = 1 + D;
+    const float* e = leaves + (size_t)slot0 * S;
+    auto clamp_hi = [](uint32_t w) { return w > 0x7FFFFFFFu ? 0x7FFFFFFFu : w; };
+    uint32_t whim = clamp_hi(whi) | (mine ? 0u : 0x80000000u);
+    float v[4 * S], nx[4 * S];
+#pragma unroll
+    for (uint32_t t = 0; t < 4 * S; ++t) v[t] = e[t];
+    for (uint32_t o = 0; o < count; o += 4) {
+        // The look-ahead of the last batch reads up to four slots past the
+        // group: the leaves array carries four spare slots for it.
+#pragma unroll
+        for (uint32_t t = 0; t < 4 * S; ++t) nx[t] = e[(size_t)(o + 4) * S + t];
+        float d2[4];
+        bool pass[4];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            d2[j] = q.dist_to(&v[j * S + 1]);
+            const uint32_t thr = whim | (__float_as_uint(v[j * S]) & 0x80000000u);
+            pass[j] = (int32_t)f2u(d2[j]) <= (int32_t)thr;
+            any |= pass[j];
+        }
+        if (any) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (pass[j]) {
+                    const uint64_t key = make_key(d2[j], __float_as_uint(v[j * S]));
+                    if (key < worst) {
+                        worst = buf.consider(key);
+                        whi = key_hi(worst);
+                        whim = clamp_hi(whi);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < 4 * S; ++t) v[t] = nx[t];
     }
     nslots += count;
 }
