@@ -191,7 +191,7 @@ class _Structure:
 
 
 def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, leaf_span=0, stream=None,
-          count_visits=False, per_lane=False, morton=False, xcd_remap=True, tile=0):
+          count_visits=False, per_lane=False, morton=False, xcd_remap=True, tile=0, learned_stats=False):
     o = QueryOpts()
     o.block = int(BLOCK_SIZE or 0)
     o.heap = int(bool(heap))
@@ -200,7 +200,8 @@ def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, le
     o.form = int(form)
     o.flags = ((0 if sort_queries else _lib.KNN_FLAG_NO_QUERY_SORT) | (_lib.KNN_FLAG_COUNT_VISITS if count_visits else 0)
                | (_lib.KNN_FLAG_PER_LANE if per_lane else 0) | (_lib.KNN_FLAG_MORTON_ORDER if morton else 0)
-               | (0 if xcd_remap else _lib.KNN_FLAG_NO_XCD_REMAP))
+               | (0 if xcd_remap else _lib.KNN_FLAG_NO_XCD_REMAP)
+               | (_lib.KNN_FLAG_LEARNED_STATS if learned_stats else 0))
     o.leaf_span = int(leaf_span)
     o.tile = int(tile)
     o.stream = stream
@@ -231,11 +232,13 @@ class _Tree(_Structure):
     def batchKnn(self, query: Query, K: int, BLOCK_SIZE: int = 0, USE_HEAP_TRACKING: bool = False,
                  USE_TRACKING_SHMEM: bool = False, USE_STACK_SHMEM: bool = False, form: int = FORM_PLAIN,
                  sort_queries: bool = True, leaf_span: int = 0, stream=None, count_visits: bool = False,
-                 per_lane: bool = False, morton: bool = False, xcd_remap: bool = True) -> None:
+                 per_lane: bool = False, morton: bool = False, xcd_remap: bool = True,
+                 learned_stats: bool = False) -> None:
         """batchKnn<K, BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM>(Query&)
         (src/KdTreeGpu.cuh:393-410, src/BallTreeGpu.cuh:631-648)."""
         self._query(query, K, _opts(BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM, form,
-                                    sort_queries, leaf_span, stream, count_visits, per_lane, morton, xcd_remap))
+                                    sort_queries, leaf_span, stream, count_visits, per_lane, morton, xcd_remap,
+                                    learned_stats=learned_stats))
 
     def build_ms(self) -> float:
         return float(_lib.load().knn_last_build_ms(self._h))

@@ -25,6 +25,7 @@ KNN_FLAG_COUNT_VISITS = 0x2
 KNN_FLAG_PER_LANE = 0x4
 KNN_FLAG_MORTON_ORDER = 0x8
 KNN_FLAG_NO_XCD_REMAP = 0x10
+KNN_FLAG_LEARNED_STATS = 0x20
 
 # Every symbol include/knn_mi355x.h declares.
 EXPORTED = (

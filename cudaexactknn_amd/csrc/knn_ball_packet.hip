@@ -5,7 +5,7 @@
 
 namespace knn {
 
-template <int D, int KIND, bool COUNT>
+template <int D, int KIND, bool COUNT, bool LEARN = false>
 __global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shrink_a, float shrink_b) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     const uint32_t BS = blockDim.x, tid = threadIdx.x;
@@ -13,7 +13,7 @@ __global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shri
     const uint32_t nwaves = (BS + 63) >> 6;
     uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth;
     uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, 1));
-    ball_packet_run<D, KIND, COUNT>(A, shrink_a, shrink_b, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
+    ball_packet_run<D, KIND, COUNT, LEARN>(A, shrink_a, shrink_b, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
 }
 
 using BallFn = void (*)(TreeArgs, float, float);
@@ -29,6 +29,30 @@ static BallFn bp_pick(int kind) {
         case BUF_REGH8: return ball_packet_kernel<D, BUF_REGH8, C>;
         case BUF_REGH16: return ball_packet_kernel<D, BUF_REGH16, C>;
         default: return ball_packet_kernel<D, BUF_REGH32, C>;
+    }
+}
+
+template <int D>
+static BallFn bp_pick_learned(int kind) {
+    switch (kind) {
+        case BUF_REG8: return ball_packet_kernel<D, BUF_REG8, false, true>;
+        case BUF_REG16: return ball_packet_kernel<D, BUF_REG16, false, true>;
+        case BUF_REG32: return ball_packet_kernel<D, BUF_REG32, false, true>;
+        case BUF_REGH8: return ball_packet_kernel<D, BUF_REGH8, false, true>;
+        case BUF_REGH16: return ball_packet_kernel<D, BUF_REGH16, false, true>;
+        case BUF_REGH32: return ball_packet_kernel<D, BUF_REGH32, false, true>;
+        default: return nullptr;
+    }
+}
+
+// The learned form's kernels (register K-buffers only).
+BallFn ball_packet_learned_kernel_for(uint32_t D, int kind) {
+    switch (D) {
+        case 2: return bp_pick_learned<2>(kind);
+        case 3: return bp_pick_learned<3>(kind);
+        case 4: return bp_pick_learned<4>(kind);
+        case 8: return bp_pick_learned<8>(kind);
+        default: return nullptr;
     }
 }
 

@@ -16,11 +16,13 @@ __device__ __forceinline__ float unif_b(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-template <int D, int KIND, bool COUNT = false>
+// LEARN: the learned form's second pass for lanes the predicted radius left
+// short, as in kd_packet_run.
+template <int D, int KIND, bool COUNT = false, bool LEARN = false>
 __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_a, float shrink_b, uint32_t i,
                                                 uint32_t* stack, uint64_t* kbuf, uint32_t BS) {
     const uint32_t tid = threadIdx.x;
-    const bool active = i < A.nq;
+    bool active = i < A.nq;
     const uint32_t qi = active ? (A.perm ? A.perm[i] : i) : 0u;
     QPoint<D> q;
 #pragma unroll
@@ -38,10 +40,12 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
     // Priming: each lane descends to its own landing bucket (nearer centroid
     // at each level) and scans it; the shared traversal then skips it.
     uint32_t land0 = 0xFFFFFFFFu, landN = 0;
+    // Capped at the scan level, so every leaf group of the shared
+    // traversal lies wholly inside or wholly outside the primed range.
+    const uint32_t pl = min(A.prime_level, stop);
     if (active) {
-        // Capped at the scan level, so every leaf group of the shared
-        // traversal lies wholly inside or wholly outside the primed range.
-        const uint32_t pl = min(A.prime_level, stop);
+        // Sorted-batch priming (as in the kd traversal) measured no faster
+        // here: 16.4 vs 16.3 ms on the 16.7M-point mixture.
         uint32_t n = 0;
         for (uint32_t l = 0; l < pl; ++l) {
             const float* nd = A.nodes + (size_t)n * nstride;
@@ -58,7 +62,26 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
     const uint32_t prime_bytes = nbytes, prime_slots = nslots;
     nbytes = nslots = 0;
     uint32_t node = 0, level = 0, sp = 0;
+    uint64_t floor = 0;
 
+    for (int pass = 0; pass < (LEARN ? 2 : 1); ++pass) {
+    if constexpr (LEARN) {
+        if (pass == 1) {
+            const bool miss = active && tightened && (uint32_t)worst == 0xFFFFFFFFu;
+            if (__ballot(miss) == 0ull) break;
+            if (miss) {
+                floor = (uint64_t)(key_hi(worst) + 1u) << 32;
+                worst = buf.relax(worst, ((uint64_t)f2u(A.maxd2) << 32) | kEmptyLo);
+                whi = key_hi(worst);
+                if (A.miss_count) atomicAdd(A.miss_count, 1u);
+            }
+            active = miss;
+            tightened = false;
+            land0 = 0xFFFFFFFFu;  // the primed range is searched again
+            landN = 0;
+            node = level = sp = 0;
+        }
+    }
     if (__ballot(active) != 0ull) {
         for (;;) {
             if (level < stop) {
@@ -97,8 +120,10 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
                 uint32_t slot0, count;
                 leaf_range(A, node, level, slot0, count);
                 const uint32_t s0 = uni_b(slot0);
-                packet_scan<D>(A.leaves, s0, uni_b(count), q, active && s0 - land0 >= landN, buf, worst, whi,
-                               nslots);
+                // The four-slot batches of the kd scan were slower here
+                // (18.0 vs 16.4 ms on the 16.7M-point mixture).
+                packet_scan<D, LEARN>(A.leaves, s0, uni_b(count), q, active && s0 - land0 >= landN, buf, worst, whi,
+                                      nslots, floor);
             }
             bool found = false;
             while (sp > 0) {
@@ -122,6 +147,8 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
             if (!found) break;
         }
     }
+    }
+    active = i < A.nq;
     const uint64_t act = __ballot(active);
     const bool lead = act != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)act) - 1);
     if (active)

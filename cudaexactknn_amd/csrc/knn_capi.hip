@@ -109,6 +109,7 @@ QueryArgs resolve_args(const Handle& h, const float* d_q, uint32_t q, float max_
     a.count_visits = (o.flags & KNN_FLAG_COUNT_VISITS) != 0;
     a.per_lane = (o.flags & KNN_FLAG_PER_LANE) != 0;
     a.xcd_remap = (o.flags & KNN_FLAG_NO_XCD_REMAP) == 0;
+    a.learned_stats = (o.flags & KNN_FLAG_LEARNED_STATS) != 0;
     a.order_morton = (o.flags & KNN_FLAG_MORTON_ORDER) != 0 || h.kind == KNN_KIND_BALL;
     a.leaf_span = o.leaf_span == 0 ? default_span(h, k, a.per_lane) : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
     a.form = (knn_form)o.form;

@@ -197,6 +197,13 @@ struct RegArray {
         }
         return s[KCAP - 1];
     }
+    // Replaces every slot holding key `from` by key `to` (> from): the
+    // buffer stays sorted when `from` keys are its largest.
+    __device__ __forceinline__ uint64_t relax(uint64_t from, uint64_t to) {
+#pragma unroll
+        for (int i = 0; i < KCAP; ++i) s[i] = s[i] == from ? to : s[i];
+        return s[KCAP - 1];
+    }
     // Raw keys, ascending (partial results of a split scan).
     __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t K) {
 #pragma unroll

@@ -175,6 +175,7 @@ struct QueryArgs {
     uint32_t block;
     bool heap, track_lds, stack_lds, sort_queries, count_visits, per_lane, order_morton;
     bool xcd_remap = true;  // trees: XCD-contiguous block order
+    bool learned_stats = false;  // learned form: compute the radius statistics after the query
     uint32_t leaf_span;  // extra levels folded into one leaf scan
     uint32_t tile = 0;   // bf16 form: point tiles staged per barrier (0 = default)
     knn_form form;
@@ -236,8 +237,13 @@ void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hip
 void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial = nullptr, size_t partial_words = 0);
 void query_bf_bf16(Handle& h, const QueryArgs& a);          // knn_bf16.hip
 void prepare_bf16(Handle& h, hipStream_t st);               // knn_bf16.hip
+// Tree traversal.  d_init_r2: per-query starting radius^2 (learned form);
+// with d_miss_count the queries it leaves short are searched again inside
+// the kernel (learned_in_kernel must hold), else they are flagged in
+// d_short_flags for the caller.
 void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
-                uint32_t* d_short_flags);                                     // knn_tree_host.hip
+                uint32_t* d_short_flags, uint32_t* d_miss_count = nullptr);  // knn_tree_host.hip
+bool learned_in_kernel(const Handle& h, const QueryArgs& a, bool ordered);
 void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm);      // knn_tree_host.hip
 // Query ordering on caller-provided scratch: keys, values, their sort
 // alternates (nq words each) and the radix sort's temporary words.

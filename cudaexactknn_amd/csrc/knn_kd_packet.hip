@@ -15,7 +15,7 @@ constexpr int kd_packet_min_waves() {
     return (D <= 4 && (KIND == BUF_REG8 || KIND == BUF_REG16 || KIND == BUF_REGH8 || KIND == BUF_REGH16)) ? 6 : 1;
 }
 
-template <int D, int KIND, bool COUNT>
+template <int D, int KIND, bool COUNT, bool LEARN = false>
 __global__ void __launch_bounds__(256, (kd_packet_min_waves<D, KIND>())) kd_packet_kernel(TreeArgs A) {
     constexpr int ENT = 1 + 2 * D;
     extern __shared__ __align__(16) uint8_t lds_raw[];
@@ -24,7 +24,7 @@ __global__ void __launch_bounds__(256, (kd_packet_min_waves<D, KIND>())) kd_pack
     const uint32_t nwaves = (BS + 63) >> 6;
     uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth * ENT;
     uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, ENT));
-    kd_packet_run<D, KIND, COUNT>(A, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
+    kd_packet_run<D, KIND, COUNT, LEARN>(A, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
 }
 
 using TreeFn = void (*)(TreeArgs);
@@ -40,6 +40,31 @@ static TreeFn kdp_pick(int kind) {
         case BUF_REGH8: return kd_packet_kernel<D, BUF_REGH8, C>;
         case BUF_REGH16: return kd_packet_kernel<D, BUF_REGH16, C>;
         default: return kd_packet_kernel<D, BUF_REGH32, C>;
+    }
+}
+
+// The learned form's kernels (register K-buffers only; heap tracking emits
+// descending rows from the same sorted buffer).
+template <int D>
+static TreeFn kdp_pick_learned(int kind) {
+    switch (kind) {
+        case BUF_REG8: return kd_packet_kernel<D, BUF_REG8, false, true>;
+        case BUF_REG16: return kd_packet_kernel<D, BUF_REG16, false, true>;
+        case BUF_REG32: return kd_packet_kernel<D, BUF_REG32, false, true>;
+        case BUF_REGH8: return kd_packet_kernel<D, BUF_REGH8, false, true>;
+        case BUF_REGH16: return kd_packet_kernel<D, BUF_REGH16, false, true>;
+        case BUF_REGH32: return kd_packet_kernel<D, BUF_REGH32, false, true>;
+        default: return nullptr;
+    }
+}
+
+TreeFn kd_packet_learned_kernel_for(uint32_t D, int kind) {
+    switch (D) {
+        case 2: return kdp_pick_learned<2>(kind);
+        case 3: return kdp_pick_learned<3>(kind);
+        case 4: return kdp_pick_learned<4>(kind);
+        case 8: return kdp_pick_learned<8>(kind);
+        default: return nullptr;
     }
 }
 

@@ -63,12 +63,19 @@ __device__ __forceinline__ float seq_sum(const float (&g)[D]) {
 // One wave's traversal for the ordered queries i = (this lane's index).
 // `stack` is this wave's LDS stack, `kbuf` this lane's LDS K-buffer slot
 // (stride BS), used only by LDS buffer kinds.
-template <int D, int KIND, bool COUNT = false>
+//
+// LEARN (learned form, register K-buffers): the traversal starts from the
+// predicted radius (TreeArgs::init_r2).  A lane that ends it with an empty
+// slot found every point within that radius and fewer than K of them; a
+// second pass of the same wave traversal, for those lanes only, keeps what
+// they found and accepts only keys above the radius (the floor), so nothing
+// is found twice and the rows equal the plain traversal's.
+template <int D, int KIND, bool COUNT = false, bool LEARN = false>
 __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uint32_t* stack, uint64_t* kbuf,
                                               uint32_t BS) {
     constexpr int ENT = 1 + 2 * D;  // words per stack entry: node, lo[D], hi[D]
     const uint32_t tid = threadIdx.x;
-    const bool active = i < A.nq;
+    bool active = i < A.nq;
     const uint32_t qi = active ? (A.perm ? A.perm[i] : i) : 0u;
     float q[D];
 #pragma unroll
@@ -157,7 +164,33 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
 #pragma unroll
     for (int d = 0; d < D; ++d) g[d] = 0.0f;
     float cur = 0.0f;
+    uint64_t floor = 0;
 
+    for (int pass = 0; pass < (LEARN ? 2 : 1); ++pass) {
+    if constexpr (LEARN) {
+        if (pass == 1) {
+            const bool miss = active && tightened && (uint32_t)worst == 0xFFFFFFFFu;
+            if (__ballot(miss) == 0ull) break;
+            if (miss) {
+                floor = (uint64_t)(key_hi(worst) + 1u) << 32;
+                worst = buf.relax(worst, ((uint64_t)f2u(A.maxd2) << 32) | kEmptyLo);
+                whi = key_hi(worst);
+                if (A.miss_count) atomicAdd(A.miss_count, 1u);
+            }
+            active = miss;
+            tightened = false;
+            land0 = 0xFFFFFFFFu;  // the primed range is searched again
+            landN = 0;
+            node = level = dim = sp = 0;
+            cur = 0.0f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                lo[d] = -INFINITY;
+                hi[d] = INFINITY;
+                g[d] = 0.0f;
+            }
+        }
+    }
     if (__ballot(active) != 0ull) {
         for (;;) {
             if (level < stop) {
@@ -251,10 +284,11 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
                 const uint32_t s0 = uni(slot0);
                 const uint32_t cnt = uni(count);
                 if (D <= 4 && cnt >= 4)
-                    packet_scan4<D>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst, whi, nslots);
+                    packet_scan4<D, LEARN>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst, whi,
+                                           nslots, floor);
                 else
-                    packet_scan<D>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst, whi,
-                               nslots);
+                    packet_scan<D, LEARN>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst, whi,
+                                          nslots, floor);
             }
             // pop until some lane needs the entry
             bool found = false;
@@ -285,6 +319,8 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
             if (!found) break;
         }
     }
+    }
+    active = i < A.nq;
     const uint64_t act = __ballot(active);
     const bool lead = act != 0ull && (threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)act) - 1);
     if (active)

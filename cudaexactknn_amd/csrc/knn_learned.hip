@@ -29,6 +29,19 @@
 namespace knn {
 
 constexpr int F_IN = 12;
+// Calibration: the training sample is split into kCalBuckets groups of equal
+// size by predicted radius, and in each group the predicted radius^2 is
+// scaled by the kCalibrationQuantile quantile of true / predicted, so that
+// queries whose predictions are less certain (typically the far ones) get
+// wider margins.  The result is capped by the landing subtree's bound.  A
+// query the radius leaves short costs a second traversal, an overshoot only
+// some pruning, so the quantile is high.
+constexpr int kCalBuckets = 8;
+constexpr double kCalibrationQuantile = 0.999;
+// A bucket whose errors spread wider than this (calibration quantile over
+// median of true / predicted) gets no predicted radius at all: its queries
+// start from the landing subtree's bound, which never leaves one short.
+constexpr double kMaxSpread = 8.0;
 constexpr int H = 64;
 // Parameter layout: W1[H][F_IN], b1[H], W2[H][H], b2[H], W3[H], b3.
 constexpr int OFF_W1 = 0;
@@ -45,6 +58,13 @@ struct Mlp {
     float scale = 1.0f;        // calibration factor on the predicted r^2
     uint32_t K = 0;
     bool trained = false;
+    // Query-time buffers (grow-only): features, landing-subtree bound,
+    // starting radius^2, and the in-kernel second-pass counter.
+    DevBuf<float> feats, ub, r2;
+    DevBuf<uint32_t> miss;
+    // Calibration: bucket edges on the predicted log2 r^2 (kCalBuckets - 1)
+    // followed by the bucket scales (kCalBuckets).
+    DevBuf<float> calib;
 };
 
 void destroy_mlp(Mlp* m) { delete m; }
@@ -58,7 +78,12 @@ __device__ __forceinline__ float bf16_round(float f) {
 __device__ __forceinline__ float safe_log2(float x) { return log2f(fmaxf(x, 1e-30f)); }
 
 // --- features --------------------------------------------------------------
-__global__ void feature_kernel(TreeArgs A, int is_ball, const float* __restrict__ bbox, float* __restrict__ feats) {
+// ub[i] (optional): the K-th smallest squared distance among the points of
+// the query's landing subtree, or +inf when it holds fewer than K: an upper
+// bound on the query's K-th squared distance (a subset's K-th is never below
+// the whole set's), so a radius capped by it never leaves a slot empty.
+__global__ void feature_kernel(TreeArgs A, int is_ball, const float* __restrict__ bbox, float* __restrict__ feats,
+                               float* __restrict__ ub) {
 #pragma clang fp contract(off)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.nq) return;
@@ -141,14 +166,16 @@ __global__ void feature_kernel(TreeArgs A, int is_ball, const float* __restrict_
     f[9] = safe_log2(dmin);
     f[10] = safe_log2(best[K - 1] < INFINITY ? best[K - 1] : dmin);
     f[11] = safe_log2(face);
+    if (ub) ub[i] = A.K <= 32 ? best[K - 1] : INFINITY;
 }
 
 // --- MLP forward (FP32 or bf16 inference) --------------------------------------
 // out[i] = log2-r2 prediction in target units (de-standardised) and, when
-// r2out is set, r2out[i] = scale * 2^pred.
+// r2out is set, r2out[i] = scale * 2^pred, capped by ub[i] when ub is set.
 __global__ void mlp_forward_kernel(const float* __restrict__ params, int use_bf16, const float* __restrict__ norm,
                                    const float* __restrict__ x, uint32_t n, float scale, float* __restrict__ out,
-                                   float* __restrict__ r2out) {
+                                   float* __restrict__ r2out, const float* __restrict__ ub = nullptr,
+                                   const float* __restrict__ calib = nullptr) {
 #pragma clang fp contract(off)
     __shared__ float w[N_PARAMS];
     for (uint32_t t = threadIdx.x; t < (uint32_t)N_PARAMS; t += blockDim.x) {
@@ -181,7 +208,16 @@ __global__ void mlp_forward_kernel(const float* __restrict__ params, int use_bf1
     }
     if (norm) y = y * norm[2 * F_IN + 1] + norm[2 * F_IN];
     if (out) out[i] = y;
-    if (r2out) r2out[i] = scale * exp2f(y);
+    if (r2out) {
+        float sc = scale;
+        if (calib) {
+            int b = 0;
+            for (int e = 0; e < kCalBuckets - 1; ++e) b += y > calib[e] ? 1 : 0;
+            sc = calib[kCalBuckets - 1 + b];
+        }
+        const float r = sc * exp2f(y);
+        r2out[i] = ub ? fminf(r, ub[i]) : r;
+    }
 }
 
 // --- training: per-sample activations and deltas ----------------------------------
@@ -292,7 +328,8 @@ static uint32_t prime_level_for(const knn_tree_info& I, uint32_t K) {
     return pl;
 }
 
-static void compute_features(Handle& h, const float* d_q, uint32_t nq, uint32_t K, float* d_feats, hipStream_t st) {
+static void compute_features(Handle& h, const float* d_q, uint32_t nq, uint32_t K, float* d_feats, float* d_ub,
+                             hipStream_t st) {
     const knn_tree_info& I = h.info;
     TreeArgs A{};
     A.nodes = h.nodes.get();
@@ -307,7 +344,7 @@ static void compute_features(Handle& h, const float* d_q, uint32_t nq, uint32_t 
     A.nq = nq;
     A.K = K;
     hipLaunchKernelGGL(feature_kernel, dim3((nq + 127) / 128), dim3(128), 0, st, A, h.kind == KNN_KIND_BALL ? 1 : 0,
-                       h.bbox.get(), d_feats);
+                       h.bbox.get(), d_feats, d_ub);
     KNN_CHECK_LAUNCH();
 }
 
@@ -396,7 +433,7 @@ void learned_train(Handle& h, const float* d_sample, uint32_t ns, uint32_t K, ui
     order_queries(h, a, &perm);
     query_tree(h, a, perm, nullptr, nullptr);
     feats.alloc((size_t)ns * F_IN);
-    compute_features(h, d_sample, ns, K, feats.get(), st);
+    compute_features(h, d_sample, ns, K, feats.get(), nullptr, st);
     target.alloc(ns);
     DevBuf<uint32_t> full;
     full.alloc(ns);
@@ -468,9 +505,15 @@ void learned_train(Handle& h, const float* d_sample, uint32_t ns, uint32_t K, ui
     dH2.alloc((size_t)B * H);
     dY.alloc(B);
     se.alloc(B);
+    // The last fifth of the (random-order) sample is held out of training
+    // and calibrates the radius: in-sample errors understate the errors on
+    // new queries.
+    const uint32_t ntrain = n >= 80 ? n - n / 5 : n;
+    const uint32_t ncal = n - ntrain ? n - ntrain : n;
+    const uint32_t cal0 = n - ncal;
     for (uint32_t ep = 0; ep < epochs; ++ep) {
-        for (uint32_t off = 0; off < n; off += B) {
-            const uint32_t nb = std::min(B, n - off);
+        for (uint32_t off = 0; off < ntrain; off += B) {
+            const uint32_t nb = std::min(B, ntrain - off);
             if (nb < 8) continue;
             hipLaunchKernelGGL(mlp_backprop_kernel, dim3((nb + 127) / 128), dim3(128), 0, st, M.params.get(),
                                dX.get() + (size_t)off * F_IN, dT.get() + off, nb, H1.get(), H2.get(), dH1.get(),
@@ -485,25 +528,50 @@ void learned_train(Handle& h, const float* d_sample, uint32_t ns, uint32_t K, ui
         }
     }
     KNN_CHECK_LAUNCH();
-    // Final loss over the sample, and the calibration factor: the 98th
-    // percentile of true / predicted r^2 with bf16 inference.
+    // Final loss over the training part, and the calibration (bf16
+    // inference, as at query time) over the held-out part.
     float loss = 0.0f;
-    mlp_gradient(M.params.get(), dX.get(), dT.get(), n, grad.get(), &loss, st);
+    mlp_gradient(M.params.get(), dX.get(), dT.get(), ntrain, grad.get(), &loss, st);
     DevBuf<float> pred;
-    pred.alloc(n);
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3((n + 127) / 128), dim3(128), 0, st, M.params.get(), 1, nullptr,
-                       dX.get(), n, 1.0f, pred.get(), nullptr);
-    std::vector<float> hp(n);
-    KNN_HIP(hipMemcpyAsync(hp.data(), pred.get(), 4ull * n, hipMemcpyDeviceToHost, st));
+    pred.alloc(ncal);
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3((ncal + 127) / 128), dim3(128), 0, st, M.params.get(), 1, nullptr,
+                       dX.get() + (size_t)cal0 * F_IN, ncal, 1.0f, pred.get(), nullptr);
+    std::vector<float> hp(ncal);
+    KNN_HIP(hipMemcpyAsync(hp.data(), pred.get(), 4ull * ncal, hipMemcpyDeviceToHost, st));
     KNN_HIP(hipStreamSynchronize(st));
-    std::vector<double> ratio(n);
-    for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t nc = ncal;
+    std::vector<std::pair<double, double>> pr(nc);  // (predicted log2 r^2, true / predicted r^2)
+    for (uint32_t i = 0; i < nc; ++i) {
         const double pl2 = hp[i] * nrm[2 * F_IN + 1] + nrm[2 * F_IN];
-        const double tl2 = T[i] * nrm[2 * F_IN + 1] + nrm[2 * F_IN];
-        ratio[i] = std::exp2(tl2 - pl2);
+        const double tl2 = T[cal0 + i] * nrm[2 * F_IN + 1] + nrm[2 * F_IN];
+        pr[i] = {pl2, std::exp2(tl2 - pl2)};
     }
-    std::sort(ratio.begin(), ratio.end());
-    M.scale = (float)(ratio[(size_t)(0.98 * (n - 1))] * 1.0001);
+    std::sort(pr.begin(), pr.end());
+    auto quantile = [](std::vector<double>& v, double q = kCalibrationQuantile) {
+        std::sort(v.begin(), v.end());
+        return v[(size_t)(q * (v.size() - 1))] * 1.0001;
+    };
+    {
+        std::vector<double> r;
+        for (auto& v : pr) r.push_back(v.second);
+        M.scale = (float)quantile(r);
+    }
+    std::vector<float> cal(2 * kCalBuckets - 1);
+    for (int b = 0; b < kCalBuckets; ++b) {
+        const size_t lo = (size_t)nc * b / kCalBuckets, hi = (size_t)nc * (b + 1) / kCalBuckets;
+        std::vector<double> r;
+        for (size_t i = lo; i < hi; ++i) r.push_back(pr[i].second);
+        float sc = M.scale;
+        if (!r.empty()) {
+            const double qh = quantile(r), qm = quantile(r, 0.5);
+            sc = qh > kMaxSpread * qm ? INFINITY : (float)qh;
+        }
+        cal[kCalBuckets - 1 + b] = sc;
+        if (b > 0) cal[b - 1] = lo < nc ? (float)pr[lo].first : INFINITY;
+    }
+    M.calib.ensure(cal.size());
+    KNN_HIP(hipMemcpy(M.calib.get(), cal.data(), cal.size() * 4, hipMemcpyHostToDevice));
+    M.miss.ensure(1);
     M.trained = true;
     if (stats) {
         stats[0] = loss;
@@ -522,57 +590,74 @@ void query_tree_learned(Handle& h, const QueryArgs& a) {
         return;
     }
     EventTimer total(a.timed, a.stream);
-    DevBuf<float> feats, r2;
-    DevBuf<uint32_t> flags, ids, nflag;
-    feats.alloc((size_t)a.nq * F_IN);
-    r2.alloc(a.nq);
-    flags.alloc(a.nq);
-    compute_features(h, a.q, a.nq, a.K, feats.get(), a.stream);
+    M.feats.ensure((size_t)a.nq * F_IN);
+    M.ub.ensure(a.nq);
+    M.r2.ensure(a.nq);
+    compute_features(h, a.q, a.nq, a.K, M.feats.get(), M.ub.get(), a.stream);
     hipLaunchKernelGGL(mlp_forward_kernel, dim3((a.nq + 127) / 128), dim3(128), 0, a.stream, M.params.get(), 1,
-                       M.norm.get(), feats.get(), a.nq, M.scale, nullptr, r2.get());
-    launch_fill_u32(flags.get(), a.nq, 0u, a.stream);
+                       M.norm.get(), M.feats.get(), a.nq, M.scale, nullptr, M.r2.get(), M.ub.get(), M.calib.get());
     KNN_CHECK_LAUNCH();
     uint32_t* perm = nullptr;
     if (a.sort_queries && a.nq > 1) order_queries(h, a, &perm);
     const float order_ms = h.stats[7];
-    query_tree(h, a, perm, r2.get(), flags.get());
-    const float kernel_ms = h.stats[4];
-    // Re-run queries left with an empty slot, with the plain traversal.
-    std::vector<uint32_t> hf(a.nq);
-    KNN_HIP(hipMemcpyAsync(hf.data(), flags.get(), 4ull * a.nq, hipMemcpyDeviceToHost, a.stream));
-    KNN_HIP(hipStreamSynchronize(a.stream));
-    std::vector<uint32_t> miss;
-    for (uint32_t i = 0; i < a.nq; ++i)
-        if (hf[i]) miss.push_back(i);
-    if (!miss.empty()) {
-        const uint32_t m = (uint32_t)miss.size(), D = h.info.dims;
-        ids.alloc(m);
-        KNN_HIP(hipMemcpyAsync(ids.get(), miss.data(), 4ull * m, hipMemcpyHostToDevice, a.stream));
-        DevBuf<float> qsub, dsub;
-        DevBuf<int32_t> isub;
-        qsub.alloc((size_t)m * D);
-        isub.alloc((size_t)m * a.K);
-        dsub.alloc((size_t)m * a.K);
-        hipLaunchKernelGGL(gather_rows_kernel, dim3((uint32_t)(((size_t)m * D + 255) / 256)), dim3(256), 0, a.stream,
-                           a.q, ids.get(), m, D, qsub.get());
-        QueryArgs b = a;
-        b.q = qsub.get();
-        b.nq = m;
-        b.oidx = isub.get();
-        b.odist = dsub.get();
-        b.count_visits = false;
-        uint32_t* p2 = nullptr;
-        if (b.sort_queries && m > 1) order_queries(h, b, &p2);
-        query_tree(h, b, p2, nullptr, nullptr);
-        hipLaunchKernelGGL(scatter_rows_kernel, dim3((uint32_t)(((size_t)m * a.K + 255) / 256)), dim3(256), 0,
-                           a.stream, isub.get(), dsub.get(), ids.get(), m, a.K, a.oidx, a.odist);
-        KNN_CHECK_LAUNCH();
+    float kernel_ms = 0.0f;
+    uint32_t n_miss = 0;
+    if (learned_in_kernel(h, a, perm != nullptr)) {
+        // Queries the radius leaves short are searched again inside the
+        // traversal kernel: no host round trip, no allocation.
+        launch_fill_u32(M.miss.get(), 1, 0u, a.stream);
+        query_tree(h, a, perm, M.r2.get(), nullptr, M.miss.get());
+        kernel_ms = h.stats[4];
+        h.stats[3] = total.stop();
+        if (a.timed) KNN_HIP(hipMemcpy(&n_miss, M.miss.get(), 4, hipMemcpyDeviceToHost));
+    } else {
+        // Per-lane traversal (or no ordered batch): short queries are flagged
+        // and re-run by the plain traversal.
+        DevBuf<uint32_t> flags, ids;
+        flags.alloc(a.nq);
+        launch_fill_u32(flags.get(), a.nq, 0u, a.stream);
+        query_tree(h, a, perm, M.r2.get(), flags.get());
+        kernel_ms = h.stats[4];
+        std::vector<uint32_t> hf(a.nq);
+        KNN_HIP(hipMemcpyAsync(hf.data(), flags.get(), 4ull * a.nq, hipMemcpyDeviceToHost, a.stream));
+        KNN_HIP(hipStreamSynchronize(a.stream));
+        std::vector<uint32_t> miss;
+        for (uint32_t i = 0; i < a.nq; ++i)
+            if (hf[i]) miss.push_back(i);
+        if (!miss.empty()) {
+            const uint32_t m = (uint32_t)miss.size(), D = h.info.dims;
+            ids.alloc(m);
+            KNN_HIP(hipMemcpyAsync(ids.get(), miss.data(), 4ull * m, hipMemcpyHostToDevice, a.stream));
+            DevBuf<float> qsub, dsub;
+            DevBuf<int32_t> isub;
+            qsub.alloc((size_t)m * D);
+            isub.alloc((size_t)m * a.K);
+            dsub.alloc((size_t)m * a.K);
+            hipLaunchKernelGGL(gather_rows_kernel, dim3((uint32_t)(((size_t)m * D + 255) / 256)), dim3(256), 0,
+                               a.stream, a.q, ids.get(), m, D, qsub.get());
+            QueryArgs b = a;
+            b.q = qsub.get();
+            b.nq = m;
+            b.oidx = isub.get();
+            b.odist = dsub.get();
+            b.count_visits = false;
+            uint32_t* p2 = nullptr;
+            if (b.sort_queries && m > 1) order_queries(h, b, &p2);
+            query_tree(h, b, p2, nullptr, nullptr);
+            hipLaunchKernelGGL(scatter_rows_kernel, dim3((uint32_t)(((size_t)m * a.K + 255) / 256)), dim3(256), 0,
+                               a.stream, isub.get(), dsub.get(), ids.get(), m, a.K, a.oidx, a.odist);
+            KNN_CHECK_LAUNCH();
+            KNN_HIP(hipStreamSynchronize(a.stream));
+        }
+        n_miss = (uint32_t)miss.size();
+        h.stats[3] = total.stop();
     }
-    h.stats[0] = (float)miss.size();
+    h.stats[0] = (float)n_miss;
     h.stats[4] = kernel_ms;
     h.stats[7] = order_ms;
-    if (a.timed) {
-        // Mean predicted / final K-th radius^2 (>1 = overshoot, <1 = miss).
+    if (a.timed && a.learned_stats) {
+        // On request (KNN_FLAG_LEARNED_STATS): mean predicted / final K-th
+        // radius^2 (>1 = overshoot, <1 = the radius left the query short).
         DevBuf<float> kth;
         DevBuf<uint32_t> full;
         kth.alloc(a.nq);
@@ -581,7 +666,7 @@ void query_tree_learned(Handle& h, const QueryArgs& a) {
                            a.K, kth.get(), full.get());
         std::vector<float> hk(a.nq), hr(a.nq);
         KNN_HIP(hipMemcpyAsync(hk.data(), kth.get(), 4ull * a.nq, hipMemcpyDeviceToHost, a.stream));
-        KNN_HIP(hipMemcpyAsync(hr.data(), r2.get(), 4ull * a.nq, hipMemcpyDeviceToHost, a.stream));
+        KNN_HIP(hipMemcpyAsync(hr.data(), M.r2.get(), 4ull * a.nq, hipMemcpyDeviceToHost, a.stream));
         KNN_HIP(hipStreamSynchronize(a.stream));
         double s = 0;
         uint32_t c = 0;
@@ -591,7 +676,6 @@ void query_tree_learned(Handle& h, const QueryArgs& a) {
                 ++c;
             }
         h.stats[2] = c ? (float)(s / c) : 0.0f;
-        h.stats[3] = total.stop();
     }
 }
 

@@ -36,6 +36,7 @@ struct TreeArgs {
     float* odist;
     const float* init_r2;     // optional per-query starting radius^2 (learned form)
     uint32_t* short_flags;    // optional: set to 1 for a query whose tightened radius left a slot empty
+    uint32_t* miss_count;     // optional (learned kernels): queries re-searched by the in-kernel second pass
     unsigned long long* counters;  // optional: [0] node bytes read, [1] leaf slots read
     uint32_t xcd_remap;    // 1: logical block = xcd_block(blockIdx.x, gridDim.x)
 };
@@ -174,10 +175,10 @@ __device__ __forceinline__ void scan_slots_merged(const float* __restrict__ leav
 // `mine` is false for a lane whose primed range holds this group: groups are
 // subtrees at the scan level and the primed range is a subtree at or above it,
 // so a group lies wholly inside or wholly outside it.
-template <int D, class Q, class B>
+template <int D, bool FLOOR = false, class Q, class B>
 __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
                                             const Q& q, bool mine, B& buf, uint64_t& worst, uint32_t& whi,
-                                            uint32_t& nslots) {
+                                            uint32_t& nslots, uint64_t floor = 0) {
     constexpr uint32_t S = 1 + D;
     const float* e = leaves + (size_t)slot0 * S;
     // One compare per slot: the lane's threshold gets its sign bit set when
@@ -195,7 +196,7 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
         const uint32_t padm = pi == -1 ? 0x80000000u : 0u;
         if ((int32_t)f2u(d2) <= (int32_t)(whim | padm)) {
             const uint64_t key = make_key(d2, (uint32_t)pi);
-            if (key < worst) {
+            if (key < worst && (!FLOOR || key >= floor)) {
                 worst = buf.consider(key);
                 whi = key_hi(worst);
                 whim = clamp_hi(whi);
@@ -237,10 +238,10 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
 // other bits.  Insertions inside a batch run in slot order, each re-tested
 // against the lane's K-th key of the moment, so the result equals the
 // one-slot loop's.
-template <int D, class Q, class B>
+template <int D, bool FLOOR = false, class Q, class B>
 __device__ __forceinline__ void packet_scan4(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
                                              const Q& q, bool mine, B& buf, uint64_t& worst, uint32_t& whi,
-                                             uint32_t& nslots) {
+                                             uint32_t& nslots, uint64_t floor = 0) {
     constexpr uint32_t S = 1 + D;
     const float* e = leaves + (size_t)slot0 * S;
     auto clamp_hi = [](uint32_t w) { return w > 0x7FFFFFFFu ? 0x7FFFFFFFu : w; };
@@ -268,7 +269,7 @@ __device__ __forceinline__ void packet_scan4(const float* __restrict__ leaves, u
             for (int j = 0; j < 4; ++j) {
                 if (pass[j]) {
                     const uint64_t key = make_key(d2[j], __float_as_uint(v[j * S]));
-                    if (key < worst) {
+                    if (key < worst && (!FLOOR || key >= floor)) {
                         worst = buf.consider(key);
                         whi = key_hi(worst);
                         whim = clamp_hi(whi);

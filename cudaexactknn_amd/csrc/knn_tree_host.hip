@@ -16,6 +16,8 @@ TreeFn kd_kernel_for(uint32_t D, int kind, bool stack_lds, bool& generic);
 BallFn ball_kernel_for(uint32_t D, int kind, bool stack_lds, bool& generic);
 TreeFn kd_packet_kernel_for(uint32_t D, int kind, bool count);
 BallFn ball_packet_kernel_for(uint32_t D, int kind, bool count);
+TreeFn kd_packet_learned_kernel_for(uint32_t D, int kind);
+BallFn ball_packet_learned_kernel_for(uint32_t D, int kind);
 int pick_buf_kind(bool heap, bool lds, uint32_t K);
 
 __global__ void morton_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D, const float* __restrict__ bbox,
@@ -131,8 +133,14 @@ void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& S, ui
     h.stats[7] = timer.stop();
 }
 
+bool learned_in_kernel(const Handle& h, const QueryArgs& a, bool ordered) {
+    const uint32_t D = h.info.dims;
+    const bool dims_ok = D == 2 || D == 3 || D == 4 || D == 8;
+    return dims_ok && ordered && !a.per_lane && a.block % 64 == 0 && a.K <= 32 && !a.count_visits;
+}
+
 void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
-                uint32_t* d_short_flags) {
+                uint32_t* d_short_flags, uint32_t* d_miss_count) {
     const knn_tree_info& I = h.info;
     TreeArgs A{};
     A.nodes = h.nodes.get();
@@ -156,13 +164,18 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     A.odist = a.odist;
     A.init_r2 = d_init_r2;
     A.short_flags = d_short_flags;
+    A.miss_count = d_miss_count;
+    // Learned form with the second pass in the kernel (register K-buffers).
+    const bool learn = d_init_r2 != nullptr && d_miss_count != nullptr;
+    require(!learn || learned_in_kernel(h, a, d_perm != nullptr), "internal: learned kernels do not apply",
+            KNN_ERR_INTERNAL);
     if (a.count_visits) {
         h.counters.ensure(2);
         launch_fill_u32(reinterpret_cast<uint32_t*>(h.counters.get()), 4, 0u, a.stream);
         A.counters = h.counters.get();
     }
 
-    const int kind = pick_buf_kind(a.heap, a.track_lds, a.K);
+    const int kind = pick_buf_kind(a.heap, a.track_lds && !learn, a.K);
     const bool lds_buf = kind == BUF_LDS_ARRAY || kind == BUF_LDS_HEAP;
     bool generic = false;
     TreeFn kfn = nullptr;
@@ -173,10 +186,16 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     const bool packet = !a.per_lane && d_perm != nullptr && BS % 64 == 0;
     if (packet) {
         if (BS > 256) BS = 256;
-        if (h.kind == KNN_KIND_KD)
+        if (learn) {
+            if (h.kind == KNN_KIND_KD)
+                kfn = kd_packet_learned_kernel_for(I.dims, kind);
+            else
+                bfn = ball_packet_learned_kernel_for(I.dims, kind);
+        } else if (h.kind == KNN_KIND_KD) {
             kfn = kd_packet_kernel_for(I.dims, kind, A.counters != nullptr);
-        else
+        } else {
             bfn = ball_packet_kernel_for(I.dims, kind, A.counters != nullptr);
+        }
     }
     // XCD-contiguous block order: measured 10 % faster for the per-lane
     // traversal on the 16.7M-point mixture; neutral (kd) or slower (ball) for

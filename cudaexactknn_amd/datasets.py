@@ -49,10 +49,13 @@ def scaled_normal(n_points: int = 1 << 20, n_queries: int = 1 << 16, dims: int =
 
 
 def anisotropic_mixture_8d(n_points: int = 1 << 22, n_queries: int = 1 << 20, n_clusters: int = 64,
-                           dims: int = 8, seed: int = 3, uniform_fraction: float = 0.2):
+                           dims: int = 8, seed: int = 3, uniform_fraction: float = 0.2, n_sample: int = 0):
     """64 rotated anisotropic Gaussians in R^8 (centres uniform in [0,10)^8, axis
     scales log-uniform in [0.01, 1], weights Dirichlet(1)); 80 % of queries from
-    the mixture, 20 % uniform over the points' bounding box."""
+    the mixture, 20 % uniform over the points' bounding box.  With n_sample > 0
+    a third array follows: n_sample more queries from the same mixture and
+    recipe (drawn after the others, which stay as they are), the learned
+    radius's training sample."""
     g = _rng(seed)
     centres = g.uniform(0.0, 10.0, (n_clusters, dims))
     scales = np.exp(g.uniform(np.log(0.01), np.log(1.0), (n_clusters, dims)))
@@ -72,12 +75,18 @@ def anisotropic_mixture_8d(n_points: int = 1 << 22, n_queries: int = 1 << 20, n_
         return x.astype(np.float32)
 
     pts = draw(n_points)
-    n_uni = int(round(n_queries * uniform_fraction))
-    qs_mix = draw(n_queries - n_uni)
     lo, hi = pts.min(axis=0).astype(np.float64), pts.max(axis=0).astype(np.float64)
-    qs_uni = (lo + (hi - lo) * g.random((n_uni, dims))).astype(np.float32)
-    qs = np.concatenate([qs_mix, qs_uni])
-    qs = qs[g.permutation(n_queries)]
+
+    def queries(m):
+        n_uni = int(round(m * uniform_fraction))
+        q_mix = draw(m - n_uni)
+        q_uni = (lo + (hi - lo) * g.random((n_uni, dims))).astype(np.float32)
+        q = np.concatenate([q_mix, q_uni])
+        return q[g.permutation(m)]
+
+    qs = queries(n_queries)
+    if n_sample:
+        return pts, qs, queries(n_sample)
     return pts, qs
 
 

@@ -46,22 +46,53 @@ def test_mlp_gradient_matches_oracle(cuda):
 
 @pytest.mark.parametrize("kind", ["kd", "ball"])
 def test_learned_form_is_exact(oracle, cuda, kind):
-    pts, qs = datasets.anisotropic_mixture_8d(200000, 12000, 64, 8, seed=3)
+    pts, qs, sample = datasets.anisotropic_mixture_8d(200000, 8000, 64, 8, seed=3, n_sample=4096)
     p = dev(pts, cuda)
     t = knn.KdTreeGpu(p, depth_skips=0) if kind == "kd" else knn.BallTreeGpu(p, depth_skips=1)
     import torch
-    sample = torch.from_numpy(qs[:4096]).to(cuda)
-    st = t.train_radius(sample, 32, epochs=15)
-    assert np.isfinite(st["loss"]) and st["loss"] < 1.0
-    test_q = qs[4096:]
-    want = oracle.bf_knn(pts, test_q, 32)
+    st = t.train_radius(torch.from_numpy(sample).to(cuda), 32, epochs=15)
+    # The loss is the mean squared error of the standardised log2 K-th
+    # radius^2: below 1 the network beats predicting the mean; here it must
+    # explain most of the variance.
+    print(kind, "training loss", st["loss"])
+    assert np.isfinite(st["loss"]) and st["loss"] < 0.5
+    want = oracle.bf_knn(pts, qs, 32)
     for heap in (False, True):
         w = want if not heap else (want[0][:, ::-1].copy(), want[1][:, ::-1].copy())
-        got = run(t, test_q, 32, cuda, USE_HEAP_TRACKING=heap, form=knn.FORM_LEARNED)
+        got = run(t, qs, 32, cuda, USE_HEAP_TRACKING=heap, form=knn.FORM_LEARNED)
         same(got, w, f"learned {kind} heap={heap}")
+    got = run(t, qs, 32, cuda, form=knn.FORM_LEARNED, learned_stats=True)
+    same(got, want, f"learned {kind} with statistics")
     s = t.last_query_stats()
-    print(kind, "reruns", s["rerun_queries"], "radius ratio", s["radius_ratio"])
-    assert s["rerun_queries"] < 0.5 * len(test_q)
-    # A radius of 0 for everyone: every query re-runs and results stay exact.
-    got = run(t, test_q, 32, cuda, md=np.inf, form=knn.FORM_LEARNED)
-    same(got, want, "learned again")
+    print(kind, "short queries", s["rerun_queries"], "radius ratio", s["radius_ratio"])
+    assert s["rerun_queries"] < 0.05 * len(qs)
+    assert s["radius_ratio"] >= 1.0 - 1e-6 or s["rerun_queries"] > 0
+    # The per-lane traversal takes the other path (short queries flagged and
+    # re-run by the plain traversal): same rows.
+    same(run(t, qs, 32, cuda, form=knn.FORM_LEARNED, per_lane=True), want, f"learned {kind} per-lane")
+
+
+@pytest.mark.parametrize("kind", ["kd", "ball"])
+def test_learned_second_pass_keeps_rows_exact(oracle, cuda, kind):
+    """A predictor trained on queries that sit on the points themselves
+    predicts radii far too small for queries spread over the bounding box:
+    most of them end the first pass short and are finished by the in-kernel
+    second pass (which keeps what the first pass found and accepts only
+    points beyond the radius).  Rows stay exact, with finite and negative
+    radii too."""
+    import torch
+    g = np.random.Generator(np.random.Philox(8))
+    pts, _ = datasets.anisotropic_mixture_8d(60000, 10, 64, 8, seed=5)
+    lo, hi = pts.min(0), pts.max(0)
+    qs = (lo + (hi - lo) * g.random((3000, 8))).astype(np.float32)
+    qs[:200] = pts[:200]
+    p = dev(pts, cuda)
+    t = knn.KdTreeGpu(p, depth_skips=0) if kind == "kd" else knn.BallTreeGpu(p, depth_skips=1)
+    t.train_radius(torch.from_numpy(pts[1000:5096].copy()).to(cuda), 16, epochs=10)
+    for md in (np.inf, 40.0, -1.0):
+        for heap in (False, True):
+            want = oracle.bf_knn(pts, qs, 16, max_d2=md, heap=heap)
+            got = run(t, qs, 16, cuda, md=md, USE_HEAP_TRACKING=heap, form=knn.FORM_LEARNED)
+            same(got, want, f"{kind} md={md} heap={heap}")
+            if md == np.inf and not heap:
+                assert t.last_query_stats()["rerun_queries"] > 100
