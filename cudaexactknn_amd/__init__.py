@@ -139,6 +139,15 @@ class _Structure:
                                               rd.ctypes.data, int(K), ctypes.byref(o), int(chunk)))
         return ri, rd
 
+    def reserve(self, q: int, K: int, **opts) -> None:
+        """Size the structure's query scratch for any batch of up to q
+        queries with this K and these options (knn_query_reserve), so that a
+        later batchKnn(..., stream=s) allocates nothing and can be captured
+        into a HIP graph.  Options as for batchKnn; stream is ignored."""
+        o = _opts(opts.pop("BLOCK_SIZE", 0), opts.pop("USE_HEAP_TRACKING", False), opts.pop("USE_TRACKING_SHMEM", False),
+                  opts.pop("USE_STACK_SHMEM", False), **opts)
+        _lib.check(_lib.load().knn_query_reserve(self._h, int(q), int(K), ctypes.byref(o)))
+
     def info(self) -> dict:
         ti = TreeInfo()
         _lib.check(_lib.load().knn_tree_get_info(self._h, ctypes.byref(ti)))

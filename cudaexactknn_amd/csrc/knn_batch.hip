@@ -289,39 +289,7 @@ __global__ void __launch_bounds__(256) batch_fallback_kernel(const Bf16Set* __re
         it = __shfl(it, 0, 64);
         if (it >= total) break;
         const uint2 e = ovf[it];
-        const Bf16Set& S = sets[e.x];
-        const uint32_t qi = e.y, K = S.K, D = S.D;
-        const uint64_t empty = ((uint64_t)f2u(S.maxd2) << 32) | kEmptyLo;
-        RegArray<32> buf;
-        uint64_t worst = buf.init(empty, K);
-        uint32_t whi = key_hi(worst);
-        const float* q = S.qs + (size_t)qi * D;
-        for (uint32_t j = lane; j < S.n; j += 64) {
-            const float d2 = dist2_rt(q, 1, S.pts + (size_t)j * D, D);
-            if (f2u(d2) <= whi) {
-                const uint64_t key = make_key(d2, j);
-                if (key < worst) {
-                    worst = buf.consider(key);
-                    whi = key_hi(worst);
-                }
-            }
-        }
-        uint64_t* mine = &s_keys[wave][lane * 32];
-        buf.write_keys(mine, K);
-        __threadfence_block();
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) {
-            RegArray<32> m;
-            uint64_t mw = m.init(empty, K);
-            for (uint32_t l = 0; l < 64; ++l)
-                for (uint32_t s = 0; s < K; ++s) {
-                    const uint64_t key = s_keys[wave][l * 32 + s];
-                    if (key < mw) mw = m.consider(key);
-                }
-            m.write(S.oidx + (size_t)qi * K, S.odist + (size_t)qi * K, K);
-        }
-        __threadfence_block();
-        __builtin_amdgcn_wave_barrier();
+        bf16_fallback_query<false>(sets[e.x], e.y, s_keys[wave]);
     }
 }
 

@@ -238,13 +238,52 @@ int pick_buf_kind(bool heap, bool lds, uint32_t K) {
 
 static bool buf_in_lds(int kind) { return kind == BUF_LDS_ARRAY || kind == BUF_LDS_HEAP; }
 
+// Point chunks (grid.y) of a brute-force query of nq rows in workgroups of
+// BS lanes: with too few query blocks to fill the CUs, the points are split
+// into S chunks, each writing partial top-K keys that a merge combines.
+static uint32_t bf_split(uint32_t n, uint32_t nq, uint32_t K, uint32_t BS) {
+    const uint32_t grid = (nq + BS - 1) / BS;
+    const uint32_t want_blocks = 4 * device_limits().cu_count;
+    uint32_t S = 1;
+    if (grid < want_blocks && n >= 2048) {
+        S = (want_blocks + grid - 1) / grid;
+        const uint32_t max_by_size = n / 1024;
+        if (S > max_by_size) S = max_by_size;
+        if (S > 64) S = 64;
+        if (S < 1) S = 1;
+        while (S > 1 && (size_t)nq * S * K * 8 > ((size_t)1 << 30)) S /= 2;
+    }
+    return S;
+}
+
+static uint32_t bf_block(const Handle& h, const QueryArgs& a) {
+    const uint32_t D = h.info.dims;
+    const bool big_d = D >= 32 && (D == 32 || D == 64 || D == 128);
+    return (big_d && a.block > 256) ? 256 : a.block;
+}
+
+size_t bf_scratch_bytes(const Handle& h, const QueryArgs& a) {
+    // The split depends on the query count only through the number of
+    // query blocks, and the partial keys grow with the rows: the largest
+    // need is at the last row count of some block count.
+    const uint32_t BS = bf_block(h, a);
+    size_t most = 0;
+    const uint32_t blocks = (a.nq + BS - 1) / BS;
+    for (uint32_t g = 1; g <= blocks; ++g) {
+        const uint32_t nq = (uint32_t)std::min<size_t>(a.nq, (size_t)g * BS);
+        const uint32_t S = bf_split(h.info.n, nq, a.K, BS);
+        if (S <= 1) break;  // more blocks only lower the split
+        most = std::max(most, (size_t)nq * S * a.K * 8);
+    }
+    return most;
+}
+
 void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial_ws, size_t partial_ws_words) {
     const uint32_t D = h.info.dims;
     const int kind = pick_buf_kind(a.heap, a.track_lds, a.K);
     // BLOCK_SIZE only changes the schedule, never the result: kernels built
     // for at most 256 lanes run larger requests as 256-lane workgroups.
-    const bool big_d = D >= 32 && (D == 32 || D == 64 || D == 128);
-    const uint32_t BS = (big_d && a.block > 256) ? 256 : a.block;
+    const uint32_t BS = bf_block(h, a);
     const uint32_t DP = (D + 3) & ~3u;
     uint32_t tile_pts = 8192 / DP;
     if (tile_pts > 1024) tile_pts = 1024;
@@ -271,21 +310,10 @@ void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial_ws, size_t partia
                 std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",
             KNN_ERR_INVALID_ARG);
     const uint32_t grid = (a.nq + BS - 1) / BS;
-    // Too few query blocks to fill the CUs: split the points into S chunks
-    // (grid.y), each writing partial top-K keys, then merge.
     const uint32_t n = h.info.n;
-    const uint32_t want_blocks = 4 * device_limits().cu_count;
-    uint32_t S = 1;
-    if (grid < want_blocks && n >= 2048) {
-        S = (want_blocks + grid - 1) / grid;
-        const uint32_t max_by_size = n / 1024;
-        if (S > max_by_size) S = max_by_size;
-        if (S > 64) S = 64;
-        if (S < 1) S = 1;
-        while (S > 1 && (size_t)a.nq * S * a.K * 8 > ((size_t)1 << 30)) S /= 2;
-        // A caller-provided buffer bounds the split (fewer chunks, same rows).
-        while (S > 1 && partial_ws && (size_t)a.nq * S * a.K > partial_ws_words) S /= 2;
-    }
+    uint32_t S = bf_split(n, a.nq, a.K, BS);
+    // A caller-provided buffer bounds the split (fewer chunks, same rows).
+    while (S > 1 && partial_ws && (size_t)a.nq * S * a.K > partial_ws_words) S /= 2;
     const uint32_t chunk = S > 1 ? (n + S - 1) / S : (n ? n : 1);
     uint64_t* partial = nullptr;
     if (S > 1 && partial_ws) {

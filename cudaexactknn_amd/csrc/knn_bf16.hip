@@ -92,21 +92,12 @@ __global__ void __launch_bounds__(256) bf16_rerank_lists_kernel(Bf16Set S, uint6
 void launch_bf_merge(const uint64_t* partial, uint32_t nq, uint32_t S, uint32_t K, bool descending, int32_t* oidx,
                      float* odist, hipStream_t st);
 
-__global__ void gather_rows_kernel(const float* __restrict__ src, const uint32_t* __restrict__ ids, uint32_t m,
-                                   uint32_t w, float* __restrict__ dst) {
-    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (size_t)m * w) return;
-    dst[g] = src[(size_t)ids[g / w] * w + g % w];
-}
-
-__global__ void scatter_rows_kernel(const int32_t* __restrict__ si, const float* __restrict__ sd,
-                                    const uint32_t* __restrict__ ids, uint32_t m, uint32_t K,
-                                    int32_t* __restrict__ oi, float* __restrict__ od) {
-    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (size_t)m * K) return;
-    const size_t dst = (size_t)ids[g / K] * K + g % K;
-    oi[dst] = si[g];
-    od[dst] = sd[g];
+// Sum of n u32 into *out (grid-stride, one atomic per thread with a nonzero sum).
+__global__ void sum_u32_kernel(const uint32_t* __restrict__ v, size_t n, unsigned long long* __restrict__ out) {
+    unsigned long long s = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += v[i];
+    if (s) atomicAdd(out, s);
 }
 
 uint32_t bf16_d16_of(uint32_t D) {
@@ -183,6 +174,94 @@ Bf16Params bf16_params(uint32_t D, uint32_t D16) {
     return P;
 }
 
+struct Bf16Config {
+    uint32_t D16, tiles, stage, want;
+    size_t stage_lds;
+    ScreenFn fn, sfn;
+};
+
+// Screen kernels and LDS staging for K on this point set, and the
+// workgroups that fill the GPU twice over.
+static Bf16Config bf16_config(const Handle& h, uint32_t K, uint32_t tile) {
+    Bf16Config c{};
+    c.D16 = bf16_d16_of(h.info.dims);
+    c.tiles = (h.info.n + 31) / 32;
+    uint32_t max_stage = 1;
+    switch (c.D16) {
+        case 1: c.fn = screen_for<1, false>(K); c.sfn = screen_for<1, true>(K); max_stage = bf16_stage_tiles_wide<1>(); break;
+        case 2: c.fn = screen_for<2, false>(K); c.sfn = screen_for<2, true>(K); max_stage = bf16_stage_tiles_wide<2>(); break;
+        case 4: c.fn = screen_for<4, false>(K); c.sfn = screen_for<4, true>(K); max_stage = bf16_stage_tiles_wide<4>(); break;
+        case 8: c.fn = screen_for<8, false>(K); c.sfn = screen_for<8, true>(K); max_stage = bf16_stage_tiles_wide<8>(); break;
+        default: c.fn = screen_for<16, false>(K); c.sfn = screen_for<16, true>(K); max_stage = bf16_stage_tiles_wide<16>(); break;
+    }
+    // Requested (or widest) staging, halved until it fits the device's
+    // per-workgroup LDS.
+    uint32_t stage = tile ? (tile < max_stage ? tile : max_stage) : max_stage;
+    while (stage > 1 && bf16_stage_bytes(c.D16, stage) > device_limits().max_lds_per_block) stage >>= 1;
+    c.stage = stage;
+    c.stage_lds = bf16_stage_bytes(c.D16, stage);
+    require(c.stage_lds <= device_limits().max_lds_per_block, "bf16 staging exceeds the per-workgroup LDS limit",
+            KNN_ERR_UNSUPPORTED);
+    // Point chunks: enough workgroups for two full rounds of the screen's
+    // resident workgroups per CU, so no partial last round leaves CUs at a
+    // fraction of their occupancy (3 resident at D <= 64 gave a 4-workgroup
+    // CU a second round at a third of its waves).
+    int per_cu = 0;
+    KNN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(c.fn), 256,
+                                                         c.stage_lds));
+    c.want = 2 * device_limits().cu_count * (uint32_t)(per_cu > 0 ? per_cu : 2);
+    return c;
+}
+
+struct Bf16Split {
+    uint32_t S, lists, cap;
+    size_t bytes;  // scratch for nq queries
+};
+
+static Bf16Split bf16_split(uint32_t n, uint32_t nq, uint32_t K, uint32_t tiles, uint32_t want) {
+    Bf16Split r{};
+    const uint32_t qblocks = (nq + 127) / 128;
+    uint32_t S = 1;
+    if (qblocks < want) {
+        S = (want + qblocks - 1) / qblocks;
+        const uint32_t max_s = tiles / 32 > 0 ? tiles / 32 : 1;
+        if (S > max_s) S = max_s;
+        if (S > 64) S = 64;
+    }
+    // One candidate list per (query, chunk, half-wave lane).  A lane that sees
+    // m points admits about K (1 + ln(m / K)) of them while its running bound
+    // tightens; the capacity doubles that, plus slack.
+    const uint32_t lists = 2 * S;
+    const double m = std::max(1.0, (double)n / (2.0 * S) / K);
+    uint32_t cap = (uint32_t)std::min(2048.0, 2.0 * K * (1.0 + std::log(m)) + 64.0);
+    cap = (cap + 15) & ~15u;
+    while (cap > 64 && (size_t)nq * lists * cap * 4 > ((size_t)1 << 30)) cap /= 2;
+    // scratch: partial keys [nq * lists * K] (u64) | cnt[nq * lists] |
+    // cand[nq * lists * cap] | ovf[nq] | ovf_flag[nq] | novf[4] | seed[nq]
+    r.S = S;
+    r.lists = lists;
+    r.cap = cap;
+    r.bytes = (size_t)nq * lists * K * 8 + ((size_t)nq * lists * (cap + 1) + 3 * (size_t)nq + 4) * 4;
+    return r;
+}
+
+// Grows the handle for bf16 queries of up to nq rows: the bf16 copy of the
+// points, and the scratch of the largest need over every row count up to nq
+// (the split depends on the row count only through the 128-row blocks).
+void bf16_reserve(Handle& h, uint32_t nq, uint32_t K, uint32_t tile, hipStream_t st) {
+    const uint32_t n = h.info.n, D = h.info.dims;
+    if (K > 32 || D > 256 || n == 0) return;
+    prepare_bf16(h, st);
+    const Bf16Config c = bf16_config(h, K, tile);
+    size_t most = 0;
+    const uint32_t blocks = (nq + 127) / 128;
+    for (uint32_t g = 1; g <= blocks; ++g) {
+        const uint32_t m = (uint32_t)std::min<size_t>(nq, (size_t)g * 128);
+        most = std::max(most, bf16_split(n, m, K, c.tiles, c.want).bytes);
+    }
+    h.q_scratch.ensure(most);
+}
+
 void query_bf_bf16(Handle& h, const QueryArgs& a) {
     const uint32_t n = h.info.n, D = h.info.dims;
     // The filter keeps its running bound in registers (K <= 32) and its query
@@ -192,65 +271,27 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
         return;
     }
     prepare_bf16(h, a.stream);
-    const uint32_t D16 = bf16_d16_of(D);
-    const uint32_t tiles = (n + 31) / 32;
+    const Bf16Config c = bf16_config(h, a.K, a.tile);
+    const uint32_t D16 = c.D16, tiles = c.tiles, stage = c.stage;
+    const size_t stage_lds = c.stage_lds;
+    const ScreenFn fn = c.fn, sfn = c.sfn;
     const size_t words = (size_t)tiles * D16 * 64;
     const uint4* phi = reinterpret_cast<const uint4*>(h.pts_bf16.get());
     const uint4* plo = phi + words;
-
-    ScreenFn fn = nullptr, sfn = nullptr;
-    uint32_t max_stage = 1;
-    switch (D16) {
-        case 1: fn = screen_for<1, false>(a.K); sfn = screen_for<1, true>(a.K); max_stage = bf16_stage_tiles_wide<1>(); break;
-        case 2: fn = screen_for<2, false>(a.K); sfn = screen_for<2, true>(a.K); max_stage = bf16_stage_tiles_wide<2>(); break;
-        case 4: fn = screen_for<4, false>(a.K); sfn = screen_for<4, true>(a.K); max_stage = bf16_stage_tiles_wide<4>(); break;
-        case 8: fn = screen_for<8, false>(a.K); sfn = screen_for<8, true>(a.K); max_stage = bf16_stage_tiles_wide<8>(); break;
-        default: fn = screen_for<16, false>(a.K); sfn = screen_for<16, true>(a.K); max_stage = bf16_stage_tiles_wide<16>(); break;
-    }
-    // Requested (or widest) staging, halved until it fits the device's
-    // per-workgroup LDS.
-    uint32_t stage = a.tile ? (a.tile < max_stage ? a.tile : max_stage) : max_stage;
-    while (stage > 1 && bf16_stage_bytes(D16, stage) > device_limits().max_lds_per_block) stage >>= 1;
-    const size_t stage_lds = bf16_stage_bytes(D16, stage);
-    require(stage_lds <= device_limits().max_lds_per_block, "bf16 staging exceeds the per-workgroup LDS limit",
-            KNN_ERR_UNSUPPORTED);
-    // Point chunks: enough workgroups for two full rounds of the screen's
-    // resident workgroups per CU, so no partial last round leaves CUs at a
-    // fraction of their occupancy (3 resident at D <= 64 gave a 4-workgroup
-    // CU a second round at a third of its waves).
     const uint32_t qblocks = (a.nq + 127) / 128;
-    int per_cu = 0;
-    KNN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), 256, stage_lds));
-    const uint32_t want = 2 * device_limits().cu_count * (uint32_t)(per_cu > 0 ? per_cu : 2);
-    uint32_t S_ = 1;
-    if (qblocks < want) {
-        S_ = (want + qblocks - 1) / qblocks;
-        const uint32_t max_s = tiles / 32 > 0 ? tiles / 32 : 1;
-        if (S_ > max_s) S_ = max_s;
-        if (S_ > 64) S_ = 64;
-    }
+    const Bf16Split sp = bf16_split(n, a.nq, a.K, tiles, c.want);
+    const uint32_t S_ = sp.S, lists = sp.lists, cap = sp.cap;
     const uint32_t tpc = (tiles + S_ - 1) / S_;
-    // One candidate list per (query, chunk, half-wave lane).  A lane that sees
-    // m points admits about K (1 + ln(m / K)) of them while its running bound
-    // tightens; the capacity doubles that, plus slack.
-    const uint32_t lists = 2 * S_;
-    const double m = std::max(1.0, (double)n / (2.0 * S_) / a.K);
-    uint32_t cap = (uint32_t)std::min(2048.0, 2.0 * a.K * (1.0 + std::log(m)) + 64.0);
-    cap = (cap + 15) & ~15u;
-    while (cap > 64 && (size_t)a.nq * lists * cap * 4 > ((size_t)1 << 30)) cap /= 2;
-    // scratch: partial keys [nq * lists * K] (u64) | cnt[nq * lists] |
-    // cand[nq * lists * cap] | ovf[nq] | ovf_flag[nq] | novf | seed[nq]
     const size_t partial_words = (size_t)a.nq * lists * a.K;
-    const size_t need = partial_words * 8 + ((size_t)a.nq * lists * (cap + 1) + 3 * (size_t)a.nq + 4) * 4;
-    h.q_scratch.ensure(need);
+    h.q_scratch.ensure(sp.bytes);
     uint64_t* partial = reinterpret_cast<uint64_t*>(h.q_scratch.get());
     uint32_t* cnt = reinterpret_cast<uint32_t*>(partial + partial_words);
     uint32_t* cand = cnt + (size_t)a.nq * lists;
     uint32_t* ovf = cand + (size_t)a.nq * lists * cap;
     uint32_t* ovf_flag = ovf + a.nq;
     uint32_t* novf = ovf_flag + a.nq;
-    uint32_t* seed = novf + 4;
-    launch_fill_u32(novf, 1, 0u, a.stream);
+    uint32_t* seed = novf + 4;  // novf[0..3]: count, work counter, 64-bit sum (8-byte aligned: even offset)
+    launch_fill_u32(novf, 2, 0u, a.stream);  // overflow count, fallback work counter
     launch_fill_u32(ovf_flag, a.nq, 0u, a.stream);
 
     Bf16Set S{};
@@ -296,45 +337,32 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
                        ovf_flag, ovf, novf);
     launch_bf_merge(partial, a.nq, lists, a.K, a.heap, a.oidx, a.odist, a.stream);
     KNN_CHECK_LAUNCH();
-    uint32_t n_ovf = 0;
-    KNN_HIP(hipMemcpyAsync(&n_ovf, novf, 4, hipMemcpyDeviceToHost, a.stream));
-    KNN_HIP(hipStreamSynchronize(a.stream));
+    // Overflowed queries are answered exactly on the device (their merged
+    // rows are overwritten), so the call never waits for the host.
+    if (a.heap)
+        hipLaunchKernelGGL(bf16_fallback_kernel<true>, dim3(2 * device_limits().cu_count), dim3(256), 0, a.stream, S,
+                           ovf, novf, novf + 1);
+    else
+        hipLaunchKernelGGL(bf16_fallback_kernel<false>, dim3(2 * device_limits().cu_count), dim3(256), 0, a.stream, S,
+                           ovf, novf, novf + 1);
+    KNN_CHECK_LAUNCH();
     h.stats[4] = timer.stop();
     if (a.timed) {
-        // survivor fraction: candidates per (query, point) pair
-        std::vector<uint32_t> hc((size_t)a.nq * lists);
-        KNN_HIP(hipMemcpy(hc.data(), cnt, 4ull * a.nq * lists, hipMemcpyDeviceToHost));
-        double tot = 0;
-        for (uint32_t v : hc) tot += v;
-        h.stats[1] = (float)(tot / ((double)a.nq * n));
-    }
-    h.stats[0] = (float)n_ovf;
-    if (n_ovf) {
-        // Overflowed queries: plain brute force on their rows, scattered back.
-        // The ids are copied out first: the plain path reuses the scratch.
-        DevBuf<float> qsub, dsub;
-        DevBuf<int32_t> isub;
-        DevBuf<uint32_t> ids;
-        ids.alloc(n_ovf);
-        KNN_HIP(hipMemcpyAsync(ids.get(), ovf, 4ull * n_ovf, hipMemcpyDeviceToDevice, a.stream));
-        qsub.alloc((size_t)n_ovf * D);
-        isub.alloc((size_t)n_ovf * a.K);
-        dsub.alloc((size_t)n_ovf * a.K);
-        hipLaunchKernelGGL(gather_rows_kernel, dim3((uint32_t)(((size_t)n_ovf * D + 255) / 256)), dim3(256), 0,
-                           a.stream, a.q, ids.get(), n_ovf, D, qsub.get());
-        QueryArgs b = a;
-        b.q = qsub.get();
-        b.nq = n_ovf;
-        b.oidx = isub.get();
-        b.odist = dsub.get();
-        b.timed = false;
-        const float keep = h.stats[4];
-        query_bf(h, b);
-        h.stats[4] = keep;
-        hipLaunchKernelGGL(scatter_rows_kernel, dim3((uint32_t)(((size_t)n_ovf * a.K + 255) / 256)), dim3(256), 0,
-                           a.stream, isub.get(), dsub.get(), ids.get(), n_ovf, a.K, a.oidx, a.odist);
+        // Synchronous calls, after the timed region: overflow count and the
+        // survivor fraction (candidates per (query, point) pair), summed on
+        // the device.
+        unsigned long long* tot = reinterpret_cast<unsigned long long*>(novf + 2);
+        launch_fill_u32(novf + 2, 2, 0u, a.stream);
+        const size_t m = (size_t)a.nq * lists;
+        hipLaunchKernelGGL(sum_u32_kernel, dim3((uint32_t)std::min<size_t>((m + 255) / 256, 4096)), dim3(256), 0,
+                           a.stream, cnt, m, tot);
         KNN_CHECK_LAUNCH();
+        uint32_t hv[4] = {0, 0, 0, 0};
+        KNN_HIP(hipMemcpyAsync(hv, novf, sizeof(hv), hipMemcpyDeviceToHost, a.stream));
         KNN_HIP(hipStreamSynchronize(a.stream));
+        const unsigned long long t = (unsigned long long)hv[2] | ((unsigned long long)hv[3] << 32);
+        h.stats[0] = (float)hv[0];
+        h.stats[1] = (float)((double)t / ((double)a.nq * n));
     }
 }
 

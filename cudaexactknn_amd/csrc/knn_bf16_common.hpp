@@ -411,4 +411,64 @@ __device__ __forceinline__ bool bf16_rerank_one(const Bf16Set& S, uint32_t qi) {
     return true;
 }
 
+// Exact answer for one query whose candidate lists overflowed, by one wave
+// over all of the set's points: each lane keeps the top K of every 64th
+// point, then one lane merges the 64 lists and writes the row (ascending, or
+// descending for heap tracking).  `s_keys` is the wave's 64 x 32 key slots
+// in LDS.  K <= 32.
+template <bool DESC>
+__device__ __forceinline__ void bf16_fallback_query(const Bf16Set& S, uint32_t qi, uint64_t* s_keys) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t K = S.K, D = S.D;
+    const uint64_t empty = ((uint64_t)f2u(S.maxd2) << 32) | kEmptyLo;
+    RegArray<32> buf;
+    uint64_t worst = buf.init(empty, K);
+    uint32_t whi = key_hi(worst);
+    const float* q = S.qs + (size_t)qi * D;
+    for (uint32_t j = lane; j < S.n; j += 64) {
+        const float d2 = dist2_rt(q, 1, S.pts + (size_t)j * D, D);
+        if (f2u(d2) <= whi) {
+            const uint64_t key = make_key(d2, j);
+            if (key < worst) {
+                worst = buf.consider(key);
+                whi = key_hi(worst);
+            }
+        }
+    }
+    buf.write_keys(s_keys + lane * 32, K);
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        RegArray<32, DESC> m;
+        uint64_t mw = m.init(empty, K);
+        for (uint32_t l = 0; l < 64; ++l)
+            for (uint32_t s = 0; s < K; ++s) {
+                const uint64_t key = s_keys[l * 32 + s];
+                if (key < mw) mw = m.consider(key);
+            }
+        m.write(S.oidx + (size_t)qi * K, S.odist + (size_t)qi * K, K);
+    }
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Single-dataset fallback: a fixed grid of waves pulls overflowed queries
+// (ovf[0 .. *novf)) from a device counter, so the host never needs the count
+// and the call never synchronises.  A launch with no overflow exits at once.
+template <bool DESC>
+__global__ void __launch_bounds__(256) bf16_fallback_kernel(Bf16Set S, const uint32_t* __restrict__ ovf,
+                                                            const uint32_t* __restrict__ novf,
+                                                            uint32_t* __restrict__ counter) {
+    __shared__ uint64_t s_keys[4][64 * 32];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t total = *novf;
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(counter, 1u);
+        it = __shfl(it, 0, 64);
+        if (it >= total) break;
+        bf16_fallback_query<DESC>(S, ovf[it], s_keys[wave]);
+    }
+}
+
 }  // namespace knn

@@ -34,6 +34,7 @@ const DeviceLimits& device_limits() {
 
 Handle::~Handle() {
     if (mlp) destroy_mlp(mlp);
+    if (done) (void)hipEventDestroy(done);
     magic = 0;
 }
 
@@ -88,6 +89,9 @@ void check_query(const Handle& h, uint32_t k, const knn_query_opts& o) {
     require(o.form <= KNN_FORM_LEARNED, "unknown form");
     require(!(o.form == KNN_FORM_BF16 && h.kind != KNN_KIND_BRUTE), "the bf16 form applies to brute force only");
     require(!(o.form == KNN_FORM_LEARNED && h.kind == KNN_KIND_BRUTE), "the learned form applies to trees only");
+    require(!((o.flags & KNN_FLAG_COUNT_VISITS) && o.stream),
+            "KNN_FLAG_COUNT_VISITS reads its counters back to the host: it needs a synchronous call "
+            "(opts->stream NULL)");
 }
 
 // Options -> resolved query parameters (tuned defaults for zero fields).
@@ -297,6 +301,23 @@ knn_status knn_ball_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_
     });
 }
 
+// A query enqueued on another stream than the handle's previous one waits
+// (on the device) for that query: the handle's scratch is reused by every
+// query, so its queries run one at a time in submission order.
+static void after_previous_query(Handle& h, hipStream_t s) {
+    if (h.done && h.done_stream != s) KNN_HIP(hipStreamWaitEvent(s, h.done, 0));
+}
+
+static void note_query(Handle& h, hipStream_t s) {
+    if (!h.done) KNN_HIP(hipEventCreateWithFlags(&h.done, hipEventDisableTiming));
+    KNN_HIP(hipEventRecord(h.done, s));
+    h.done_stream = s;
+}
+
+static const char* const kCaptureAlloc =
+    "knn_query under HIP graph capture would allocate device scratch: call knn_query_reserve with at least this "
+    "many queries (same k and options) before capturing";
+
 knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max_d2, int32_t* d_idx, float* d_dist,
                      uint32_t k, const knn_query_opts* opts) {
     return guarded([&] {
@@ -306,24 +327,59 @@ knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max
         check_query(h, k, o);
         if (q == 0) return;
         require(d_q && d_idx && d_dist, "query or result pointer is NULL");
+        std::lock_guard<std::mutex> lock(h.mu);
         QueryArgs a = resolve_args(h, d_q, q, max_d2, d_idx, d_dist, k, o);
-        a.timed = (opts == nullptr || opts->stream == nullptr);
-        for (float& v : h.stats) v = 0.0f;
-        hipEvent_t e0, e1;
-        KNN_HIP(hipEventCreate(&e0));
-        KNN_HIP(hipEventCreate(&e1));
-        KNN_HIP(hipEventRecord(e0, a.stream));
-        dispatch_query(h, a);
-        KNN_HIP(hipEventRecord(e1, a.stream));
-        if (opts == nullptr || opts->stream == nullptr) {
-            KNN_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            KNN_HIP(hipEventElapsedTime(&ms, e0, e1));
-            h.stats[3] = ms;
-            KNN_HIP(hipDeviceSynchronize());
+        const bool sync = o.stream == nullptr;
+        a.timed = sync;
+        if (!sync) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            KNN_HIP(hipStreamIsCapturing(a.stream, &cs));
+            a.capturing = cs != hipStreamCaptureStatusNone;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        for (float& v : h.stats) v = 0.0f;
+        // Under capture nothing may allocate or wait on work recorded
+        // outside the graph; the caller orders graph replays.
+        NoAllocScope no_alloc(a.capturing, kCaptureAlloc);
+        if (!a.capturing) after_previous_query(h, a.stream);
+        EventTimer timer(sync, a.stream);
+        dispatch_query(h, a);
+        if (sync) {
+            h.stats[3] = timer.stop();
+            KNN_HIP(hipDeviceSynchronize());
+        } else if (!a.capturing) {
+            note_query(h, a.stream);
+        }
+    });
+}
+
+knn_status knn_query_reserve(void* handle, uint32_t q, uint32_t k, const knn_query_opts* opts) {
+    return guarded([&] {
+        Handle& h = *as_handle(handle);
+        knn_query_opts o{};
+        if (opts) o = *opts;
+        o.stream = nullptr;
+        check_query(h, k, o);
+        if (q == 0) return;
+        std::lock_guard<std::mutex> lock(h.mu);
+        QueryArgs a = resolve_args(h, nullptr, q, INFINITY, nullptr, nullptr, k, o);
+        if (h.done) KNN_HIP(hipEventSynchronize(h.done));  // scratch may be replaced below
+        if (h.kind == KNN_KIND_BRUTE) {
+            if (a.form == KNN_FORM_BF16 && a.K <= 32 && h.info.dims <= 256 && h.info.n > 0)
+                bf16_reserve(h, q, a.K, a.tile, nullptr);
+            else
+                h.q_scratch.ensure(bf_scratch_bytes(h, a));
+        } else {
+            if (a.sort_queries && q > 1) {
+                h.q_keys.ensure(q);
+                h.q_vals.ensure(q);
+                h.q_keys_alt.ensure(q);
+                h.q_vals_alt.ensure(q);
+                h.q_temp.ensure(radix_temp_words(q));
+            }
+            if (a.form == KNN_FORM_LEARNED) learned_reserve(h, q);
+            if (a.count_visits) h.counters.ensure(2);
+        }
+        KNN_HIP(hipDeviceSynchronize());
     });
 }
 
@@ -335,6 +391,8 @@ knn_status knn_query_host(const void* handle, const float* h_q, uint32_t q, floa
         if (opts) o = *opts;
         require(o.stream == nullptr, "knn_query_host manages its own streams: opts->stream must be NULL");
         check_query(h, k, o);
+        std::lock_guard<std::mutex> lock(h.mu);
+        if (h.done) KNN_HIP(hipEventSynchronize(h.done));
         for (float& v : h.stats) v = 0.0f;
         if (q == 0) return;
         require(h_q && h_idx && h_dist, "query or result pointer is NULL");

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <stdexcept>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,6 +51,18 @@ knn_status guarded(F&& f) {
 }
 
 // Owning device buffer (grow-only when used as scratch).
+// Set (to the error message) while a call must not allocate device memory:
+// a query enqueued on a stream under HIP graph capture.
+inline thread_local const char* t_alloc_forbidden = nullptr;
+
+struct NoAllocScope {
+    const char* prev;
+    explicit NoAllocScope(bool on, const char* why) : prev(t_alloc_forbidden) {
+        if (on) t_alloc_forbidden = why;
+    }
+    ~NoAllocScope() { t_alloc_forbidden = prev; }
+};
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -64,6 +77,7 @@ struct DevBuf {
         n = 0;
     }
     void alloc(size_t count) {
+        require(t_alloc_forbidden == nullptr, t_alloc_forbidden ? t_alloc_forbidden : "");
         release();
         if (count == 0) count = 1;
         void* q = nullptr;
@@ -100,6 +114,12 @@ struct Handle {
     float stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     DevBuf<unsigned long long> counters;  // visit counters (KNN_FLAG_COUNT_VISITS)
     Mlp* mlp = nullptr;
+    // One query in flight per handle: host threads take `mu` for the whole
+    // call, and a query enqueued on another stream than the previous one
+    // waits on the device for `done` (recorded after each query).
+    std::mutex mu;
+    hipEvent_t done = nullptr;
+    hipStream_t done_stream = nullptr;
     ~Handle();
 };
 
@@ -181,6 +201,7 @@ struct QueryArgs {
     knn_form form;
     hipStream_t stream;
     bool timed;  // synchronous call: time kernels with events into Handle::stats
+    bool capturing = false;  // the stream is under HIP graph capture
 };
 
 // knn_capi.hip: argument checks and option resolution shared by the query
@@ -237,6 +258,10 @@ void compute_bbox(const float* d_pts, uint32_t n, uint32_t D, float* d_bbox, hip
 void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial = nullptr, size_t partial_words = 0);
 void query_bf_bf16(Handle& h, const QueryArgs& a);          // knn_bf16.hip
 void prepare_bf16(Handle& h, hipStream_t st);               // knn_bf16.hip
+// Scratch bytes the plain brute-force query needs for any row count up to a.nq.
+size_t bf_scratch_bytes(const Handle& h, const QueryArgs& a);
+// bf16 form: the bf16 copy of the points and scratch for up to nq rows.
+void bf16_reserve(Handle& h, uint32_t nq, uint32_t K, uint32_t tile, hipStream_t st);
 // Tree traversal.  d_init_r2: per-query starting radius^2 (learned form);
 // with d_miss_count the queries it leaves short are searched again inside
 // the kernel (learned_in_kernel must hold), else they are flagged in
@@ -258,6 +283,7 @@ void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& s, ui
 void learned_train(Handle& h, const float* d_sample, uint32_t n_sample, uint32_t k, uint32_t epochs, float lr,
                    float* stats, hipStream_t st);
 void query_tree_learned(Handle& h, const QueryArgs& a);
+void learned_reserve(Handle& h, uint32_t nq);  // query-time buffers for up to nq rows
 void destroy_mlp(Mlp* m);
 void mlp_gradient(const float* d_params, const float* d_x, const float* d_t, uint32_t n, float* d_grad, float* loss,
                   hipStream_t st);

@@ -316,11 +316,24 @@ __global__ void rows_kth_kernel(const float* __restrict__ dist, const int32_t* _
     full[i] = (idx[(size_t)i * K] >= 0 && idx[(size_t)i * K + K - 1] >= 0) ? 1u : 0u;
 }
 
+// dst[m][w] = src[ids[r]][w] for the m listed rows.
 __global__ void gather_rows_kernel(const float* __restrict__ src, const uint32_t* __restrict__ ids, uint32_t m,
-                                   uint32_t w, float* __restrict__ dst);
+                                   uint32_t w, float* __restrict__ dst) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)m * w) return;
+    dst[g] = src[(size_t)ids[g / w] * w + g % w];
+}
+
+// Result rows r of (si, sd) written back to rows ids[r] of (oi, od).
 __global__ void scatter_rows_kernel(const int32_t* __restrict__ si, const float* __restrict__ sd,
                                     const uint32_t* __restrict__ ids, uint32_t m, uint32_t K,
-                                    int32_t* __restrict__ oi, float* __restrict__ od);
+                                    int32_t* __restrict__ oi, float* __restrict__ od) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)m * K) return;
+    const size_t dst = (size_t)ids[g / K] * K + g % K;
+    oi[dst] = si[g];
+    od[dst] = sd[g];
+}
 
 static uint32_t prime_level_for(const knn_tree_info& I, uint32_t K) {
     uint32_t pl = 0;
@@ -579,6 +592,14 @@ void learned_train(Handle& h, const float* d_sample, uint32_t ns, uint32_t K, ui
     }
 }
 
+void learned_reserve(Handle& h, uint32_t nq) {
+    if (!h.mlp || !h.mlp->trained) return;
+    Mlp& M = *h.mlp;
+    M.feats.ensure((size_t)nq * F_IN);
+    M.ub.ensure(nq);
+    M.r2.ensure(nq);
+}
+
 void query_tree_learned(Handle& h, const QueryArgs& a) {
     require(h.mlp && h.mlp->trained, "the learned form needs knn_learned_train first");
     Mlp& M = *h.mlp;
@@ -613,6 +634,10 @@ void query_tree_learned(Handle& h, const QueryArgs& a) {
     } else {
         // Per-lane traversal (or no ordered batch): short queries are flagged
         // and re-run by the plain traversal.
+        require(!a.capturing,
+                "the learned form re-runs short queries from the host on this path (per-lane traversal, or a "
+                "dimension without the wave-cooperative kernel): it synchronises and cannot be captured",
+                KNN_ERR_UNSUPPORTED);
         DevBuf<uint32_t> flags, ids;
         flags.alloc(a.nq);
         launch_fill_u32(flags.get(), a.nq, 0u, a.stream);

@@ -15,7 +15,19 @@
  *    distances (Query::rIndexes / rDistances, src/KnnTrees.cuh:224-231).
  *  - Every call is synchronous unless knn_query_opts.stream is non-NULL, in
  *    which case work is enqueued on that hipStream_t and the call returns
- *    after enqueueing (the caller synchronises).
+ *    after enqueueing (the caller synchronises).  A stream call never waits
+ *    for the device, except: the learned form on a per-lane traversal, which
+ *    re-runs short queries from the host, and a call that must first grow
+ *    the handle's scratch (hipMalloc/hipFree synchronise; knn_query_reserve
+ *    grows it ahead of time).  KNN_FLAG_COUNT_VISITS is rejected on stream
+ *    calls; KNN_FLAG_LEARNED_STATS is ignored on them.
+ *  - One query in flight per handle: a handle's queries share its scratch,
+ *    so the library runs them one at a time in submission order -- a query
+ *    enqueued on another stream than the handle's previous query waits on
+ *    the device for that query, and host threads calling one handle are
+ *    serialised by a lock.  A query captured into a HIP graph is not ordered
+ *    this way: replay the graph only while no other query on the same handle
+ *    is in flight.
  *  - A non-zero status leaves a thread-local message in knn_last_error().
  *
  * Result semantics (identical for every structure and option):
@@ -59,7 +71,8 @@ typedef enum knn_form { KNN_FORM_PLAIN = 0, KNN_FORM_BF16 = 1, KNN_FORM_LEARNED 
 
 /* Flags for knn_query_opts.flags. */
 #define KNN_FLAG_NO_QUERY_SORT 0x1u  /* trees: keep caller order (no spatial reordering) */
-#define KNN_FLAG_COUNT_VISITS 0x2u   /* trees: count node bytes and leaf slots read (stats[5], [6]) */
+#define KNN_FLAG_COUNT_VISITS 0x2u   /* trees: count node bytes and leaf slots read (stats[5], [6]);
+                                        synchronous calls only */
 #define KNN_FLAG_NO_XCD_REMAP 0x10u  /* per-lane tree traversal: plain blockIdx order instead of
                                         one contiguous block range per XCD (L2) */
 #define KNN_FLAG_MORTON_ORDER 0x8u   /* kd: order queries by Morton code instead of tree cell
@@ -121,6 +134,16 @@ knn_status knn_ball_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_
  * opts may be NULL (defaults). */
 knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max_d2,
                      int32_t* d_idx, float* d_dist, uint32_t k, const knn_query_opts* opts);
+
+/* Workspace for stream capture (no counterpart in the reference: its
+ * batchKnn allocates nothing, src/KnnTrees.cuh:209-243).  Grows the
+ * handle's scratch -- and, for the bf16 form, its bf16 copy of the points --
+ * to what knn_query needs for any q' <= q queries with this k and these
+ * options (opts->stream is ignored), then synchronises.  After it, such a
+ * knn_query enqueued on a stream allocates nothing, so it can be captured
+ * into a HIP graph.  A knn_query under capture that would still need more
+ * scratch fails with KNN_ERR_INVALID_ARG instead of allocating. */
+knn_status knn_query_reserve(void* handle, uint32_t q, uint32_t k, const knn_query_opts* opts);
 
 /* knn_query without allocation (no counterpart in the reference: its
  * batchKnn allocates nothing, but it also sorts no queries and splits no
