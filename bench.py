@@ -295,6 +295,13 @@ def leg_batch(dev):
             times.append((_t.perf_counter() - t0) * 1e3)
         return min(times)
 
+    # Options: the batch and the same-forms sequential calls leave BLOCK_SIZE,
+    # leaf span and tile at 0, so both run the autotuner's table compiled into
+    # the library (per kd item: its (D, K, size) entry; per bf16 launch: the
+    # entry of its largest item).
+    from cudaexactknn_amd import autotune
+    tuned = sum(autotune.library_lookup("kd" if it.kind == knn.KIND_KD else "bf16", int(it.points.shape[1]), it.k,
+                                        int(it.points.shape[0])) is not None for it in items)
     same_ms = sequential(True)
     identical_same = all(_same(r, (it.r_indexes, it.r_distances)) for it, r in zip(items, seq_rows))
     plain_ms = sequential(False)
@@ -310,7 +317,8 @@ def leg_batch(dev):
            "speedup_incl_build": round(plain_ms / total, 3), "identical": bool(identical_same and identical_plain),
            "timing": "best of 3 warm passes each; result checks outside the timed passes",
            "overflow_queries": best["overflow_queries"], "work_items": best["work_items"],
-           "build_graph_nodes": best["build_graph_nodes"]}
+           "build_graph_nodes": best["build_graph_nodes"],
+           "autotuned": tuned > 0, "tuned_items": int(tuned), "items": len(items)}
     log(f"batch: build {best['build_ms']:.1f} ms + query {best['query_ms']:.1f} ms vs sequential same forms "
         f"{same_ms:.1f} ms ({out['batching_speedup_same_forms']}x) and plain {plain_ms:.1f} ms "
         f"({out['speedup_incl_build']}x), identical {out['identical']}")

@@ -34,7 +34,7 @@ EXPORTED = (
     "knn_last_build_ms", "knn_learned_train", "knn_last_query_stats", "knn_mlp_param_count", "knn_mlp_forward",
     "knn_mlp_gradient", "knn_batch_create", "knn_batch_run", "knn_batch_stats", "knn_batch_destroy",
     "knn_tree_serialized_size", "knn_tree_serialize", "knn_tree_deserialize", "knn_query_host",
-    "knn_query_noalloc", "knn_query_reserve",
+    "knn_query_noalloc", "knn_query_reserve", "knn_query_resolved_opts", "knn_tuned_lookup",
 )
 
 
@@ -99,6 +99,8 @@ def load() -> ctypes.CDLL:
     L.knn_query_host.argtypes = [P, P, u32, f32, P, P, u32, ctypes.POINTER(QueryOpts), u32]
     L.knn_query_noalloc.argtypes = [P, P, u32, f32, P, P, u32, ctypes.POINTER(QueryOpts)]
     L.knn_query_reserve.argtypes = [P, u32, u32, ctypes.POINTER(QueryOpts)]
+    L.knn_query_resolved_opts.argtypes = [P, u32, u32, ctypes.POINTER(QueryOpts), ctypes.POINTER(QueryOpts)]
+    L.knn_tuned_lookup.argtypes = [u32, u32, u32, u32, ctypes.POINTER(ctypes.c_int32)]
     L.knn_destroy.argtypes = [P]
     L.knn_last_error.restype = ctypes.c_char_p
     L.knn_last_error.argtypes = []
@@ -125,7 +127,7 @@ def load() -> ctypes.CDLL:
                  "knn_learned_train", "knn_last_query_stats", "knn_mlp_param_count", "knn_mlp_forward",
                  "knn_mlp_gradient", "knn_batch_create", "knn_batch_run", "knn_batch_stats", "knn_batch_destroy",
                  "knn_tree_serialized_size", "knn_tree_serialize", "knn_tree_deserialize", "knn_query_host",
-                 "knn_query_noalloc", "knn_query_reserve"):
+                 "knn_query_noalloc", "knn_query_reserve", "knn_query_resolved_opts", "knn_tuned_lookup"):
         getattr(L, name).restype = st
     _LIB = L
     return L

@@ -1,94 +1,111 @@
 """Autotuner for batchKnn launch options.
 
 For one structure and a representative query set it times every candidate
-option set (BLOCK_SIZE, K-buffer kind = USE_HEAP_TRACKING x
-USE_TRACKING_SHMEM, USE_STACK_SHMEM, traversal shape, leaf span, bf16 tile
-staging, form) with the library's hipEvent timing, checks that every
-candidate returns the same result (heap-mode rows compared in reverse, as
-they are emitted descending), and keeps the fastest.  Winners are stored in a
-JSON table keyed by (kind, D, K, size bucket = ceil(log2 n)) so later calls
-look them up instead of re-tuning.
+set of result-neutral launch options -- BLOCK_SIZE, the leaf span of the
+wave-cooperative tree traversal (relative to the library's default), the
+bf16 form's tile staging -- with the library's hipEvent timing, checks that
+every candidate returns the same rows, and keeps the fastest.  The caller's
+semantic options (USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM, the
+form) are held fixed.  Winners are stored in a JSON table keyed by
+(class, D, K, size bucket = ceil(log2 n)), class one of "brute" (plain form),
+"bf16" (bf16 form), "kd", "ball".
 
-Options never change results: the tuner raises if two candidates disagree.
+The library compiles the committed table (tuned_options.json next to this
+file) into knn_query's and knn_batch_create's defaults: a zero BLOCK_SIZE,
+leaf span or tile takes the table's choice for the nearest tuned size within
+one bucket.  `python -m cudaexactknn_amd.autotune --sweep` re-measures the
+table on the workload shapes the library is benchmarked on.
 """
 from __future__ import annotations
 
+import argparse
+import ctypes
 import itertools
 import json
 import math
 import os
+import sys
+import time
 from typing import Any, Iterable
 
 from . import FORM_BF16, FORM_PLAIN, BallTreeGpu, BruteForceGpu, KdTreeGpu, Query, KnnError
 from . import _lib
 
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_options.json")
+CLASSES = {"brute": 0, "kd": 1, "ball": 2, "bf16": 3}
+# Built-in defaults of the library (knn_capi.hip default_block): what a query
+# runs with when the table has no entry.
+BUILTIN_BLOCK = {"brute": 256, "bf16": 256, "kd": 64, "ball": 64}
 
 
-def _kind_name(struct) -> str:
+def class_of(struct, form: int = FORM_PLAIN) -> str:
     if isinstance(struct, KdTreeGpu):
         return "kd"
     if isinstance(struct, BallTreeGpu):
         return "ball"
     if isinstance(struct, BruteForceGpu):
-        return "brute"
+        return "bf16" if form == FORM_BF16 else "brute"
     raise TypeError(f"not a knn structure: {type(struct)!r}")
 
 
-def table_key(kind: str, dims: int, k: int, n: int) -> str:
-    return f"{kind}:D{dims}:K{k}:N2^{max(0, math.ceil(math.log2(max(1, n))))}"
+def table_key(cls: str, dims: int, k: int, n: int) -> str:
+    return f"{cls}:D{dims}:K{k}:N2^{max(0, math.ceil(math.log2(max(1, n))))}"
 
 
-def candidates(kind: str, dims: int, k: int) -> list[dict]:
-    """Candidate option sets for a structure kind (all are exact)."""
-    out = []
-    if kind == "brute":
-        for bs, heap, lds in itertools.product((64, 128, 256), (False, True), (False, True)):
-            out.append(dict(BLOCK_SIZE=bs, USE_HEAP_TRACKING=heap, USE_TRACKING_SHMEM=lds, form=FORM_PLAIN))
-        if k <= 32 and dims <= 256:
-            for heap, tile in itertools.product((False, True), (0, 1, 2, 4, 8, 16, 32)):
-                out.append(dict(BLOCK_SIZE=256, USE_HEAP_TRACKING=heap, USE_TRACKING_SHMEM=False, form=FORM_BF16,
-                                tile=tile))
-        return out
-    for bs, heap, span in itertools.product((64, 128, 256), (False, True), (0, -1, 1)):
-        out.append(dict(BLOCK_SIZE=bs, USE_HEAP_TRACKING=heap, USE_TRACKING_SHMEM=False, USE_STACK_SHMEM=False,
-                        per_lane=False, leaf_span_delta=span))
-    for bs, heap, lds, stack in itertools.product((64, 128, 256), (False, True), (False, True), (False, True)):
-        out.append(dict(BLOCK_SIZE=bs, USE_HEAP_TRACKING=heap, USE_TRACKING_SHMEM=lds, USE_STACK_SHMEM=stack,
-                        per_lane=True, leaf_span_delta=0))
-    return out
+def candidates(cls: str, dims: int, k: int) -> list[dict]:
+    """Result-neutral option sets for a class; the first is the built-in default."""
+    if cls == "brute":
+        return [dict(BLOCK_SIZE=b) for b in (256, 64, 128, 512, 1024)]
+    if cls == "bf16":
+        return [dict(tile=t, form=FORM_BF16) for t in (0, 1, 2, 4)]
+    return [dict(BLOCK_SIZE=b, leaf_span_delta=s) for b, s in itertools.product((64, 128, 256), (0, -1, 1, 2, 3))]
 
 
-def _default_span(kind: str, info: dict, k: int, per_lane: bool) -> int:
-    """The library's default leaf span (knn_capi.hip default_span)."""
-    target = k // 2 if (kind == "ball" and info["dims"] >= 8 and k >= 2) else k
+def _default_span(cls: str, info: dict, k: int) -> int:
+    """The library's default leaf span of the wave-cooperative traversal
+    (knn_internal.hpp leaf_span_default)."""
+    target = k
+    if cls == "ball" and info["dims"] >= 8 and k >= 2:
+        target = k // 2
+    if cls == "kd":
+        target = max(2 * k, 32)
     mc = max(1, info["max_children"])
-    s = 0 if target <= mc else math.ceil(math.log2((target + mc - 1) // mc))
+    s = 0
+    while target > (mc << s):
+        s += 1
     return min(s, info["node_depth"])
 
 
-def _resolve(kind: str, struct, k: int, cand: dict) -> dict:
-    """Candidate -> batchKnn keyword arguments (leaf span made absolute)."""
+def _resolve(cls: str, struct, k: int, cand: dict) -> dict:
+    """Candidate -> explicit batchKnn keyword arguments (no field left for the
+    table to fill: the leaf span made absolute, the tile given)."""
     opts = {key: v for key, v in cand.items() if key != "leaf_span_delta"}
-    if kind != "brute":
-        base = _default_span(kind, struct.info(), k, cand.get("per_lane", False))
-        span = base + cand.get("leaf_span_delta", 0)
-        span = max(0, min(span, struct.info()["node_depth"]))
+    if cls in ("kd", "ball"):
+        info = struct.info()
+        span = _default_span(cls, info, k) + cand.get("leaf_span_delta", 0)
+        span = max(0, min(span, info["node_depth"]))
         opts["leaf_span"] = 0xFFFFFFFF if span == 0 else span
+    if cls == "bf16" and not opts.get("tile"):
+        opts["tile"] = 8  # the widest staging (clamped to what fits)
+    if "BLOCK_SIZE" not in opts:
+        opts["BLOCK_SIZE"] = BUILTIN_BLOCK[cls]
     return opts
 
 
-def tune(struct, queries: Any, k: int, max_distance2: float = float("inf"),
-         cands: Iterable[dict] | None = None, repeats: int = 3) -> dict:
-    """Time every candidate on `queries` (a device tensor [q][D]) and return
-    {"best": options, "ms": best time, "results": [(options, ms), ...]}.
-    Raises KnnError if two candidates return different results."""
+def tune(struct, queries: Any, k: int, max_distance2: float = float("inf"), form: int = FORM_PLAIN,
+         base: dict | None = None, cands: Iterable[dict] | None = None, repeats: int = 3) -> dict:
+    """Time every candidate on `queries` (a device tensor [q][D]) with the
+    caller's semantic options `base` held fixed and return {"best": options,
+    "ms": best time, "default_ms": time of the built-in defaults, "results":
+    [(options, ms), ...]}.  Raises KnnError if two candidates return
+    different rows."""
     import torch
 
-    kind = _kind_name(struct)
+    cls = class_of(struct, form)
     dims = int(queries.shape[1])
     nq = int(queries.shape[0])
-    cands = list(cands) if cands is not None else candidates(kind, dims, k)
+    cands = list(cands) if cands is not None else candidates(cls, dims, k)
+    base = dict(base or {})
     dev = queries.device
     ri = torch.empty((nq, k), dtype=torch.int32, device=dev)
     rd = torch.empty((nq, k), dtype=torch.float32, device=dev)
@@ -96,7 +113,7 @@ def tune(struct, queries: Any, k: int, max_distance2: float = float("inf"),
     ref = None
     results = []
     for cand in cands:
-        opts = _resolve(kind, struct, k, cand)
+        opts = {**base, **_resolve(cls, struct, k, cand)}
         try:
             struct.batchKnn(q, k, **opts)
         except KnnError as e:
@@ -104,12 +121,10 @@ def tune(struct, queries: Any, k: int, max_distance2: float = float("inf"),
                 continue  # e.g. an LDS request above the workgroup limit
             raise
         torch.cuda.synchronize()
-        got_i, got_d = ri.clone(), rd.clone()
-        if opts.get("USE_HEAP_TRACKING"):
-            got_i, got_d = got_i.flip(1), got_d.flip(1)
+        got = (ri.clone(), rd.clone())
         if ref is None:
-            ref = (got_i, got_d)
-        elif not (torch.equal(ref[0], got_i) and torch.equal(ref[1].view(torch.int32), got_d.view(torch.int32))):
+            ref = got
+        elif not (torch.equal(ref[0], got[0]) and torch.equal(ref[1].view(torch.int32), got[1].view(torch.int32))):
             raise KnnError(_lib.KNN_ERR_INTERNAL, f"option set {cand} changed the result")
         times = []
         for _ in range(repeats):
@@ -119,7 +134,7 @@ def tune(struct, queries: Any, k: int, max_distance2: float = float("inf"),
     if not results:
         raise KnnError(_lib.KNN_ERR_UNSUPPORTED, "no candidate option set ran")
     best = min(results, key=lambda r: r[1])
-    return {"best": best[0], "ms": best[1], "results": results}
+    return {"best": best[0], "ms": best[1], "default_ms": results[0][1], "results": results}
 
 
 def load_table(path: str = DEFAULT_TABLE) -> dict:
@@ -136,22 +151,33 @@ def save_table(table: dict, path: str = DEFAULT_TABLE) -> None:
     os.replace(tmp, path)
 
 
-def tuned_options(struct, k: int, table: dict | None = None, path: str = DEFAULT_TABLE) -> dict:
-    """batchKnn keyword arguments for `struct` from the table ({} = library
-    defaults when the bucket was never tuned)."""
-    kind = _kind_name(struct)
+def tuned_options(struct, k: int, form: int = FORM_PLAIN, table: dict | None = None,
+                  path: str = DEFAULT_TABLE) -> dict:
+    """Explicit batchKnn keyword arguments for `struct` from a table ({} when
+    the bucket was never tuned)."""
+    cls = class_of(struct, form)
     table = load_table(path) if table is None else table
-    entry = table.get(table_key(kind, struct.dims, k, struct.size))
-    return _resolve(kind, struct, k, entry["best"]) if entry else {}
+    entry = table.get(table_key(cls, struct.dims, k, struct.size))
+    return _resolve(cls, struct, k, entry["best"]) if entry else {}
 
 
-def tune_and_store(struct, queries: Any, k: int, max_distance2: float = float("inf"),
+def library_lookup(cls: str, dims: int, k: int, n: int) -> dict | None:
+    """The entry compiled into the library for (class, D, K, n), or None."""
+    out = (ctypes.c_int32 * 4)()
+    _lib.check(_lib.load().knn_tuned_lookup(CLASSES[cls], dims, k, n, out))
+    if not out[0]:
+        return None
+    return {"BLOCK_SIZE": out[1], "leaf_span_delta": out[2], "tile": out[3]}
+
+
+def tune_and_store(struct, queries: Any, k: int, max_distance2: float = float("inf"), form: int = FORM_PLAIN,
                    path: str = DEFAULT_TABLE, **kw) -> dict:
-    """Tune and record the winner under (kind, D, K, size bucket)."""
-    r = tune(struct, queries, k, max_distance2, **kw)
+    """Tune and record the winner under (class, D, K, size bucket)."""
+    r = tune(struct, queries, k, max_distance2, form=form, **kw)
     table = load_table(path)
-    table[table_key(_kind_name(struct), struct.dims, k, struct.size)] = {
-        "best": r["best"], "ms": r["ms"], "queries": int(queries.shape[0]), "candidates": len(r["results"])}
+    table[table_key(class_of(struct, form), struct.dims, k, struct.size)] = {
+        "best": r["best"], "ms": round(r["ms"], 4), "default_ms": round(r["default_ms"], 4),
+        "queries": int(queries.shape[0]), "candidates": len(r["results"])}
     save_table(table, path)
     return r
 
@@ -186,3 +212,86 @@ def tune_batch(items: list, cands: Iterable[dict] | None = None, repeats: int = 
         b.close()
     best = min(results, key=lambda r: r[1])
     return {"best": best[0], "ms": best[1], "results": results}
+
+
+def sweep_shapes() -> list[dict]:
+    """The shapes the library is benchmarked on: the batched workload's
+    (D, K, size) grid (uniform points, queries = size / 4) and the
+    single-structure workloads at their own sizes and data."""
+    shapes = []
+    for b in range(10, 21):
+        for k in (4, 8, 16):
+            shapes.append(dict(cls="kd", dims=3, k=k, n=1 << b, nq=(1 << b) // 4, data="uniform", seed=4))
+            shapes.append(dict(cls="bf16", dims=16, k=k, n=1 << b, nq=(1 << b) // 4, data="uniform", seed=4))
+    shapes += [
+        dict(cls="brute", dims=3, k=8, n=65536, nq=16384, data="uniform", seed=0, heap=True),
+        dict(cls="kd", dims=3, k=8, n=65536, nq=16384, data="uniform", seed=0, heap=True),
+        dict(cls="kd", dims=3, k=16, n=1 << 24, nq=1 << 22, data="gaussian3", seed=1),
+        dict(cls="ball", dims=3, k=16, n=1 << 24, nq=1 << 22, data="gaussian3", seed=1),
+        dict(cls="kd", dims=8, k=32, n=1 << 22, nq=1 << 20, data="aniso8", seed=3, repeats=2),
+        dict(cls="ball", dims=8, k=32, n=1 << 22, nq=1 << 20, data="aniso8", seed=3, repeats=2),
+    ]
+    for d in (32, 64, 128):
+        shapes.append(dict(cls="bf16", dims=d, k=32, n=1 << 20, nq=1 << 16, data="scaled", seed=2, repeats=2))
+    return shapes
+
+
+def _data(shape: dict):
+    from . import datasets
+    n, nq, d, seed = shape["n"], shape["nq"], shape["dims"], shape["seed"]
+    if shape["data"] == "gaussian3":
+        return datasets.gaussian_mixture_3d(n, nq, seed=seed)
+    if shape["data"] == "aniso8":
+        return datasets.anisotropic_mixture_8d(n, nq, seed=seed)
+    if shape["data"] == "scaled":
+        return datasets.scaled_normal(n, nq, d, seed=seed)
+    return datasets.uniform_cube(n, nq, d, seed=seed)
+
+
+def sweep(path: str = DEFAULT_TABLE, only: str | None = None) -> dict:
+    """Tune every sweep shape and store the winners in `path`."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    table = load_table(path)
+    for shape in sweep_shapes():
+        if only and shape["cls"] != only:
+            continue
+        t0 = time.time()
+        pts, qs = _data(shape)
+        p = torch.from_numpy(pts).to(dev)
+        q = torch.from_numpy(qs).to(dev)
+        del pts, qs
+        cls = shape["cls"]
+        s = KdTreeGpu(p, depth_skips=0) if cls == "kd" else BallTreeGpu(p, depth_skips=1) if cls == "ball" \
+            else BruteForceGpu(p)
+        base = {"USE_HEAP_TRACKING": True} if shape.get("heap") else {}
+        form = FORM_BF16 if cls == "bf16" else FORM_PLAIN
+        r = tune(s, q, shape["k"], form=form, base=base, repeats=shape.get("repeats", 3))
+        key = table_key(cls, shape["dims"], shape["k"], shape["n"])
+        table[key] = {"best": r["best"], "ms": round(r["ms"], 4), "default_ms": round(r["default_ms"], 4),
+                      "queries": shape["nq"], "candidates": len(r["results"])}
+        save_table(table, path)
+        print(f"[tune] {key}: best {r['best']} {r['ms']:.4f} ms (built-in default {r['default_ms']:.4f} ms), "
+              f"{time.time() - t0:.1f} s", flush=True)
+        s.close()
+        del p, q
+        torch.cuda.empty_cache()
+    return table
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--sweep", action="store_true", help="re-measure the table on the benchmark shapes")
+    ap.add_argument("--only", default=None, help="restrict the sweep to one class (brute, bf16, kd, ball)")
+    ap.add_argument("--out", default=DEFAULT_TABLE)
+    a = ap.parse_args(argv)
+    if not a.sweep:
+        ap.print_help()
+        return 0
+    sweep(a.out, a.only)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -31,8 +31,41 @@ COMMON = [
 SOURCES = [
     "knn_prims.hip", "knn_build.hip", "knn_bf.hip", "knn_bf16.hip", "knn_kd_query.hip",
     "knn_ball_query.hip", "knn_kd_packet.hip", "knn_ball_packet.hip", "knn_tree_host.hip", "knn_learned.hip", "knn_capi.hip",
-    "knn_batch.hip", "knn_serialize.hip", "knn_noalloc.hip",
+    "knn_batch.hip", "knn_serialize.hip", "knn_noalloc.hip", "knn_tuned.hip",
 ]
+
+TUNED_JSON = os.path.join(HERE, "tuned_options.json")
+TUNED_INC = os.path.join(CSRC, "knn_tuned_table.inc")
+_CLASSES = {"brute": 0, "kd": 1, "ball": 2, "bf16": 3}
+
+
+def gen_tuned_table(src: str = TUNED_JSON, dst: str = TUNED_INC) -> int:
+    """Writes the C++ form of the autotuner's table (one TunedEntry per key
+    "kind:D<d>:K<k>:N2^<b>"); returns the entry count."""
+    import json
+    table = {}
+    if os.path.exists(src):
+        with open(src) as f:
+            table = json.load(f)
+    rows = []
+    for key in sorted(table):
+        kind, d, k, nb = key.split(":")
+        best = table[key]["best"]
+        block = int(best.get("BLOCK_SIZE", 0) or 0)
+        delta = int(best.get("leaf_span_delta", 0) or 0)
+        tile = int(best.get("tile", 0) or 0)
+        rows.append(f"    {{{_CLASSES[kind]}, {int(nb[3:])}, {int(d[1:])}, {int(k[1:])}, {block}, {delta}, {tile}}},"
+                    f"  // {key}: {table[key].get('ms', 0):.4g} ms")
+    body = "\n".join(rows) if rows else "    {0, 0, 0, 0, 0, 0, 0},  // no tuned entries"
+    text = ("// Generated from tuned_options.json by cudaexactknn_amd/build.py; do not edit.\n"
+            "// {cls, log2n, dims, k, block, span_delta, tile}\n"
+            f"const TunedEntry kTunedTable[] = {{\n{body}\n}};\n"
+            f"const size_t kTunedCount = {len(rows)};\n")
+    old = open(dst).read() if os.path.exists(dst) else None
+    if old != text:
+        with open(dst, "w") as f:
+            f.write(text)
+    return len(rows)
 
 
 def _compile(src: str) -> str:
@@ -61,6 +94,7 @@ def build_cpp_test() -> str:
 
 def build(jobs: int | None = None, verbose: bool = True) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
+    gen_tuned_table()
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4, 8)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))

@@ -301,6 +301,7 @@ struct Launch {
     uint32_t grid;
     size_t lds;
     uint32_t stack_words;
+    uint32_t tile;  // bf16 tiles staged per barrier (0 = widest)
 };
 
 struct BfSetBufs {
@@ -554,9 +555,12 @@ static void plan(Batch& b) {
         A.sd_inv = (65536u + I.split_dims - 1) / I.split_dims;
         uint32_t span = b.opts.leaf_span;
         if (span == 0) {
-            const uint64_t target = it.k;
-            const uint64_t mc = I.max_children;
-            span = target > mc ? ceil_log2_u((target + mc - 1) / mc) : 0;
+            // The library's default groups, moved by the autotuner's choice
+            // for this (structure, D, K, size).
+            int s = (int)leaf_span_default(h.kind, I.dims, it.k, I.max_children, I.node_depth, false);
+            if (const TunedEntry* te = tuned_lookup(h.kind == KNN_KIND_BALL ? TUNE_BALL : TUNE_KD, I.dims, it.k, I.n))
+                s += te->span_delta;
+            span = (uint32_t)std::max(0, s);
         } else if (span == 0xFFFFFFFFu) {
             span = 0;
         }
@@ -613,11 +617,22 @@ static void plan(Batch& b) {
         L.nitems = (uint32_t)all_items.size() - L.item0;
         if (L.nitems == 0) return;
         L.kmax = 1;
+        uint64_t heaviest = 0;  // bf16 set with the most (query block, tile) pairs picks the staging
         for (uint32_t j = L.item0; j < L.item0 + L.nitems; ++j) {
             const uint4 w = all_items[j];
             const uint32_t set = w.x & 0x0FFFFFFFu;
-            const uint32_t k = (w.x >> 28) == ITEM_BF16 ? bsets[set].K : tsets[set].K;
+            const bool bf = (w.x >> 28) == ITEM_BF16;
+            const uint32_t k = bf ? bsets[set].K : tsets[set].K;
             L.kmax = std::max(L.kmax, k);
+            if (bf && w.y == 0) {
+                const Bf16Set& S = bsets[set];
+                const uint64_t work = (uint64_t)S.tiles * ((S.nq + 127) / 128);
+                if (work > heaviest) {
+                    heaviest = work;
+                    const TunedEntry* te = tuned_lookup(TUNE_BF16, S.D, S.K, S.n);
+                    L.tile = te ? te->tile : 0u;
+                }
+            }
         }
         L.stack_words = DT ? stack_words : 0;
         L.lds = std::max<size_t>((size_t)4 * L.stack_words * 4, bf16_lds_of(D16));
@@ -741,7 +756,8 @@ static void plan(Batch& b) {
         for (uint32_t l = 0; l < nl; ++l) {
             const Launch& L = b.launches[l];
             const uint32_t stage_max = L.D16 ? bf16_stage_max(L.D16) : 1;
-            const uint32_t stage = b.opts.tile ? std::min(b.opts.tile, stage_max) : stage_max;
+            const uint32_t want = b.opts.tile ? b.opts.tile : L.tile;
+            const uint32_t stage = want ? std::min(want, stage_max) : stage_max;
             hipLaunchKernelGGL(batch_fn(L.DT, L.D16, L.kmax), dim3(L.grid), dim3(256), L.lds, st, b.d_tsets.get(),
                                b.d_bsets.get(), b.d_items.get() + L.item0, L.nitems, b.ctr.get() + l,
                                L.stack_words, stage);

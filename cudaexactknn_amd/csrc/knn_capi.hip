@@ -55,32 +55,17 @@ static void check_pts(const float* d_pts, uint32_t n, uint32_t dims) {
 
 static hipStream_t null_stream() { return nullptr; }
 
-// Tuned defaults (BLOCK_SIZE, leaf span, query order) when the caller passes 0,
-// measured on the 16.7M-point / 4.2M-query 3-D mixture with K = 16:
+// Built-in defaults (BLOCK_SIZE, leaf span, query order) when the caller
+// passes 0 and the tuned table has no entry, measured on the 16.7M-point /
+// 4.2M-query 3-D mixture with K = 16:
 //  - trees: one wave per workgroup (the wave-cooperative traversal keeps its
 //    stack per wave; 64 measured fastest);
-//  - kd: leaf groups of ~K points, queries in tree-cell order;
-//  - ball: leaf groups of ~K points, queries in Morton order;
-//  - per-lane traversal: groups of ~K points.
+//  - kd: queries in tree-cell order; ball: queries in Morton order;
+//  - leaf spans: leaf_span_default.
 static uint32_t default_block(const Handle& h) { return h.kind == KNN_KIND_BRUTE ? 256 : 64; }
-static uint32_t ceil_log2(uint64_t x) {
-    uint32_t r = 0;
-    while ((1ull << r) < x) ++r;
-    return r;
-}
-// Leaf groups of about K points (measured best for both trees on the
-// 16.7M-point 3-D mixture, K = 16), and of about K/2 points for ball trees
-// of D >= 8, whose slot tests cost more per node visit saved (4.2M-point
-// 8-D mixture, K = 32: groups of 8 / 16 / 32 / 64 points, 165 / 164 / 181 /
-// 211 ms).
 static uint32_t default_span(const Handle& h, uint32_t K, bool per_lane) {
     if (h.kind == KNN_KIND_BRUTE || h.info.max_children == 0) return 0;
-    (void)per_lane;
-    const uint64_t target = (h.kind == KNN_KIND_BALL && h.info.dims >= 8 && K >= 2) ? K / 2 : K;
-    const uint64_t mc = h.info.max_children;
-    uint32_t s = target > mc ? ceil_log2((target + mc - 1) / mc) : 0;
-    if (s > h.info.node_depth) s = h.info.node_depth;
-    return s;
+    return leaf_span_default(h.kind, h.info.dims, K, h.info.max_children, h.info.node_depth, per_lane);
 }
 
 void check_query(const Handle& h, uint32_t k, const knn_query_opts& o) {
@@ -94,7 +79,22 @@ void check_query(const Handle& h, uint32_t k, const knn_query_opts& o) {
             "(opts->stream NULL)");
 }
 
-// Options -> resolved query parameters (tuned defaults for zero fields).
+// The autotuner's entry for this structure and K (the wave-cooperative
+// traversal for trees; the per-lane shape keeps the built-in defaults).
+static const TunedEntry* tuned_for(const Handle& h, uint32_t k, const knn_query_opts& o) {
+    int cls;
+    if (h.kind == KNN_KIND_BRUTE)
+        cls = o.form == KNN_FORM_BF16 ? TUNE_BF16 : TUNE_BRUTE;
+    else if (o.flags & KNN_FLAG_PER_LANE)
+        return nullptr;
+    else
+        cls = h.kind == KNN_KIND_KD ? TUNE_KD : TUNE_BALL;
+    return tuned_lookup(cls, h.info.dims, k, h.info.n);
+}
+
+// Options -> resolved query parameters.  Zero fields take the autotuner's
+// choice for (structure, D, K, size) when the table has one, else the
+// built-in defaults above.
 QueryArgs resolve_args(const Handle& h, const float* d_q, uint32_t q, float max_d2, int32_t* d_idx, float* d_dist,
                        uint32_t k, const knn_query_opts& o) {
     QueryArgs a{};
@@ -104,7 +104,8 @@ QueryArgs resolve_args(const Handle& h, const float* d_q, uint32_t q, float max_
     a.oidx = d_idx;
     a.odist = d_dist;
     a.K = k;
-    a.block = o.block ? o.block : default_block(h);
+    const TunedEntry* t = (o.block == 0 || o.leaf_span == 0 || o.tile == 0) ? tuned_for(h, k, o) : nullptr;
+    a.block = o.block ? o.block : (t && t->block ? t->block : default_block(h));
     require(a.block <= device_limits().max_threads, "BLOCK_SIZE above the device maximum");
     a.heap = o.heap != 0;
     a.track_lds = o.track_lds != 0;
@@ -115,9 +116,15 @@ QueryArgs resolve_args(const Handle& h, const float* d_q, uint32_t q, float max_
     a.xcd_remap = (o.flags & KNN_FLAG_NO_XCD_REMAP) == 0;
     a.learned_stats = (o.flags & KNN_FLAG_LEARNED_STATS) != 0;
     a.order_morton = (o.flags & KNN_FLAG_MORTON_ORDER) != 0 || h.kind == KNN_KIND_BALL;
-    a.leaf_span = o.leaf_span == 0 ? default_span(h, k, a.per_lane) : (o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span);
+    if (o.leaf_span == 0) {
+        int span = (int)default_span(h, k, a.per_lane) + (t && h.kind != KNN_KIND_BRUTE ? t->span_delta : 0);
+        span = std::max(0, std::min(span, (int)h.info.node_depth));
+        a.leaf_span = (uint32_t)span;
+    } else {
+        a.leaf_span = o.leaf_span == 0xFFFFFFFFu ? 0u : o.leaf_span;
+    }
     a.form = (knn_form)o.form;
-    a.tile = o.tile;
+    a.tile = o.tile ? o.tile : (t ? t->tile : 0u);
     a.stream = (hipStream_t)o.stream;
     return a;
 }
@@ -380,6 +387,34 @@ knn_status knn_query_reserve(void* handle, uint32_t q, uint32_t k, const knn_que
             if (a.count_visits) h.counters.ensure(2);
         }
         KNN_HIP(hipDeviceSynchronize());
+    });
+}
+
+knn_status knn_query_resolved_opts(const void* handle, uint32_t q, uint32_t k, const knn_query_opts* in,
+                                   knn_query_opts* out) {
+    return guarded([&] {
+        const Handle& h = *as_handle(handle);
+        require(out != nullptr, "out is NULL");
+        knn_query_opts o{};
+        if (in) o = *in;
+        check_query(h, k, o);
+        const QueryArgs a = resolve_args(h, nullptr, q, INFINITY, nullptr, nullptr, k, o);
+        *out = o;
+        out->block = a.block;
+        out->leaf_span = h.kind == KNN_KIND_BRUTE ? 0u : (a.leaf_span == 0 ? 0xFFFFFFFFu : a.leaf_span);
+        out->tile = a.tile;
+    });
+}
+
+knn_status knn_tuned_lookup(uint32_t cls, uint32_t dims, uint32_t k, uint32_t n, int32_t* out4) {
+    return guarded([&] {
+        require(out4 != nullptr, "out4 is NULL");
+        require(cls <= TUNE_BF16, "unknown tuning class");
+        const TunedEntry* t = tuned_lookup((int)cls, dims, k, n);
+        out4[0] = t ? 1 : 0;
+        out4[1] = t ? t->block : 0;
+        out4[2] = t ? t->span_delta : 0;
+        out4[3] = t ? t->tile : 0;
     });
 }
 

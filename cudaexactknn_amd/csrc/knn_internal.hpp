@@ -2,6 +2,8 @@
 // and the launch-facing declarations shared by the .hip translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include <stdexcept>
@@ -203,6 +205,44 @@ struct QueryArgs {
     bool timed;  // synchronous call: time kernels with events into Handle::stats
     bool capturing = false;  // the stream is under HIP graph capture
 };
+
+// Default leaf span (leaf levels scanned per visit: groups of 2^span
+// buckets) of a tree query.  Measured on MI355X:
+//  - kd, wave-cooperative traversal: groups of max(2K, 32) points (3-D
+//    uniform points of 2^10..2^20 with K = 4, 8, 16, the 16.7M-point 3-D
+//    mixture with K = 16 and the 4.2M-point 8-D mixture with K = 32 are all
+//    fastest there: 32, 32, 32, 32 and 64 points);
+//  - ball: groups of about K points, K/2 at D >= 8, whose slot tests cost
+//    more per node visit saved (8-D mixture, K = 32: groups of 8 / 16 / 32 /
+//    64 points, 165 / 164 / 181 / 211 ms);
+//  - per-lane traversal: groups of about K points.
+inline uint32_t leaf_span_default(knn_kind kind, uint32_t dims, uint32_t K, uint32_t max_children,
+                                  uint32_t node_depth, bool per_lane) {
+    uint64_t target = K;
+    if (kind == KNN_KIND_BALL && dims >= 8 && K >= 2) target = K / 2;
+    if (kind == KNN_KIND_KD && !per_lane) target = std::max<uint64_t>(2ull * K, 32);
+    const uint64_t mc = max_children ? max_children : 1;
+    uint32_t s = 0;
+    while (target > (mc << s)) ++s;
+    return s > node_depth ? node_depth : s;
+}
+
+// Tuned launch options (knn_tuned.hip).  Classes: plain brute force, kd
+// tree, ball tree, bf16-form brute force.
+enum TunedClass { TUNE_BRUTE = 0, TUNE_KD = 1, TUNE_BALL = 2, TUNE_BF16 = 3 };
+struct TunedEntry {
+    uint8_t cls;
+    uint8_t log2n;       // size bucket: ceil(log2 n)
+    uint16_t dims;
+    uint16_t k;
+    uint16_t block;      // BLOCK_SIZE (0 = library default)
+    int8_t span_delta;   // trees: leaf levels relative to the default span
+    uint8_t tile;        // bf16: 32-point tiles staged per barrier (0 = widest)
+};
+// The entry for (cls, dims, k) at the nearest tuned size bucket within one
+// of ceil(log2 n), or nullptr.
+const TunedEntry* tuned_lookup(int cls, uint32_t dims, uint32_t k, uint32_t n);
+size_t tuned_count();
 
 // knn_capi.hip: argument checks and option resolution shared by the query
 // entry points.

@@ -145,6 +145,22 @@ knn_status knn_query(const void* handle, const float* d_q, uint32_t q, float max
  * scratch fails with KNN_ERR_INVALID_ARG instead of allocating. */
 knn_status knn_query_reserve(void* handle, uint32_t q, uint32_t k, const knn_query_opts* opts);
 
+/* The options a knn_query with these arguments would run with (no
+ * counterpart in the reference, whose options are template arguments):
+ * *out = *in with block, leaf_span (absolute; 0xFFFFFFFF = the tree's own
+ * buckets) and tile resolved.  Fields left 0 take the autotuner's choice
+ * for (structure, D, k, size bucket) compiled into the library, else the
+ * built-in defaults. */
+knn_status knn_query_resolved_opts(const void* handle, uint32_t q, uint32_t k, const knn_query_opts* in,
+                                   knn_query_opts* out);
+
+/* The compiled-in autotuner table (cudaexactknn_amd/tuned_options.json):
+ * cls 0 = brute force (plain form), 1 = kd tree, 2 = ball tree, 3 = brute
+ * force (bf16 form).  out4 = {found, block, leaf-span delta from the default
+ * span, bf16 tile}; the entry is the nearest tuned size bucket
+ * (ceil(log2 n)) within one.  Host-only: needs no GPU. */
+knn_status knn_tuned_lookup(uint32_t cls, uint32_t dims, uint32_t k, uint32_t n, int32_t* out4);
+
 /* knn_query without allocation (no counterpart in the reference: its
  * batchKnn allocates nothing, but it also sorts no queries and splits no
  * scans).  For callers that cannot allocate device memory: code under HIP
@@ -276,8 +292,10 @@ typedef struct knn_batch_item {
 typedef struct knn_batch_opts {
     uint32_t build_streams;  /* parallel graph branches for the builds (0 = 4) */
     uint32_t blocks_per_cu;  /* persistent workgroups per CU (0 = occupancy limit) */
-    uint32_t tile;           /* bf16 items: 32-point tiles staged per barrier (0 = 8 / D16) */
-    uint32_t leaf_span;      /* tree items: leaf levels per scan (0 = default per item) */
+    uint32_t tile;           /* bf16 items: 32-point tiles staged per barrier (0 = the tuned
+                                choice of the launch's largest bf16 item, else the widest) */
+    uint32_t leaf_span;      /* tree items: leaf levels per scan (0 = tuned per item, else
+                                groups of about K points) */
 } knn_batch_opts;
 
 /* Statistics of the last synchronous knn_batch_run: [0] build ms, [1] query

@@ -125,3 +125,25 @@ def test_python_layer_rejects_wrong_dtypes_and_sizes():
                        torch.zeros((2, 4)))
     with pytest.raises(knn.KnnError, match="int32"):
         knn.Batch([it])
+
+
+def test_tuned_table_is_compiled_into_the_library(tmp_path):
+    """The library was built from the committed autotuner table, and
+    knn_tuned_lookup returns every committed entry (a host-only call)."""
+    from cudaexactknn_amd import autotune
+    from cudaexactknn_amd import build as B
+    inc = tmp_path / "table.inc"
+    n = B.gen_tuned_table(dst=str(inc))
+    assert inc.read_text() == open(B.TUNED_INC).read(), "rebuild: the library's table is stale"
+    table = autotune.load_table()
+    assert n == len(table)
+    for key, e in table.items():
+        cls, d, k, nb = key.split(":")
+        got = autotune.library_lookup(cls, int(d[1:]), int(k[1:]), 1 << int(nb[3:]))
+        assert got is not None, key
+        assert got["BLOCK_SIZE"] == int(e["best"].get("BLOCK_SIZE", 0) or 0), key
+        assert got["leaf_span_delta"] == int(e["best"].get("leaf_span_delta", 0) or 0), key
+        assert got["tile"] == int(e["best"].get("tile", 0) or 0), key
+    # more than one size bucket from every tuned size: no entry
+    assert all(int(key.split(":")[3][3:]) < 29 for key in table)
+    assert autotune.library_lookup("kd", 3, 16, 1 << 30) is None
