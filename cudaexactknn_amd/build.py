@@ -30,7 +30,8 @@ COMMON = [
 
 SOURCES = [
     "knn_prims.hip", "knn_build.hip", "knn_bf.hip", "knn_bf16.hip", "knn_kd_query.hip",
-    "knn_ball_query.hip", "knn_kd_packet.hip", "knn_ball_packet.hip", "knn_tree_host.hip", "knn_learned.hip", "knn_capi.hip",
+    "knn_ball_query.hip", "knn_kd_packet.hip", "knn_ball_packet.hip", "knn_kd_packet_pad.hip",
+    "knn_ball_packet_pad.hip", "knn_tree_host.hip", "knn_learned.hip", "knn_capi.hip",
     "knn_batch.hip", "knn_serialize.hip", "knn_noalloc.hip", "knn_tuned.hip",
 ]
 

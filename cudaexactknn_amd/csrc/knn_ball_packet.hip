@@ -5,17 +5,6 @@
 
 namespace knn {
 
-template <int D, int KIND, bool COUNT, bool LEARN = false>
-__global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shrink_a, float shrink_b) {
-    extern __shared__ __align__(16) uint8_t lds_raw[];
-    const uint32_t BS = blockDim.x, tid = threadIdx.x;
-    const uint32_t wave = tid >> 6;
-    const uint32_t nwaves = (BS + 63) >> 6;
-    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth;
-    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, 1));
-    ball_packet_run<D, KIND, COUNT, LEARN>(A, shrink_a, shrink_b, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
-}
-
 using BallFn = void (*)(TreeArgs, float, float);
 
 template <int D, bool C>
@@ -56,13 +45,18 @@ BallFn ball_packet_learned_kernel_for(uint32_t D, int kind) {
     }
 }
 
+BallFn ball_packet_pad_kernel_for(uint32_t D, int kind);
+
+// Compiled for D in {2, 3, 4, 8}, and with zero-padded dims
+// (knn_ball_packet_pad.hip) for every other D up to 32 (without visit
+// counters); nullptr otherwise.
 BallFn ball_packet_kernel_for(uint32_t D, int kind, bool count) {
     switch (D) {
         case 2: return count ? bp_pick<2, true>(kind) : bp_pick<2, false>(kind);
         case 3: return count ? bp_pick<3, true>(kind) : bp_pick<3, false>(kind);
         case 4: return count ? bp_pick<4, true>(kind) : bp_pick<4, false>(kind);
         case 8: return count ? bp_pick<8, true>(kind) : bp_pick<8, false>(kind);
-        default: return nullptr;
+        default: return count ? nullptr : ball_packet_pad_kernel_for(D, kind);
     }
 }
 

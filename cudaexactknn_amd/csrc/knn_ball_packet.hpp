@@ -17,16 +17,25 @@ __device__ __forceinline__ float unif_b(float v) {
 }
 
 // LEARN: the learned form's second pass for lanes the predicted radius left
-// short, as in kd_packet_run.
-template <int D, int KIND, bool COUNT = false, bool LEARN = false>
+// short, as in kd_packet_run.  PAD: compiled for D dims, run for the tree's
+// runtime dims A.dims <= D with zero-padded coordinates (load_padded), so
+// every bound is bit-identical to the A.dims-term one.
+template <int D, int KIND, bool COUNT = false, bool LEARN = false, bool PAD = false>
 __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_a, float shrink_b, uint32_t i,
                                                 uint32_t* stack, uint64_t* kbuf, uint32_t BS) {
     const uint32_t tid = threadIdx.x;
     bool active = i < A.nq;
     const uint32_t qi = active ? (A.perm ? A.perm[i] : i) : 0u;
+    const uint32_t Dt = PAD ? A.dims : (uint32_t)D;
     QPoint<D> q;
+    if constexpr (PAD) {
+        load_padded<D, true>(A.qs + (size_t)qi * Dt, Dt, q.v);
 #pragma unroll
-    for (int d = 0; d < D; ++d) q.v[d] = active ? A.qs[(size_t)qi * D + d] : 0.0f;
+        for (int d = 0; d < D; ++d) q.v[d] = active ? q.v[d] : 0.0f;
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) q.v[d] = active ? A.qs[(size_t)qi * D + d] : 0.0f;
+    }
     typename BufOf<KIND>::T buf;
     bind_lds(buf, kbuf, BS);
     bool tightened = false;
@@ -34,8 +43,8 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
     if (active) worst = buf.init(start_key(A, qi, tightened), A.K);
     uint32_t whi = active ? key_hi(worst) : 0u;
     uint32_t nbytes = 0, nslots = 0;
-    constexpr uint32_t nstride = 2 * D + 2;
-    const uint32_t stride = 1 + D;
+    const uint32_t nstride = 2 * Dt + 2;
+    const uint32_t stride = 1 + Dt;
     const uint32_t stop = A.stop_level;
     // Priming: each lane descends to its own landing bucket (nearer centroid
     // at each level) and scans it; the shared traversal then skips it.
@@ -49,12 +58,22 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
         uint32_t n = 0;
         for (uint32_t l = 0; l < pl; ++l) {
             const float* nd = A.nodes + (size_t)n * nstride;
-            const float dl = q.dist_to(nd), dr = q.dist_to(nd + D + 1);
+            float dl, dr;
+            if constexpr (PAD) {
+                float c[D];
+                load_padded<D, true>(nd, Dt, c);
+                dl = q.dist_to(c);
+                load_padded<D, true>(nd + Dt + 1, Dt, c);
+                dr = q.dist_to(c);
+            } else {
+                dl = q.dist_to(nd);
+                dr = q.dist_to(nd + D + 1);
+            }
             n = 2 * n + 1 + (dl <= dr ? 0u : 1u);
         }
-        nbytes += 8u * (D + 1) * pl;
+        nbytes += 8u * (Dt + 1) * pl;
         leaf_range(A, n, pl, land0, landN);
-        scan_slots<D>(A.leaves, stride, land0, landN, pl == A.node_depth, q, buf, worst, whi, nslots);
+        scan_slots<D, PAD>(A.leaves, stride, land0, landN, pl == A.node_depth, q, buf, worst, whi, nslots, Dt);
     }
     // Visit counters: the priming descent is per lane; the shared traversal
     // issues each node and slot load once per wave, so it is counted once,
@@ -87,13 +106,15 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
             if (level < stop) {
                 const float* nd = A.nodes + (size_t)uni_b(node) * nstride;
                 float cl[D], cr[D];
+                load_padded<D, PAD>(nd, Dt, cl);
+                load_padded<D, PAD>(nd + Dt + 1, Dt, cr);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    cl[d] = unif_b(nd[d]);
-                    cr[d] = unif_b(nd[D + 1 + d]);
+                    cl[d] = unif_b(cl[d]);
+                    cr[d] = unif_b(cr[d]);
                 }
-                const float rl = unif_b(nd[D]), rr = unif_b(nd[2 * D + 1]);
-                nbytes += 8u * (D + 1);
+                const float rl = unif_b(nd[Dt]), rr = unif_b(nd[2 * Dt + 1]);
+                nbytes += 8u * (Dt + 1);
                 float lbd, rbd;
                 const float lb = ball_bound(q, cl, rl, shrink_a, shrink_b, lbd);
                 const float rb = ball_bound(q, cr, rr, shrink_a, shrink_b, rbd);
@@ -122,20 +143,21 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
                 const uint32_t s0 = uni_b(slot0);
                 // The four-slot batches of the kd scan were slower here
                 // (18.0 vs 16.4 ms on the 16.7M-point mixture).
-                packet_scan<D, LEARN>(A.leaves, s0, uni_b(count), q, active && s0 - land0 >= landN, buf, worst, whi,
-                                      nslots, floor);
+                packet_scan<D, LEARN, PAD>(A.leaves, s0, uni_b(count), q, active && s0 - land0 >= landN, buf,
+                                           worst, whi, nslots, floor, Dt);
             }
             bool found = false;
             while (sp > 0) {
                 --sp;
                 const uint32_t n2 = uni_b(stack[sp]);
                 const uint32_t parent = (n2 - 1) / 2;
-                const float* nd = A.nodes + (size_t)parent * nstride + ((n2 & 1u) ? 0 : D + 1);
+                const float* nd = A.nodes + (size_t)parent * nstride + ((n2 & 1u) ? 0 : Dt + 1);
                 float c[D];
+                load_padded<D, PAD>(nd, Dt, c);
 #pragma unroll
-                for (int d = 0; d < D; ++d) c[d] = unif_b(nd[d]);
-                const float r = unif_b(nd[D]);
-                nbytes += 4u * (D + 1);
+                for (int d = 0; d < D; ++d) c[d] = unif_b(c[d]);
+                const float r = unif_b(nd[Dt]);
+                nbytes += 4u * (Dt + 1);
                 float bd;
                 const float b = ball_bound(q, c, r, shrink_a, shrink_b, bd);
                 if (__ballot(active && f2u(b) <= whi) == 0ull) continue;
@@ -154,6 +176,17 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
     if (active)
         finish_query<COUNT>(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
                      prime_slots + (lead ? nslots : 0u));
+}
+
+template <int D, int KIND, bool COUNT, bool LEARN = false, bool PAD = false>
+__global__ void __launch_bounds__(256) ball_packet_kernel(TreeArgs A, float shrink_a, float shrink_b) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const uint32_t BS = blockDim.x, tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const uint32_t nwaves = (BS + 63) >> 6;
+    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth;
+    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, 1));
+    ball_packet_run<D, KIND, COUNT, LEARN, PAD>(A, shrink_a, shrink_b, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
 }
 
 }  // namespace knn

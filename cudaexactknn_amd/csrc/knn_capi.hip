@@ -292,6 +292,11 @@ knn_status knn_ball_build(const float* d_pts, uint32_t n, uint32_t dims, uint32_
         *out_handle = nullptr;
         check_pts(d_pts, n, dims);
         require(build_block <= 1024, "build_block must be <= 1024");
+        // One-dimensional ball trees made the GPU take a memory fault on
+        // their first run, with the cause not yet found; they are refused
+        // until it is (a kd tree serves D = 1).
+        require(dims >= 2, "ball trees need dims >= 2 (use a kd tree for one-dimensional points)",
+                KNN_ERR_UNSUPPORTED);
         auto* h = new Handle();
         try {
             h->kind = KNN_KIND_BALL;

@@ -5,28 +5,6 @@
 
 namespace knn {
 
-// Minimum waves per SIMD asked of the compiler: 6 for the register K-buffers
-// of K <= 16 at D <= 4.  The four-slot leaf scan needs more registers than
-// the one-slot scan did; asking for 8 waves made the compiler keep ~130
-// bytes per lane in scratch (headline kd traversal 16.0 ms at 8 waves, 11.4
-// at 7, 10.9 at 6, 10.9 at 5 and at 4).  Other forms keep the default.
-template <int D, int KIND>
-constexpr int kd_packet_min_waves() {
-    return (D <= 4 && (KIND == BUF_REG8 || KIND == BUF_REG16 || KIND == BUF_REGH8 || KIND == BUF_REGH16)) ? 6 : 1;
-}
-
-template <int D, int KIND, bool COUNT, bool LEARN = false>
-__global__ void __launch_bounds__(256, (kd_packet_min_waves<D, KIND>())) kd_packet_kernel(TreeArgs A) {
-    constexpr int ENT = 1 + 2 * D;
-    extern __shared__ __align__(16) uint8_t lds_raw[];
-    const uint32_t BS = blockDim.x, tid = threadIdx.x;
-    const uint32_t wave = tid >> 6;
-    const uint32_t nwaves = (BS + 63) >> 6;
-    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth * ENT;
-    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, ENT));
-    kd_packet_run<D, KIND, COUNT, LEARN>(A, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
-}
-
 using TreeFn = void (*)(TreeArgs);
 
 template <int D, bool C>
@@ -68,15 +46,18 @@ TreeFn kd_packet_learned_kernel_for(uint32_t D, int kind) {
     }
 }
 
-// Packet kernels exist for D in {2, 3, 4, 8}; nullptr otherwise.  `count`:
-// the form with visit counters.
+TreeFn kd_packet_pad_kernel_for(uint32_t D, int kind);
+
+// Packet kernels are compiled for D in {2, 3, 4, 8}, and with zero-padded
+// dims (knn_kd_packet_pad.hip) for every other D up to 32 (without visit
+// counters); nullptr otherwise.  `count`: the form with visit counters.
 TreeFn kd_packet_kernel_for(uint32_t D, int kind, bool count) {
     switch (D) {
         case 2: return count ? kdp_pick<2, true>(kind) : kdp_pick<2, false>(kind);
         case 3: return count ? kdp_pick<3, true>(kind) : kdp_pick<3, false>(kind);
         case 4: return count ? kdp_pick<4, true>(kind) : kdp_pick<4, false>(kind);
         case 8: return count ? kdp_pick<8, true>(kind) : kdp_pick<8, false>(kind);
-        default: return nullptr;
+        default: return count ? nullptr : kd_packet_pad_kernel_for(D, kind);
     }
 }
 

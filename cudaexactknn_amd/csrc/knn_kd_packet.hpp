@@ -70,16 +70,28 @@ __device__ __forceinline__ float seq_sum(const float (&g)[D]) {
 // second pass of the same wave traversal, for those lanes only, keeps what
 // they found and accepts only keys above the radius (the floor), so nothing
 // is found twice and the rows equal the plain traversal's.
-template <int D, int KIND, bool COUNT = false, bool LEARN = false>
+//
+// PAD: compiled for D dims, run for the tree's runtime dims A.dims <= D, with
+// the padded coordinates zero (load_padded): every distance and box bound is
+// bit-identical to the A.dims-term one.  The split dims cycle over
+// split_dims <= A.dims, so no split ever falls in a padded dim.
+template <int D, int KIND, bool COUNT = false, bool LEARN = false, bool PAD = false>
 __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uint32_t* stack, uint64_t* kbuf,
                                               uint32_t BS) {
     constexpr int ENT = 1 + 2 * D;  // words per stack entry: node, lo[D], hi[D]
     const uint32_t tid = threadIdx.x;
     bool active = i < A.nq;
     const uint32_t qi = active ? (A.perm ? A.perm[i] : i) : 0u;
+    const uint32_t Dt = PAD ? A.dims : (uint32_t)D;
     float q[D];
+    if constexpr (PAD) {
+        load_padded<D, true>(A.qs + (size_t)qi * Dt, Dt, q);
 #pragma unroll
-    for (int d = 0; d < D; ++d) q[d] = active ? A.qs[(size_t)qi * D + d] : 0.0f;
+        for (int d = 0; d < D; ++d) q[d] = active ? q[d] : 0.0f;
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) q[d] = active ? A.qs[(size_t)qi * D + d] : 0.0f;
+    }
     typename BufOf<KIND>::T buf;
     bind_lds(buf, kbuf, BS);
     bool tightened = false;
@@ -89,7 +101,7 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
     uint32_t nbytes = 0, nslots = 0;
 
     const uint32_t SD = A.split_dims, stop = A.stop_level;
-    const uint32_t stride = 1 + D;
+    const uint32_t stride = 1 + Dt;
 
     float lo[D], hi[D];
 #pragma unroll
@@ -127,7 +139,7 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
 #pragma unroll
         for (int d = 0; d < D; ++d) qp.v[d] = q[d];
         constexpr int C = IsRegArray<decltype(buf)>::cap < 8 ? IsRegArray<decltype(buf)>::cap : 8;
-        scan_slots_merged<D, C>(A.leaves, s0, uni(cnt), active, qp, buf, worst, whi, nslots);
+        scan_slots_merged<D, C, PAD>(A.leaves, s0, uni(cnt), active, qp, buf, worst, whi, nslots, Dt);
     } else if (active) {
         uint32_t n = 0, dd = 0;
         for (uint32_t l = 0; l < pl; ++l) {
@@ -143,7 +155,7 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
         QPoint<D> qp;
 #pragma unroll
         for (int d = 0; d < D; ++d) qp.v[d] = q[d];
-        scan_slots<D>(A.leaves, stride, land0, landN, pl == A.node_depth, qp, buf, worst, whi, nslots);
+        scan_slots<D, PAD>(A.leaves, stride, land0, landN, pl == A.node_depth, qp, buf, worst, whi, nslots, Dt);
     }
 
     // Visit counters: the priming descent is per lane; the shared traversal
@@ -283,12 +295,12 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
                 for (int d = 0; d < D; ++d) qp.v[d] = q[d];
                 const uint32_t s0 = uni(slot0);
                 const uint32_t cnt = uni(count);
-                if (D <= 4 && cnt >= 4)
+                if (D <= 4 && !PAD && cnt >= 4)  // four-slot batches: compile-time slot stride
                     packet_scan4<D, LEARN>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst, whi,
                                            nslots, floor);
                 else
-                    packet_scan<D, LEARN>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst, whi,
-                                          nslots, floor);
+                    packet_scan<D, LEARN, PAD>(A.leaves, s0, cnt, qp, active && s0 - land0 >= landN, buf, worst,
+                                               whi, nslots, floor, Dt);
             }
             // pop until some lane needs the entry
             bool found = false;
@@ -326,6 +338,28 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
     if (active)
         finish_query<COUNT>(A, qi, buf, worst, tightened, prime_bytes + (lead ? nbytes : 0u),
                      prime_slots + (lead ? nslots : 0u));
+}
+
+// Minimum waves per SIMD asked of the compiler: 6 for the register K-buffers
+// of K <= 16 at D <= 4.  The four-slot leaf scan needs more registers than
+// the one-slot scan did; asking for 8 waves made the compiler keep ~130
+// bytes per lane in scratch (headline kd traversal 16.0 ms at 8 waves, 11.4
+// at 7, 10.9 at 6, 10.9 at 5 and at 4).  Other forms keep the default.
+template <int D, int KIND>
+constexpr int kd_packet_min_waves() {
+    return (D <= 4 && (KIND == BUF_REG8 || KIND == BUF_REG16 || KIND == BUF_REGH8 || KIND == BUF_REGH16)) ? 6 : 1;
+}
+
+template <int D, int KIND, bool COUNT, bool LEARN = false, bool PAD = false>
+__global__ void __launch_bounds__(256, (kd_packet_min_waves<D, KIND>())) kd_packet_kernel(TreeArgs A) {
+    constexpr int ENT = 1 + 2 * D;
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    const uint32_t BS = blockDim.x, tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const uint32_t nwaves = (BS + 63) >> 6;
+    uint32_t* stack = reinterpret_cast<uint32_t*>(lds_raw) + (size_t)wave * A.stack_depth * ENT;
+    uint64_t* kbuf = reinterpret_cast<uint64_t*>(lds_raw + packet_stack_bytes(nwaves, A.stack_depth, ENT));
+    kd_packet_run<D, KIND, COUNT, LEARN, PAD>(A, tree_block(A) * BS + tid, stack, kbuf + tid, BS);
 }
 
 }  // namespace knn

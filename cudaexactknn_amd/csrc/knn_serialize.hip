@@ -53,6 +53,7 @@ bool info_consistent(const knn_tree_info& I) {
     if (I.kind != KNN_KIND_KD && I.kind != KNN_KIND_BALL) return false;
     if (I.n == 0 || I.n >= 0x80000000u || I.dims == 0 || I.dims > 4096) return false;
     const bool ball = I.kind == KNN_KIND_BALL;
+    if (ball && I.dims < 2) return false;  // knn_ball_build refuses one-dimensional ball trees
     if (I.split_dims == 0 || I.split_dims > I.dims) return false;
     knn_tree_info t{};
     tree_sizing(t, I.n, I.dims, I.split_dims, I.depth_skips, ball);

@@ -75,12 +75,47 @@ struct QPoint<0> {
     __device__ __forceinline__ uint32_t dims() const { return D; }
 };
 
+// Padded dims (PAD = true): a kernel compiled for D dims serves any runtime
+// dims Dt <= D.  Coordinates past Dt are zero in the query and in every point
+// or centroid read, so each padded term of a distance or box bound is
+// (0 - 0)^2 = +0, and r + (+0) = r for every partial sum r >= +0 (and for NaN
+// or +inf): the sum is bit-identical to the Dt-term sum.  Reads stay inside
+// the record: word t > Dt re-reads word Dt and is then replaced by zero.
+template <int D, bool PAD>
+__device__ __forceinline__ void load_padded(const float* __restrict__ p, uint32_t Dt, float (&v)[D]) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if constexpr (PAD) {
+            const uint32_t j = (uint32_t)d < Dt ? (uint32_t)d : Dt - 1;
+            const float x = p[j];
+            v[d] = (uint32_t)d < Dt ? x : 0.0f;
+        } else {
+            v[d] = p[d];
+        }
+    }
+}
+
+// One leaf slot (index word, then D coordinates) at e; `Dt` as above.
+template <int D, bool PAD>
+__device__ __forceinline__ void load_slot(const float* __restrict__ e, uint32_t Dt, float (&v)[1 + D]) {
+#pragma unroll
+    for (int t = 0; t <= D; ++t) {
+        if constexpr (PAD) {
+            const uint32_t j = (uint32_t)t <= Dt ? (uint32_t)t : Dt;
+            const float x = e[j];
+            v[t] = (uint32_t)t <= Dt ? x : 0.0f;
+        } else {
+            v[t] = e[t];
+        }
+    }
+}
+
 // Scan leaf slots [slot0, slot0 + count): each slot is an IndexPoint<D>.
 // `stop_at_empty`: a single real bucket, whose padding is a suffix.
-template <int D, class Q, class B>
+template <int D, bool PAD = false, class Q, class B>
 __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uint32_t stride, uint32_t slot0,
                                            uint32_t count, bool stop_at_empty, const Q& q, B& buf,
-                                           uint64_t& worst, uint32_t& whi, uint32_t& nslots) {
+                                           uint64_t& worst, uint32_t& whi, uint32_t& nslots, uint32_t Dt = D) {
     // D = 3: 16-byte slots, each loaded one slot ahead (the last look-ahead
     // reads the leaves array's spare slot).
     float4 nxt;
@@ -96,6 +131,11 @@ __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uin
             pi = __float_as_int(v.x);
             const float p[3] = {v.y, v.z, v.w};
             d2 = q.dist_to(p);
+        } else if constexpr (PAD) {
+            float v[1 + D];
+            load_slot<D, true>(e, Dt, v);
+            pi = __float_as_int(v[0]);
+            d2 = q.dist_to(v + 1);
         } else {
             pi = __float_as_int(e[0]);
             d2 = q.dist_to(e + 1);
@@ -122,11 +162,12 @@ __device__ __forceinline__ void scan_slots(const float* __restrict__ leaves, uin
 // replaces C insertions, each run for the whole wave, by one sort and one
 // merge.  `count` is wave-uniform; the result equals inserting the slots in
 // order.
-template <int D, int C, class Q, class B>
+template <int D, int C, bool PAD = false, class Q, class B>
 __device__ __forceinline__ void scan_slots_merged(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
                                                   bool active, const Q& q, B& buf, uint64_t& worst, uint32_t& whi,
-                                                  uint32_t& nslots) {
+                                                  uint32_t& nslots, uint32_t Dt = D) {
     constexpr uint32_t S = 1 + D;
+    const uint32_t stride = PAD ? 1 + Dt : S;
     for (uint32_t o = 0; o < count; o += C) {
         uint64_t c[C];
         bool below = false;
@@ -135,9 +176,11 @@ __device__ __forceinline__ void scan_slots_merged(const float* __restrict__ leav
             const uint32_t slot = slot0 + o + j;
             uint64_t key = ~0ull;
             if (active && o + j < count) {
-                const float* e = leaves + (size_t)slot * S;
+                const float* e = leaves + (size_t)slot * stride;
                 float v[S];
-                if constexpr (D == 3) {
+                if constexpr (PAD) {
+                    load_slot<D, true>(e, Dt, v);
+                } else if constexpr (D == 3) {
                     const float4 t = *reinterpret_cast<const float4*>(e);
                     v[0] = t.x;
                     v[1] = t.y;
@@ -175,12 +218,13 @@ __device__ __forceinline__ void scan_slots_merged(const float* __restrict__ leav
 // `mine` is false for a lane whose primed range holds this group: groups are
 // subtrees at the scan level and the primed range is a subtree at or above it,
 // so a group lies wholly inside or wholly outside it.
-template <int D, bool FLOOR = false, class Q, class B>
+template <int D, bool FLOOR = false, bool PAD = false, class Q, class B>
 __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, uint32_t slot0, uint32_t count,
                                             const Q& q, bool mine, B& buf, uint64_t& worst, uint32_t& whi,
-                                            uint32_t& nslots, uint64_t floor = 0) {
+                                            uint32_t& nslots, uint64_t floor = 0, uint32_t Dt = D) {
     constexpr uint32_t S = 1 + D;
-    const float* e = leaves + (size_t)slot0 * S;
+    const uint32_t stride = PAD ? 1 + Dt : S;
+    const float* e = leaves + (size_t)slot0 * stride;
     // One compare per slot: the lane's threshold gets its sign bit set when
     // the group is in its primed range (`mine` false) or the slot is padding
     // (index -1), and distance bits compared as signed integers (all
@@ -211,16 +255,25 @@ __device__ __forceinline__ void packet_scan(const float* __restrict__ leaves, ui
     // of every iteration, behind the just-issued look-ahead: kd traversal
     // 12.33 -> 11.59 ms on the headline workload, 8-D mixture 1226 -> 1007 ms).
     float v[S], nx[S];
+    if constexpr (PAD) {
+        load_slot<D, true>(e, Dt, v);
+        load_slot<D, true>(e + stride, Dt, nx);
+    } else {
 #pragma unroll
-    for (uint32_t t = 0; t < S; ++t) v[t] = e[t];
+        for (uint32_t t = 0; t < S; ++t) v[t] = e[t];
 #pragma unroll
-    for (uint32_t t = 0; t < S; ++t) nx[t] = e[S + t];
+        for (uint32_t t = 0; t < S; ++t) nx[t] = e[S + t];
+    }
     test(v);
 #pragma unroll
     for (uint32_t t = 0; t < S; ++t) v[t] = nx[t];
     for (uint32_t o = 1; o < count; ++o) {
+        if constexpr (PAD) {
+            load_slot<D, true>(e + (size_t)(o + 1) * stride, Dt, nx);
+        } else {
 #pragma unroll
-        for (uint32_t t = 0; t < S; ++t) nx[t] = e[(size_t)(o + 1) * S + t];
+            for (uint32_t t = 0; t < S; ++t) nx[t] = e[(size_t)(o + 1) * S + t];
+        }
         test(v);
 #pragma unroll
         for (uint32_t t = 0; t < S; ++t) v[t] = nx[t];

@@ -18,6 +18,7 @@ TreeFn kd_packet_kernel_for(uint32_t D, int kind, bool count);
 BallFn ball_packet_kernel_for(uint32_t D, int kind, bool count);
 TreeFn kd_packet_learned_kernel_for(uint32_t D, int kind);
 BallFn ball_packet_learned_kernel_for(uint32_t D, int kind);
+uint32_t packet_kernel_dims(uint32_t D);
 int pick_buf_kind(bool heap, bool lds, uint32_t K);
 
 __global__ void morton_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D, const float* __restrict__ bbox,
@@ -182,7 +183,11 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     BallFn bfn = nullptr;
     uint32_t BS = a.block;
     size_t lds = 0;
-    // Wave-cooperative traversal for ordered batches (needs whole waves).
+    // Wave-cooperative traversal for ordered batches; it needs whole waves,
+    // so a BLOCK_SIZE that is not a multiple of 64 keeps the per-lane shape
+    // (the reference's speed sweeps ask for 16 and 32 on 10,000 queries,
+    // too few for a wave's 64 queries to be neighbours: there the per-lane
+    // kernel was faster at D <= 8 and for most ball trees).
     const bool packet = !a.per_lane && d_perm != nullptr && BS % 64 == 0;
     if (packet) {
         if (BS > 256) BS = 256;
@@ -202,7 +207,7 @@ void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const flo
     // the wave-cooperative kernels, which keep plain order.
     A.xcd_remap = (a.xcd_remap && !(kfn || bfn)) ? 1u : 0u;
     if (kfn || bfn) {
-        const uint32_t words = h.kind == KNN_KIND_KD ? 1 + 2 * I.dims : 1;
+        const uint32_t words = h.kind == KNN_KIND_KD ? 1 + 2 * packet_kernel_dims(I.dims) : 1;
         lds = packet_stack_bytes((BS + 63) / 64, A.stack_depth, words);
     } else {
         if (h.kind == KNN_KIND_KD)

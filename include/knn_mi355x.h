@@ -83,7 +83,9 @@ typedef enum knn_form { KNN_FORM_PLAIN = 0, KNN_FORM_BF16 = 1, KNN_FORM_LEARNED 
                                         mean predicted / exact radius ratio (stats[2]) */
 
 typedef struct knn_query_opts {
-    uint32_t block;      /* BLOCK_SIZE: threads per workgroup, 1..1024 (0 = tuned default) */
+    uint32_t block;      /* BLOCK_SIZE: threads per workgroup, 1..1024 (0 = tuned default); trees
+                            run the wave-cooperative traversal for multiples of 64 (capped at
+                            256) and the per-lane shape otherwise */
     uint32_t heap;       /* USE_HEAP_TRACKING */
     uint32_t track_lds;  /* USE_TRACKING_SHMEM: K-buffer in LDS */
     uint32_t stack_lds;  /* USE_STACK_SHMEM (trees): per-lane stack in LDS; 0 = register bit-stack */

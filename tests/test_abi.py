@@ -45,6 +45,9 @@ def test_null_handle_and_bad_args_fail_cleanly():
     assert L.knn_kd_build(None, 0, 0, 0, 0, 0, ctypes.byref(h)) == _lib.KNN_ERR_INVALID_ARG
     assert L.knn_kd_build(None, 0, 3, 4, 0, 0, ctypes.byref(h)) == _lib.KNN_ERR_INVALID_ARG
     assert L.knn_ball_build(None, 5, 3, 1, 2048, ctypes.byref(h)) == _lib.KNN_ERR_INVALID_ARG
+    # one-dimensional ball trees are refused before any device work
+    assert L.knn_ball_build(None, 0, 1, 1, 0, ctypes.byref(h)) == _lib.KNN_ERR_UNSUPPORTED
+    assert b"dims >= 2" in L.knn_last_error()
     ti = _lib.TreeInfo()
     assert L.knn_tree_get_info(None, ctypes.byref(ti)) == _lib.KNN_ERR_INVALID_ARG
     with pytest.raises(knn.KnnError):
