@@ -14,14 +14,38 @@
 
 namespace knn {
 
+// Candidate queue (QUEUE kernels, register K-buffers, D <= kBfQueueMaxD).
+// A pair that passes the lane's threshold is not inserted at once: its key
+// goes to the lane's queue in LDS (stride BS), and after each group of
+// kBfGroup points, if some lane holds more than kBfQueue - kBfGroup keys,
+// every lane sorts its queue (bitonic network, empty slots ~0) and merges it
+// into its buffer.  The buffer is then the K smallest keys of itself and the
+// queued keys, which is what inserting them one by one leaves (a key that is
+// not below the K-th key of its moment is never among the K smallest at the
+// end), so rows are unchanged.  It pays where insertions are dense: a wave of
+// 64 queries starting a chunk of a few thousand points has some lane
+// inserting at almost every point, and each wave-wide insertion costs
+// several pair tests; the queue turns those into an LDS store, and a merge
+// per few candidates of the fullest lane.  On long chunks insertions are rare
+// and the plain kernel is faster (the host picks, bf_queue_pays).
+constexpr int kBfQueue = 8;
+constexpr int kBfGroup = 4;
+constexpr uint32_t kBfQueueMaxD = 16;
+__host__ __device__ constexpr size_t bf_queue_bytes(uint32_t BS) { return (size_t)BS * kBfQueue * 8; }
+// A chunk of at most 1024 K points: its first ~64 K points (one per lane
+// and K-buffer slot) and the ~64 K ln(chunk / 64 K) after them find some
+// lane inserting at most points.
+inline bool bf_queue_pays(uint32_t chunk, uint32_t K) { return (uint64_t)chunk <= 1024ull * K; }
+
 // Split scans: blockIdx.y selects the point chunk [y*chunk, (y+1)*chunk); with
-// `partial` set, each (query, chunk) writes its K smallest keys ascending to
-// partial[(q * gridDim.y + y) * K ...] for bf_merge_kernel.
+// `partial` set, each (query, chunk) writes its K smallest keys ascending,
+// key t of chunk y at partial[(y * K + t) * nq + q] (query-minor, so the
+// stores and the merge's loads are coalesced).
 // Large D keeps the whole query in VGPRs: allow up to 256 lanes per
 // workgroup so the compiler may use more than 128 VGPRs instead of spilling.
 constexpr int bf_max_threads(int D) { return D >= 32 ? 256 : 1024; }
 
-template <int D, int KIND>
+template <int D, int KIND, bool QUEUE = false>
 __global__ void __launch_bounds__(bf_max_threads(D)) bf_kernel(const float* __restrict__ pts_all, uint32_t n_all,
                                                   const float* __restrict__ qs, uint32_t nq, float maxd2,
                                                   uint32_t K, int32_t* __restrict__ oidx,
@@ -29,7 +53,6 @@ __global__ void __launch_bounds__(bf_max_threads(D)) bf_kernel(const float* __re
                                                   uint64_t* __restrict__ partial) {
     const uint32_t c0 = blockIdx.y * chunk;
     const uint32_t n = c0 < n_all ? min(chunk, n_all - c0) : 0u;
-    const float* pts = pts_all + (size_t)c0 * D;
     constexpr int DP = (D + 3) & ~3;
     extern __shared__ __align__(16) uint8_t lds_raw[];
     float* tile = reinterpret_cast<float*>(lds_raw);
@@ -46,10 +69,45 @@ __global__ void __launch_bounds__(bf_max_threads(D)) bf_kernel(const float* __re
     uint64_t worst = empty;
     if (active) worst = buf.init(empty, K);
     uint32_t whi = key_hi(worst);
+    static_assert(!QUEUE || IsRegArray<decltype(buf)>::value, "the queue merges into register buffers");
+    // The queue follows the tile (register kinds have no LDS K-buffer).
+    uint64_t* queue = kbuf + tid;
+    uint32_t qn = 0;
+    auto flush = [&]() {
+        if constexpr (QUEUE) {
+            uint64_t c[kBfQueue];
+#pragma unroll
+            for (int t = 0; t < kBfQueue; ++t) c[t] = (uint32_t)t < qn ? queue[(size_t)t * BS] : ~0ull;
+            bitonic_sort<kBfQueue>(c);
+            worst = buf.merge_sorted(c);
+            whi = key_hi(worst);
+            qn = 0;
+        }
+    };
+    // One pair: the lane's threshold test, then a queued or an immediate
+    // insertion.
+    auto offer = [&](float d2, uint32_t idx) {
+        if (f2u(d2) <= whi) {
+            const uint64_t key = make_key(d2, idx);
+            if constexpr (QUEUE) {
+                queue[(size_t)qn * BS] = key;
+                ++qn;
+            } else if (key < worst) {
+                worst = buf.consider(key);
+                whi = key_hi(worst);
+            }
+        }
+    };
+    // After a group of kBfGroup pairs: room for the next group in every queue.
+    auto room = [&]() {
+        if constexpr (QUEUE) {
+            if (__ballot(qn > (uint32_t)(kBfQueue - kBfGroup)) != 0ull) flush();
+        }
+    };
     for (uint32_t base = 0; base < n; base += tile_pts) {
         const uint32_t cnt = min(tile_pts, n - base);
         __syncthreads();
-        const float* src = pts + (size_t)base * D;
+        const float* src = pts_all + ((size_t)c0 + base) * D;
         for (uint32_t e = tid; e < cnt * D; e += BS) {
             const uint32_t p = e / D, d = e - p * D;
             tile[p * DP + d] = src[e];
@@ -62,40 +120,61 @@ __global__ void __launch_bounds__(bf_max_threads(D)) bf_kernel(const float* __re
                 float cur[DP];
 #pragma unroll
                 for (int d = 0; d < DP; ++d) cur[d] = tile[d];
-                for (uint32_t j = 0; j < cnt; ++j) {
+                uint32_t j = 0;
+                if constexpr (QUEUE) {
+                    // Groups of kBfGroup points, the next group's LDS reads
+                    // issued before this group is tested.
+                    float grp[kBfGroup][DP];
+#pragma unroll
+                    for (int g = 0; g < kBfGroup; ++g)
+#pragma unroll
+                        for (int d = 0; d < DP; ++d) grp[g][d] = tile[min((uint32_t)g, cnt - 1) * DP + d];
+                    for (; j + kBfGroup <= cnt; j += kBfGroup) {
+                        float nxt[kBfGroup][DP];
+#pragma unroll
+                        for (int g = 0; g < kBfGroup; ++g)
+#pragma unroll
+                            for (int d = 0; d < DP; ++d)
+                                nxt[g][d] = tile[min(j + kBfGroup + g, cnt - 1) * DP + d];
+#pragma unroll
+                        for (int g = 0; g < kBfGroup; ++g) offer(dist2<D>(q, grp[g]), c0 + base + j + g);
+                        room();
+#pragma unroll
+                        for (int g = 0; g < kBfGroup; ++g)
+#pragma unroll
+                            for (int d = 0; d < DP; ++d) grp[g][d] = nxt[g][d];
+                    }
+#pragma unroll
+                    for (int d = 0; d < DP; ++d) cur[d] = tile[min(j, cnt - 1) * DP + d];
+                }
+                for (; j < cnt; ++j) {
                     const float* nx = tile + min(j + 1, cnt - 1) * DP;
                     float nxt[DP];
 #pragma unroll
                     for (int d = 0; d < DP; ++d) nxt[d] = nx[d];
-                    const float d2 = dist2<D>(q, cur);
-                    if (f2u(d2) <= whi) {
-                        const uint64_t key = make_key(d2, c0 + base + j);
-                        if (key < worst) {
-                            worst = buf.consider(key);
-                            whi = key_hi(worst);
-                        }
-                    }
+                    offer(dist2<D>(q, cur), c0 + base + j);
 #pragma unroll
                     for (int d = 0; d < DP; ++d) cur[d] = nxt[d];
                 }
+            } else if constexpr (QUEUE) {
+                uint32_t j = 0;
+                for (; j + kBfGroup <= cnt; j += kBfGroup) {
+#pragma unroll
+                    for (int g = 0; g < kBfGroup; ++g) offer(dist2<D>(q, tile + (j + g) * DP), c0 + base + j + g);
+                    room();
+                }
+                for (; j < cnt; ++j) offer(dist2<D>(q, tile + j * DP), c0 + base + j);
             } else {
 #pragma unroll 4
-                for (uint32_t j = 0; j < cnt; ++j) {
-                    const float d2 = dist2<D>(q, tile + j * DP);
-                    if (f2u(d2) <= whi) {
-                        const uint64_t key = make_key(d2, c0 + base + j);
-                        if (key < worst) {
-                            worst = buf.consider(key);
-                            whi = key_hi(worst);
-                        }
-                    }
-                }
+                for (uint32_t j = 0; j < cnt; ++j) offer(dist2<D>(q, tile + j * DP), c0 + base + j);
             }
+            room();
         }
     }
     if (active) {
+        flush();
         if (partial)
-            buf.write_keys(partial + ((size_t)qi * gridDim.y + blockIdx.y) * K, K);
+            buf.write_keys(partial + (size_t)blockIdx.y * K * nq + qi, K, nq);
         else
             buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
     }
@@ -150,7 +229,7 @@ __global__ void __launch_bounds__(1024) bf_kernel_rt(const float* __restrict__ p
     }
     if (active) {
         if (partial)
-            buf.write_keys(partial + ((size_t)qi * gridDim.y + blockIdx.y) * K, K);
+            buf.write_keys(partial + (size_t)blockIdx.y * K * nq + qi, K, nq);
         else
             buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
     }
@@ -161,13 +240,13 @@ __global__ void __launch_bounds__(1024) bf_kernel_rt(const float* __restrict__ p
 // (array mode) or descending (heap mode) rows.
 __global__ void __launch_bounds__(256) bf_merge_kernel(const uint64_t* __restrict__ partial, uint32_t nq, uint32_t S,
                                                        uint32_t K, bool descending, int32_t* __restrict__ oidx,
-                                                       float* __restrict__ odist) {
+                                                       float* __restrict__ odist, size_t qs, size_t ts) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     uint16_t* head = reinterpret_cast<uint16_t*>(lds_raw);
     const uint32_t BS = blockDim.x, tid = threadIdx.x;
     const uint32_t qi = blockIdx.x * BS + tid;
     if (qi >= nq) return;
-    const uint64_t* L = partial + (size_t)qi * S * K;
+    const uint64_t* L = partial + (size_t)qi * qs;
     for (uint32_t s = 0; s < S; ++s) head[(size_t)s * BS + tid] = 0;
     for (uint32_t k = 0; k < K; ++k) {
         uint64_t best = ~0ull;
@@ -175,7 +254,7 @@ __global__ void __launch_bounds__(256) bf_merge_kernel(const uint64_t* __restric
         for (uint32_t s = 0; s < S; ++s) {
             const uint32_t h = head[(size_t)s * BS + tid];
             if (h < K) {
-                const uint64_t v = L[(size_t)s * K + h];
+                const uint64_t v = L[((size_t)s * K + h) * ts];
                 if (v < best) { best = v; bs = s; }
             }
         }
@@ -186,11 +265,51 @@ __global__ void __launch_bounds__(256) bf_merge_kernel(const uint64_t* __restric
     }
 }
 
+// The same merge for K <= 32: each lane merges its query's S sorted lists
+// into a register K-buffer one list at a time (merge_sorted: the buffer
+// becomes the K smallest keys of itself and the list), reading each list
+// once; the head scan above reads S heads per output slot.
+template <int KCAP, bool DESC>
+__global__ void __launch_bounds__(256) bf_merge_reg_kernel(const uint64_t* __restrict__ partial, uint32_t nq,
+                                                           uint32_t S, uint32_t K, int32_t* __restrict__ oidx,
+                                                           float* __restrict__ odist, size_t qs, size_t ts) {
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    RegArray<KCAP, DESC> buf;
+    buf.init(~0ull, K);
+    const uint64_t* L = partial + (size_t)qi * qs;
+    // List s + 1 is loaded while list s is merged.
+    uint64_t nx[KCAP];
+#pragma unroll
+    for (int t = 0; t < KCAP; ++t) nx[t] = (uint32_t)t < K ? L[(size_t)t * ts] : ~0ull;
+    for (uint32_t s = 0; s < S; ++s) {
+        uint64_t c[KCAP];
+#pragma unroll
+        for (int t = 0; t < KCAP; ++t) c[t] = nx[t];
+        const uint32_t sn = s + 1 < S ? s + 1 : s;
+#pragma unroll
+        for (int t = 0; t < KCAP; ++t) nx[t] = (uint32_t)t < K ? L[((size_t)sn * K + t) * ts] : ~0ull;
+        buf.merge_sorted(c);
+    }
+    buf.write(oidx + (size_t)qi * K, odist + (size_t)qi * K, K);
+}
+
+// Key t of list s of query q at partial[q * qs + (s * K + t) * ts]: the
+// bf16 form's lists are query-major (qs = S K, ts = 1), the split scans'
+// query-minor (qs = 1, ts = nq).
 void launch_bf_merge(const uint64_t* partial, uint32_t nq, uint32_t S, uint32_t K, bool descending, int32_t* oidx,
-                     float* odist, hipStream_t st) {
+                     float* odist, hipStream_t st, size_t qs, size_t ts) {
+    if (K <= 32) {
+        using MergeFn = void (*)(const uint64_t*, uint32_t, uint32_t, uint32_t, int32_t*, float*, size_t, size_t);
+        MergeFn fn = K <= 8    ? (descending ? bf_merge_reg_kernel<8, true> : bf_merge_reg_kernel<8, false>)
+                     : K <= 16 ? (descending ? bf_merge_reg_kernel<16, true> : bf_merge_reg_kernel<16, false>)
+                               : (descending ? bf_merge_reg_kernel<32, true> : bf_merge_reg_kernel<32, false>);
+        hipLaunchKernelGGL(fn, dim3((nq + 255) / 256), dim3(256), 0, st, partial, nq, S, K, oidx, odist, qs, ts);
+        return;
+    }
     const uint32_t mb = 128;
     hipLaunchKernelGGL(bf_merge_kernel, dim3((nq + mb - 1) / mb), dim3(mb), (size_t)mb * S * 2, st, partial, nq, S, K,
-                       descending, oidx, odist);
+                       descending, oidx, odist, qs, ts);
 }
 
 using BfFn = void (*)(const float*, uint32_t, const float*, uint32_t, float, uint32_t, int32_t*, float*, uint32_t,
@@ -199,7 +318,24 @@ using BfRtFn = void (*)(const float*, uint32_t, uint32_t, const float*, uint32_t
                         uint32_t, uint32_t, uint64_t*);
 
 template <int D>
-static BfFn bf_pick(int kind) {
+static BfFn bf_pick_queue(int kind) {
+    if constexpr ((uint32_t)D <= kBfQueueMaxD) {
+        switch (kind) {
+            case BUF_REG8: return bf_kernel<D, BUF_REG8, true>;
+            case BUF_REG16: return bf_kernel<D, BUF_REG16, true>;
+            case BUF_REG32: return bf_kernel<D, BUF_REG32, true>;
+            case BUF_REGH8: return bf_kernel<D, BUF_REGH8, true>;
+            case BUF_REGH16: return bf_kernel<D, BUF_REGH16, true>;
+            case BUF_REGH32: return bf_kernel<D, BUF_REGH32, true>;
+            default: return nullptr;
+        }
+    }
+    return nullptr;
+}
+
+template <int D>
+static BfFn bf_pick(int kind, bool queue) {
+    if (queue) return bf_pick_queue<D>(kind);
     switch (kind) {
         case BUF_REG8: return bf_kernel<D, BUF_REG8>;
         case BUF_REG16: return bf_kernel<D, BUF_REG16>;
@@ -288,39 +424,40 @@ void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial_ws, size_t partia
     uint32_t tile_pts = 8192 / DP;
     if (tile_pts > 1024) tile_pts = 1024;
     if (tile_pts < 8) tile_pts = 8;
-    BfFn fn = nullptr;
-    switch (D) {
-        case 2: fn = bf_pick<2>(kind); break;
-        case 3: fn = bf_pick<3>(kind); break;
-        case 4: fn = bf_pick<4>(kind); break;
-        case 5: fn = bf_pick<5>(kind); break;
-        case 6: fn = bf_pick<6>(kind); break;
-        case 8: fn = bf_pick<8>(kind); break;
-        case 16: fn = bf_pick<16>(kind); break;
-        case 32: fn = bf_pick<32>(kind); break;
-        case 64: fn = bf_pick<64>(kind); break;
-        case 128: fn = bf_pick<128>(kind); break;
-        default: break;
-    }
-    size_t lds = (size_t)tile_pts * DP * 4;
-    if (!fn) lds += (size_t)BS * D * 4 + 8;
-    if (buf_in_lds(kind)) lds += (size_t)BS * a.K * 8;
-    require(lds <= device_limits().max_lds_per_block,
-            "LDS request of " + std::to_string(lds) + " bytes exceeds the per-workgroup limit of " +
-                std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",
-            KNN_ERR_INVALID_ARG);
-    const uint32_t grid = (a.nq + BS - 1) / BS;
     const uint32_t n = h.info.n;
     uint32_t S = bf_split(n, a.nq, a.K, BS);
     // A caller-provided buffer bounds the split (fewer chunks, same rows).
     while (S > 1 && partial_ws && (size_t)a.nq * S * a.K > partial_ws_words) S /= 2;
     const uint32_t chunk = S > 1 ? (n + S - 1) / S : (n ? n : 1);
+    const bool queue = !buf_in_lds(kind) && D <= kBfQueueMaxD && bf_queue_pays(chunk, a.K);
+    BfFn fn = nullptr;
+    switch (D) {
+        case 2: fn = bf_pick<2>(kind, queue); break;
+        case 3: fn = bf_pick<3>(kind, queue); break;
+        case 4: fn = bf_pick<4>(kind, queue); break;
+        case 5: fn = bf_pick<5>(kind, queue); break;
+        case 6: fn = bf_pick<6>(kind, queue); break;
+        case 8: fn = bf_pick<8>(kind, queue); break;
+        case 16: fn = bf_pick<16>(kind, queue); break;
+        case 32: fn = bf_pick<32>(kind, queue); break;
+        case 64: fn = bf_pick<64>(kind, queue); break;
+        case 128: fn = bf_pick<128>(kind, queue); break;
+        default: break;
+    }
+    size_t lds = (size_t)tile_pts * DP * 4;
+    if (!fn) lds += (size_t)BS * D * 4 + 8;
+    if (buf_in_lds(kind)) lds += (size_t)BS * a.K * 8;
+    if (fn && queue) lds += bf_queue_bytes(BS);
+    require(lds <= device_limits().max_lds_per_block,
+            "LDS request of " + std::to_string(lds) + " bytes exceeds the per-workgroup limit of " +
+                std::to_string(device_limits().max_lds_per_block) + " (reduce BLOCK_SIZE or K)",
+            KNN_ERR_INVALID_ARG);
+    const uint32_t grid = (a.nq + BS - 1) / BS;
     uint64_t* partial = nullptr;
     if (S > 1 && partial_ws) {
         partial = partial_ws;
     } else if (S > 1) {
-        const size_t words = (size_t)a.nq * S * a.K;
-        h.q_scratch.ensure(words * 8);
+        h.q_scratch.ensure((size_t)a.nq * S * a.K * 8);
         partial = reinterpret_cast<uint64_t*>(h.q_scratch.get());
     }
     EventTimer timer(a.timed, a.stream);
@@ -332,11 +469,7 @@ void query_bf(Handle& h, const QueryArgs& a, uint64_t* partial_ws, size_t partia
         hipLaunchKernelGGL(rf, dim3(grid, S), dim3(BS), lds, a.stream, h.pts.get(), n, D, a.q, a.nq, a.maxd2, a.K,
                            a.oidx, a.odist, tile_pts, chunk, partial);
     }
-    if (S > 1) {
-        const uint32_t mb = 128;
-        hipLaunchKernelGGL(bf_merge_kernel, dim3((a.nq + mb - 1) / mb), dim3(mb), (size_t)mb * S * 2, a.stream,
-                           partial, a.nq, S, a.K, a.heap, a.oidx, a.odist);
-    }
+    if (S > 1) launch_bf_merge(partial, a.nq, S, a.K, a.heap, a.oidx, a.odist, a.stream, 1, a.nq);
     KNN_CHECK_LAUNCH();
     h.stats[4] = timer.stop();
 }

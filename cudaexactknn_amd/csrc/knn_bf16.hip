@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) bf16_rerank_lists_kernel(Bf16Set S, uint6
 }
 
 void launch_bf_merge(const uint64_t* partial, uint32_t nq, uint32_t S, uint32_t K, bool descending, int32_t* oidx,
-                     float* odist, hipStream_t st);
+                     float* odist, hipStream_t st, size_t qs, size_t ts);
 
 // Sum of n u32 into *out (grid-stride, one atomic per thread with a nonzero sum).
 __global__ void sum_u32_kernel(const uint32_t* __restrict__ v, size_t n, unsigned long long* __restrict__ out) {
@@ -335,7 +335,7 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     const size_t lanes = (size_t)a.nq * lists;
     hipLaunchKernelGGL(rerank_for(a.K), dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, a.stream, S, partial,
                        ovf_flag, ovf, novf);
-    launch_bf_merge(partial, a.nq, lists, a.K, a.heap, a.oidx, a.odist, a.stream);
+    launch_bf_merge(partial, a.nq, lists, a.K, a.heap, a.oidx, a.odist, a.stream, (size_t)lists * a.K, 1);
     KNN_CHECK_LAUNCH();
     // Overflowed queries are answered exactly on the device (their merged
     // rows are overwritten), so the call never waits for the host.

@@ -204,12 +204,13 @@ struct RegArray {
         for (int i = 0; i < KCAP; ++i) s[i] = s[i] == from ? to : s[i];
         return s[KCAP - 1];
     }
-    // Raw keys, ascending (partial results of a split scan).
-    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t K) {
+    // Raw keys, ascending (partial results of a split scan), key j at
+    // ok[j * stride].
+    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t K, size_t stride = 1) {
 #pragma unroll
         for (int i = 0; i < KCAP; ++i) {
             int j = i - (KCAP - (int)K);
-            if (j >= 0) ok[j] = s[i];
+            if (j >= 0) ok[(size_t)j * stride] = s[i];
         }
     }
     __device__ __forceinline__ void write(int32_t* __restrict__ oi, float* __restrict__ od, uint32_t K) {
@@ -260,8 +261,8 @@ struct LdsArray {
             od[i] = u2f(key_hi(v));
         }
     }
-    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t) {
-        for (uint32_t i = 0; i < K; ++i) ok[i] = s[i];
+    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t, size_t stride = 1) {
+        for (uint32_t i = 0; i < K; ++i) ok[(size_t)i * stride] = s[i];
     }
 };
 
@@ -303,9 +304,9 @@ struct Heap {
             sift(last, 0, n - 1);
         }
     }
-    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t) {
+    __device__ __forceinline__ void write_keys(uint64_t* __restrict__ ok, uint32_t, size_t stride = 1) {
         sort_ascending();
-        for (uint32_t i = 0; i < K; ++i) ok[i] = s[i];
+        for (uint32_t i = 0; i < K; ++i) ok[(size_t)i * stride] = s[i];
     }
     // Emit descending (canonical max-heap order).
     __device__ __forceinline__ void write(int32_t* __restrict__ oi, float* __restrict__ od, uint32_t) {
