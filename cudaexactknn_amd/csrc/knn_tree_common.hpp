@@ -318,10 +318,19 @@ __device__ __forceinline__ void packet_scan4(const float* __restrict__ leaves, u
             any |= pass[j];
         }
         if (any) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (pass[j]) {
-                    const uint64_t key = make_key(d2[j], __float_as_uint(v[j * S]));
+            // Compacted insertions: each round inserts every lane's first
+            // pending key, so the batch costs as many wave-wide insertions
+            // as the busiest lane has passes, not as many as there are slots
+            // that some lane passed.  The buffer ends as the K smallest of
+            // itself and the inserted keys in any order.
+            uint32_t pm = (pass[0] ? 1u : 0u) | (pass[1] ? 2u : 0u) | (pass[2] ? 4u : 0u) | (pass[3] ? 8u : 0u);
+            while (__ballot(pm != 0u) != 0ull) {
+                if (pm != 0u) {
+                    const bool b0 = (pm & 1u) != 0u, b1 = (pm & 2u) != 0u, b2 = (pm & 4u) != 0u;
+                    const float dk = b0 ? d2[0] : b1 ? d2[1] : b2 ? d2[2] : d2[3];
+                    const float ik = b0 ? v[0] : b1 ? v[S] : b2 ? v[2 * S] : v[3 * S];
+                    pm &= pm - 1u;
+                    const uint64_t key = make_key(dk, __float_as_uint(ik));
                     if (key < worst && (!FLOOR || key >= floor)) {
                         worst = buf.consider(key);
                         whi = key_hi(worst);
