@@ -222,26 +222,34 @@ def leg_learned(dev):
         tree = knn.KdTreeGpu(p, depth_skips=0) if kind == "kd" else knn.BallTreeGpu(p, depth_skips=1)
         tr = tree.train_radius(torch.from_numpy(sample).to(dev), K)
         t_plain, _, r_plain = _timed_query(tree, q, nq, K, dev, reps=1)
+        # The learned form: queries predicted far from the data ordered after
+        # the others; and with the predicted radius as each traversal's
+        # starting bound as well (learned_radius).
         t_l, s_l, r_l = _timed_query(tree, q, nq, K, dev, reps=2, form=knn.FORM_LEARNED)
+        t_r, s_r, r_r = _timed_query(tree, q, nq, K, dev, reps=2, form=knn.FORM_LEARNED, learned_radius=True)
         # Radius statistics from an untimed call.
         tree.batchKnn(knn.Query(q, nq, float("inf"), *_rows(nq, K, dev)), K, form=knn.FORM_LEARNED,
-                      learned_stats=True)
-        ratio = tree.last_query_stats()["radius_ratio"]
+                      learned_stats=True, learned_radius=True)
+        st = tree.last_query_stats()
+        ratio, far = st["radius_ratio"], st["far_fraction"]
         # The plain HIP path of this configuration: the reference's per-lane
         # traversal on the same tree (one timed call; the kd one runs ~17 s).
         t_lane, _, r_lane = _timed_query(tree, q, nq, K, dev, reps=1, per_lane=True)
         out[kind] = {"plain_qps": round(nq / t_plain * 1e3, 1), "learned_qps": round(nq / t_l * 1e3, 1),
                      "speedup": round(t_plain / t_l, 3), "identical": _same(r_plain, r_l),
+                     "far_fraction": round(far, 4),
+                     "with_radius_qps": round(nq / t_r * 1e3, 1), "with_radius_speedup": round(t_plain / t_r, 3),
+                     "with_radius_identical": _same(r_plain, r_r),
                      "per_lane_qps": round(nq / t_lane * 1e3, 1),
                      "wave_coop_vs_per_lane": round(t_lane / t_plain, 3),
                      "learned_vs_per_lane": round(t_lane / t_l, 3),
                      "per_lane_identical": _same(r_plain, r_lane),
-                     "short_rate": round(s_l["rerun_queries"] / nq, 5),
+                     "short_rate_with_radius": round(s_r["rerun_queries"] / nq, 5),
                      "mean_predicted_over_exact_radius2": round(ratio, 4),
                      "train_ms": round(tr["train_ms"], 1), "build_ms": round(tree.build_ms(), 2)}
-        log(f"learned {kind}: wave-cooperative {t_plain:.1f} ms, learned {t_l:.1f} ms, short "
-            f"{s_l['rerun_queries']:.0f}, per-lane {t_lane:.1f} ms, identical {out[kind]['identical']} "
-            f"{out[kind]['per_lane_identical']}")
+        log(f"learned {kind}: wave-cooperative {t_plain:.1f} ms, learned {t_l:.1f} ms (far {far:.3f}), with radius "
+            f"{t_r:.1f} ms (short {s_r['rerun_queries']:.0f}), per-lane {t_lane:.1f} ms, identical "
+            f"{out[kind]['identical']} {out[kind]['with_radius_identical']} {out[kind]['per_lane_identical']}")
         tree.close()
     return out
 

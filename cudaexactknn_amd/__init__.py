@@ -179,7 +179,7 @@ class _Structure:
         _lib.check(_lib.load().knn_last_query_stats(self._h, buf))
         return {"rerun_queries": buf[0], "bf16_survivor_fraction": buf[1], "radius_ratio": buf[2],
                 "query_ms": buf[3], "kernel_ms": buf[4], "node_bytes": buf[5], "leaf_slots": buf[6],
-                "order_ms": buf[7], "bf16_seeded": buf[5]}
+                "order_ms": buf[7], "bf16_seeded": buf[5], "far_fraction": buf[1]}
 
     def close(self) -> None:
         if self._h and self._h.value:
@@ -200,7 +200,8 @@ class _Structure:
 
 
 def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, leaf_span=0, stream=None,
-          count_visits=False, per_lane=False, morton=False, xcd_remap=True, tile=0, learned_stats=False):
+          count_visits=False, per_lane=False, morton=False, xcd_remap=True, tile=0, learned_stats=False,
+          learned_radius=False):
     o = QueryOpts()
     o.block = int(BLOCK_SIZE or 0)
     o.heap = int(bool(heap))
@@ -210,7 +211,8 @@ def _opts(BLOCK_SIZE, heap, track, stack, form=FORM_PLAIN, sort_queries=True, le
     o.flags = ((0 if sort_queries else _lib.KNN_FLAG_NO_QUERY_SORT) | (_lib.KNN_FLAG_COUNT_VISITS if count_visits else 0)
                | (_lib.KNN_FLAG_PER_LANE if per_lane else 0) | (_lib.KNN_FLAG_MORTON_ORDER if morton else 0)
                | (0 if xcd_remap else _lib.KNN_FLAG_NO_XCD_REMAP)
-               | (_lib.KNN_FLAG_LEARNED_STATS if learned_stats else 0))
+               | (_lib.KNN_FLAG_LEARNED_STATS if learned_stats else 0)
+               | (_lib.KNN_FLAG_LEARNED_RADIUS if learned_radius else 0))
     o.leaf_span = int(leaf_span)
     o.tile = int(tile)
     o.stream = stream
@@ -242,12 +244,14 @@ class _Tree(_Structure):
                  USE_TRACKING_SHMEM: bool = False, USE_STACK_SHMEM: bool = False, form: int = FORM_PLAIN,
                  sort_queries: bool = True, leaf_span: int = 0, stream=None, count_visits: bool = False,
                  per_lane: bool = False, morton: bool = False, xcd_remap: bool = True,
-                 learned_stats: bool = False) -> None:
+                 learned_stats: bool = False, learned_radius: bool = False) -> None:
         """batchKnn<K, BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM>(Query&)
-        (src/KdTreeGpu.cuh:393-410, src/BallTreeGpu.cuh:631-648)."""
+        (src/KdTreeGpu.cuh:393-410, src/BallTreeGpu.cuh:631-648).  Learned form
+        (form=FORM_LEARNED): `learned_radius` also starts each traversal from
+        the predicted radius."""
         self._query(query, K, _opts(BLOCK_SIZE, USE_HEAP_TRACKING, USE_TRACKING_SHMEM, USE_STACK_SHMEM, form,
                                     sort_queries, leaf_span, stream, count_visits, per_lane, morton, xcd_remap,
-                                    learned_stats=learned_stats))
+                                    learned_stats=learned_stats, learned_radius=learned_radius))
 
     def build_ms(self) -> float:
         return float(_lib.load().knn_last_build_ms(self._h))

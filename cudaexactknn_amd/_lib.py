@@ -26,6 +26,7 @@ KNN_FLAG_PER_LANE = 0x4
 KNN_FLAG_MORTON_ORDER = 0x8
 KNN_FLAG_NO_XCD_REMAP = 0x10
 KNN_FLAG_LEARNED_STATS = 0x20
+KNN_FLAG_LEARNED_RADIUS = 0x40
 
 # Every symbol include/knn_mi355x.h declares.
 EXPORTED = (

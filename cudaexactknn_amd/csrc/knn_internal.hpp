@@ -198,6 +198,7 @@ struct QueryArgs {
     bool heap, track_lds, stack_lds, sort_queries, count_visits, per_lane, order_morton;
     bool xcd_remap = true;  // trees: XCD-contiguous block order
     bool learned_stats = false;  // learned form: compute the radius statistics after the query
+    bool learned_radius = false; // learned form: start traversals from the predicted radius
     uint32_t leaf_span;  // extra levels folded into one leaf scan
     uint32_t tile = 0;   // bf16 form: point tiles staged per barrier (0 = default)
     knn_form form;
@@ -309,7 +310,10 @@ void bf16_reserve(Handle& h, uint32_t nq, uint32_t K, uint32_t tile, hipStream_t
 void query_tree(Handle& h, const QueryArgs& a, const uint32_t* d_perm, const float* d_init_r2,
                 uint32_t* d_short_flags, uint32_t* d_miss_count = nullptr);  // knn_tree_host.hip
 bool learned_in_kernel(const Handle& h, const QueryArgs& a, bool ordered);
-void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm);      // knn_tree_host.hip
+// `major` (optional): a class per query of `major_bits` bits, sorted ahead
+// of the spatial key.
+void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm, const uint32_t* major = nullptr,
+                   uint32_t major_bits = 0);  // knn_tree_host.hip
 // Query ordering on caller-provided scratch: keys, values, their sort
 // alternates (nq words each) and the radix sort's temporary words.
 struct OrderScratch {
@@ -317,7 +321,8 @@ struct OrderScratch {
 };
 size_t order_scratch_words(uint32_t nq);                      // u32 words for nq queries
 OrderScratch order_scratch_at(uint32_t* base, uint32_t nq);  // carve one block of that size
-void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& s, uint32_t** d_perm);
+void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& s, uint32_t** d_perm,
+                        const uint32_t* major = nullptr, uint32_t major_bits = 0);
 
 // Learned radius (knn_learned.hip)
 void learned_train(Handle& h, const float* d_sample, uint32_t n_sample, uint32_t k, uint32_t epochs, float lr,

@@ -12,12 +12,18 @@
 // Training runs on the GPU in FP32 with the kernels below (forward, backward,
 // deterministic fixed-order gradient sums, Adam) on exact radii that the
 // plain traversal computes for sample queries.  Inference uses bf16 weights
-// and bf16 activations with FP32 accumulation.  The prediction, scaled by a
-// calibration factor (the 98th percentile of true / predicted radius on the
-// training sample), only tightens the starting K-th key of the traversal:
-// every point inside the radius is still found exactly, and a query that ends
-// with an empty slot is recomputed by the plain traversal, so results never
-// change.
+// and bf16 activations with FP32 accumulation.
+//
+// The prediction schedules the batch: queries predicted far from the data
+// (kFarLog2) are ordered after the others, so that the wave-cooperative
+// traversal, whose waves take the union of their 64 queries' search regions,
+// does not give every near query's wave a far query's region.  With
+// KNN_FLAG_LEARNED_RADIUS the prediction, scaled per calibration bucket
+// (kCalibrationQuantile of true / predicted radius on held-out samples),
+// also tightens the starting K-th key of each traversal: every point inside
+// the radius is still found exactly, and a query that ends with an empty slot
+// is searched again, so results never change.  Either way the rows are the
+// plain traversal's.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -37,6 +43,10 @@ constexpr int F_IN = 12;
 // query the radius leaves short costs a second traversal, an overshoot only
 // some pruning, so the quantile is high.
 constexpr int kCalBuckets = 8;
+// Far queries: predicted log2 r^2 more than kFarLog2 above the median
+// prediction over the calibration sample (r more than 8 times the median).
+// The learned form orders them after the others, so they share waves.
+constexpr double kFarLog2 = 6.0;
 constexpr double kCalibrationQuantile = 0.999;
 // A bucket whose errors spread wider than this (calibration quantile over
 // median of true / predicted) gets no predicted radius at all: its queries
@@ -56,12 +66,14 @@ struct Mlp {
     DevBuf<float> params;      // FP32 master weights
     DevBuf<float> norm;        // feature mean[12], std[12], target mean, std
     float scale = 1.0f;        // calibration factor on the predicted r^2
+    float far_log2 = INFINITY; // predicted log2 r^2 above which a query is far
     uint32_t K = 0;
     bool trained = false;
     // Query-time buffers (grow-only): features, landing-subtree bound,
     // starting radius^2, and the in-kernel second-pass counter.
     DevBuf<float> feats, ub, r2;
     DevBuf<uint32_t> miss;
+    DevBuf<uint32_t> cls;      // per query: calibration bucket of its predicted radius
     // Calibration: bucket edges on the predicted log2 r^2 (kCalBuckets - 1)
     // followed by the bucket scales (kCalBuckets).
     DevBuf<float> calib;
@@ -82,11 +94,14 @@ __device__ __forceinline__ float safe_log2(float x) { return log2f(fmaxf(x, 1e-3
 // the query's landing subtree, or +inf when it holds fewer than K: an upper
 // bound on the query's K-th squared distance (a subset's K-th is never below
 // the whole set's), so a radius capped by it never leaves a slot empty.
+// Thread t computes the features of query perm[t] (A.perm, when given: a
+// spatial order, so that neighbouring lanes scan neighbouring subtrees).
 __global__ void feature_kernel(TreeArgs A, int is_ball, const float* __restrict__ bbox, float* __restrict__ feats,
                                float* __restrict__ ub) {
 #pragma clang fp contract(off)
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.nq) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.nq) return;
+    const uint32_t i = A.perm ? A.perm[t] : t;
     const uint32_t D = A.dims;
     const float* q = A.qs + (size_t)i * D;
     float* f = feats + (size_t)i * F_IN;
@@ -175,7 +190,8 @@ __global__ void feature_kernel(TreeArgs A, int is_ball, const float* __restrict_
 __global__ void mlp_forward_kernel(const float* __restrict__ params, int use_bf16, const float* __restrict__ norm,
                                    const float* __restrict__ x, uint32_t n, float scale, float* __restrict__ out,
                                    float* __restrict__ r2out, const float* __restrict__ ub = nullptr,
-                                   const float* __restrict__ calib = nullptr) {
+                                   const float* __restrict__ calib = nullptr, uint32_t* __restrict__ cls = nullptr,
+                                   float far_log2 = INFINITY) {
 #pragma clang fp contract(off)
     __shared__ float w[N_PARAMS];
     for (uint32_t t = threadIdx.x; t < (uint32_t)N_PARAMS; t += blockDim.x) {
@@ -210,13 +226,14 @@ __global__ void mlp_forward_kernel(const float* __restrict__ params, int use_bf1
     if (out) out[i] = y;
     if (r2out) {
         float sc = scale;
+        int b = 0;
         if (calib) {
-            int b = 0;
             for (int e = 0; e < kCalBuckets - 1; ++e) b += y > calib[e] ? 1 : 0;
             sc = calib[kCalBuckets - 1 + b];
         }
         const float r = sc * exp2f(y);
         r2out[i] = ub ? fminf(r, ub[i]) : r;
+        if (cls) cls[i] = y > far_log2 ? 1u : 0u;
     }
 }
 
@@ -342,7 +359,7 @@ static uint32_t prime_level_for(const knn_tree_info& I, uint32_t K) {
 }
 
 static void compute_features(Handle& h, const float* d_q, uint32_t nq, uint32_t K, float* d_feats, float* d_ub,
-                             hipStream_t st) {
+                             hipStream_t st, const uint32_t* d_perm = nullptr) {
     const knn_tree_info& I = h.info;
     TreeArgs A{};
     A.nodes = h.nodes.get();
@@ -354,6 +371,7 @@ static void compute_features(Handle& h, const float* d_q, uint32_t nq, uint32_t 
     A.prime_level = prime_level_for(I, K);
     A.dims = I.dims;
     A.qs = d_q;
+    A.perm = d_perm;
     A.nq = nq;
     A.K = K;
     hipLaunchKernelGGL(feature_kernel, dim3((nq + 127) / 128), dim3(128), 0, st, A, h.kind == KNN_KIND_BALL ? 1 : 0,
@@ -440,7 +458,7 @@ void learned_train(Handle& h, const float* d_sample, uint32_t ns, uint32_t K, ui
         while (((uint64_t)h.info.max_children << s) < target_pts && s < h.info.node_depth) ++s;
         a.leaf_span = s;
     }
-    a.order_morton = h.kind == KNN_KIND_BALL;
+    a.order_morton = h.kind == KNN_KIND_BALL && h.info.dims <= 6;  // as knn_query's default
     a.stream = st;
     uint32_t* perm = nullptr;
     order_queries(h, a, &perm);
@@ -569,6 +587,7 @@ void learned_train(Handle& h, const float* d_sample, uint32_t ns, uint32_t K, ui
         for (auto& v : pr) r.push_back(v.second);
         M.scale = (float)quantile(r);
     }
+    M.far_log2 = (float)(pr[nc / 2].first + kFarLog2);  // pr is sorted by prediction
     std::vector<float> cal(2 * kCalBuckets - 1);
     for (int b = 0; b < kCalBuckets; ++b) {
         const size_t lo = (size_t)nc * b / kCalBuckets, hi = (size_t)nc * (b + 1) / kCalBuckets;
@@ -598,6 +617,7 @@ void learned_reserve(Handle& h, uint32_t nq) {
     M.feats.ensure((size_t)nq * F_IN);
     M.ub.ensure(nq);
     M.r2.ensure(nq);
+    M.cls.ensure(nq);
 }
 
 void query_tree_learned(Handle& h, const QueryArgs& a) {
@@ -614,16 +634,36 @@ void query_tree_learned(Handle& h, const QueryArgs& a) {
     M.feats.ensure((size_t)a.nq * F_IN);
     M.ub.ensure(a.nq);
     M.r2.ensure(a.nq);
-    compute_features(h, a.q, a.nq, a.K, M.feats.get(), M.ub.get(), a.stream);
+    M.cls.ensure(a.nq);
+    // Features in the spatial order (each lane scans its landing subtree,
+    // neighbouring lanes neighbouring subtrees), then the batch order.
+    uint32_t* perm0 = nullptr;
+    float order0_ms = 0.0f;
+    if (a.sort_queries && a.nq > 1) {
+        order_queries(h, a, &perm0);
+        order0_ms = h.stats[7];
+    }
+    compute_features(h, a.q, a.nq, a.K, M.feats.get(), M.ub.get(), a.stream, perm0);
     hipLaunchKernelGGL(mlp_forward_kernel, dim3((a.nq + 127) / 128), dim3(128), 0, a.stream, M.params.get(), 1,
-                       M.norm.get(), M.feats.get(), a.nq, M.scale, nullptr, M.r2.get(), M.ub.get(), M.calib.get());
+                       M.norm.get(), M.feats.get(), a.nq, M.scale, nullptr, M.r2.get(), M.ub.get(), M.calib.get(),
+                       M.cls.get(), M.far_log2);
     KNN_CHECK_LAUNCH();
+    // The batch is ordered far queries last, each class by tree cell: a
+    // query predicted far from the data (a radius many times the typical
+    // one) then shares waves with other far queries instead of joining the
+    // near queries of its landing cell, whose wave would take its large
+    // search region.
     uint32_t* perm = nullptr;
-    if (a.sort_queries && a.nq > 1) order_queries(h, a, &perm);
-    const float order_ms = h.stats[7];
+    if (a.sort_queries && a.nq > 1) order_queries(h, a, &perm, M.cls.get(), 1);
+    const float order_ms = order0_ms + h.stats[7];
     float kernel_ms = 0.0f;
     uint32_t n_miss = 0;
-    if (learned_in_kernel(h, a, perm != nullptr)) {
+    if (!a.learned_radius) {
+        // The plain traversal in the predicted order (no starting radius).
+        query_tree(h, a, perm, nullptr, nullptr);
+        kernel_ms = h.stats[4];
+        h.stats[3] = total.stop();
+    } else if (learned_in_kernel(h, a, perm != nullptr)) {
         // Queries the radius leaves short are searched again inside the
         // traversal kernel: no host round trip, no allocation.
         launch_fill_u32(M.miss.get(), 1, 0u, a.stream);
@@ -701,6 +741,12 @@ void query_tree_learned(Handle& h, const QueryArgs& a) {
                 ++c;
             }
         h.stats[2] = c ? (float)(s / c) : 0.0f;
+        // [1]: the share of queries ordered as far.
+        std::vector<uint32_t> hc(a.nq);
+        KNN_HIP(hipMemcpy(hc.data(), M.cls.get(), 4ull * a.nq, hipMemcpyDeviceToHost));
+        size_t nfar = 0;
+        for (uint32_t v : hc) nfar += v;
+        h.stats[1] = (float)((double)nfar / a.nq);
     }
 }
 

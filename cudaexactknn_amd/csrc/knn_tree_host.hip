@@ -82,9 +82,17 @@ __global__ void ball_order_kernel(const float* __restrict__ qs, uint32_t nq, uin
     vals[i] = i;
 }
 
+// keys[i] = (major[i] << bits) | (keys[i] >> drop): the class ahead of the
+// spatial key (its low `drop` bits dropped to fit 32 bits).
+__global__ void major_key_kernel(uint32_t* __restrict__ keys, const uint32_t* __restrict__ major, uint32_t nq,
+                                 uint32_t bits, uint32_t drop) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) keys[i] = (major[i] << bits) | (keys[i] >> drop);
+}
+
 size_t order_scratch_words(uint32_t nq) { return 4 * (size_t)nq + radix_temp_words(nq); }
 
-void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
+void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm, const uint32_t* major, uint32_t major_bits) {
     const uint32_t n = a.nq;
     h.q_keys.ensure(n);
     h.q_vals.ensure(n);
@@ -92,14 +100,15 @@ void order_queries(Handle& h, const QueryArgs& a, uint32_t** d_perm) {
     h.q_vals_alt.ensure(n);
     h.q_temp.ensure(radix_temp_words(n));
     OrderScratch s{h.q_keys.get(), h.q_vals.get(), h.q_keys_alt.get(), h.q_vals_alt.get(), h.q_temp.get()};
-    order_queries_into(h, a, s, d_perm);
+    order_queries_into(h, a, s, d_perm, major, major_bits);
 }
 
 OrderScratch order_scratch_at(uint32_t* base, uint32_t nq) {
     return OrderScratch{base, base + nq, base + 2 * (size_t)nq, base + 3 * (size_t)nq, base + 4 * (size_t)nq};
 }
 
-void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& S, uint32_t** d_perm) {
+void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& S, uint32_t** d_perm,
+                        const uint32_t* major, uint32_t major_bits) {
     const uint32_t n = a.nq, D = h.info.dims;
     EventTimer timer(a.timed, a.stream);
     int bits;
@@ -126,6 +135,12 @@ void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& S, ui
         hipLaunchKernelGGL(ball_order_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D,
                            h.nodes.get(), tree_levels, S.keys, S.vals);
         bits = (int)tree_levels;
+    }
+    if (major && major_bits) {
+        const uint32_t drop = (uint32_t)bits + major_bits > 32 ? (uint32_t)bits + major_bits - 32 : 0u;
+        hipLaunchKernelGGL(major_key_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, S.keys, major, n,
+                           (uint32_t)bits - drop, drop);
+        bits = bits - (int)drop + (int)major_bits;
     }
     const int end = (bits + 7) / 8 * 8;
     bool alt = radix_sort_pairs(S.keys, S.vals, S.keys_alt, S.vals_alt, n, 0, end, S.temp, a.stream);

@@ -79,6 +79,8 @@ typedef enum knn_form { KNN_FORM_PLAIN = 0, KNN_FORM_BF16 = 1, KNN_FORM_LEARNED 
                                         (ball trees always use Morton order) */
 #define KNN_FLAG_PER_LANE 0x4u       /* trees: one independent traversal per lane (the reference's
                                         shape) instead of the wave-cooperative traversal */
+#define KNN_FLAG_LEARNED_RADIUS 0x40u /* learned form: also start each query's traversal from its
+                                         predicted radius (short queries searched again) */
 #define KNN_FLAG_LEARNED_STATS 0x20u /* learned form, synchronous calls: after the query, compute the
                                         mean predicted / exact radius ratio (stats[2]) */
 
@@ -241,8 +243,10 @@ knn_status knn_learned_train(void* handle, const float* d_sample, uint32_t n_sam
                              uint32_t epochs, float lr, float* stats);
 
 /* Statistics of the last synchronous knn_query on this handle:
- * [0] queries the learned radius left short, searched again (learned form),
- * [1] BF16 survivor fraction (bf16 form), [2] mean predicted / exact radius^2
+ * [0] queries the learned radius left short, searched again (learned form
+ * with KNN_FLAG_LEARNED_RADIUS), [1] BF16 survivor fraction (bf16 form) or
+ * the share of queries ordered as far (learned form with
+ * KNN_FLAG_LEARNED_STATS), [2] mean predicted / exact radius^2
  * ratio (learned form with KNN_FLAG_LEARNED_STATS), [3] ms of the whole
  * query, [4] ms of the main
  * kernel (traversal or scan), [5] node bytes read and [6] leaf slots read

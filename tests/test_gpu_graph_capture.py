@@ -44,7 +44,7 @@ def _case_points(kind, seed):
         v = g.standard_normal((6000, 32))
         return (v / np.linalg.norm(v, axis=1, keepdims=True) * 4.0).astype(np.float32), g
     n, d = {"kd": (60000, 3), "ball": (40000, 4), "bf": (80000, 3), "bf16": (65536, 32),
-            "learned": (60000, 3), "noalloc": (60000, 3)}[kind]
+            "learned": (60000, 3), "learned_radius": (60000, 3), "noalloc": (60000, 3)}[kind]
     pts = (g.standard_normal((n, d)) * np.exp(g.uniform(-0.5, 0.5, d))).astype(np.float32)
     pts[7] = pts[8]
     return pts, g
@@ -69,6 +69,7 @@ CASES = [
     ("bf16", "bf", _lib.KNN_FORM_BF16, 32, 0, 512, 100),
     ("overflow", "bf", _lib.KNN_FORM_BF16, 32, 1, 64, 9),
     ("learned", "kd", _lib.KNN_FORM_LEARNED, 16, 0, 4096, 1000),
+    ("learned_radius", "kd", _lib.KNN_FORM_LEARNED, 16, 0, 4096, 1000),
     ("noalloc", "kd", _lib.KNN_FORM_PLAIN, 16, 0, 4096, 1000),
 ]
 
@@ -89,14 +90,15 @@ def test_captured_query_replays_match_oracle(oracle, cuda, case):
     L = _lib.load()
     fn = L.knn_query_noalloc if name == "noalloc" else L.knn_query
     try:
-        if name == "learned":
+        if name.startswith("learned"):
             t.train_radius(torch.from_numpy(_queries(name, pts, g, 20000)).to(cuda), k)
+        flags = _lib.KNN_FLAG_LEARNED_RADIUS if name == "learned_radius" else 0
         stream = torch.cuda.Stream(device=cuda)
         opts = _lib.QueryOpts()
-        opts.form, opts.heap = form, heap
+        opts.form, opts.heap, opts.flags = form, heap, flags
         opts.stream = stream.cuda_stream
         sync = _lib.QueryOpts()
-        sync.form, sync.heap = form, heap
+        sync.form, sync.heap, sync.flags = form, heap, flags
         _lib.check(L.knn_query_reserve(t._h, nq, k, ctypes.byref(opts)))
         for m in (nq, nq_small):
             print(f"[capture] {name} nq={m} k={k} heap={heap}", flush=True)

@@ -57,19 +57,29 @@ def test_learned_form_is_exact(oracle, cuda, kind):
     print(kind, "training loss", st["loss"])
     assert np.isfinite(st["loss"]) and st["loss"] < 0.5
     want = oracle.bf_knn(pts, qs, 32)
-    for heap in (False, True):
-        w = want if not heap else (want[0][:, ::-1].copy(), want[1][:, ::-1].copy())
-        got = run(t, qs, 32, cuda, USE_HEAP_TRACKING=heap, form=knn.FORM_LEARNED)
-        same(got, w, f"learned {kind} heap={heap}")
-    got = run(t, qs, 32, cuda, form=knn.FORM_LEARNED, learned_stats=True)
-    same(got, want, f"learned {kind} with statistics")
-    s = t.last_query_stats()
-    print(kind, "short queries", s["rerun_queries"], "radius ratio", s["radius_ratio"])
-    assert s["rerun_queries"] < 0.05 * len(qs)
-    assert s["radius_ratio"] >= 1.0 - 1e-6 or s["rerun_queries"] > 0
-    # The per-lane traversal takes the other path (short queries flagged and
-    # re-run by the plain traversal): same rows.
-    same(run(t, qs, 32, cuda, form=knn.FORM_LEARNED, per_lane=True), want, f"learned {kind} per-lane")
+    for radius in (False, True):
+        for heap in (False, True):
+            w = want if not heap else (want[0][:, ::-1].copy(), want[1][:, ::-1].copy())
+            got = run(t, qs, 32, cuda, USE_HEAP_TRACKING=heap, form=knn.FORM_LEARNED, learned_radius=radius)
+            same(got, w, f"learned {kind} heap={heap} radius={radius}")
+        got = run(t, qs, 32, cuda, form=knn.FORM_LEARNED, learned_stats=True, learned_radius=radius)
+        same(got, want, f"learned {kind} with statistics radius={radius}")
+        s = t.last_query_stats()
+        print(kind, "radius", radius, "short queries", s["rerun_queries"], "radius ratio", s["radius_ratio"],
+              "far", s["far_fraction"])
+        # 20 % of the queries are uniform over the bounding box, far from
+        # the thin clusters: the predictor orders most of them as far, and
+        # few mixture queries.
+        assert 0.1 < s["far_fraction"] < 0.4
+        if radius:
+            assert s["rerun_queries"] < 0.05 * len(qs)
+            assert s["radius_ratio"] >= 1.0 - 1e-6 or s["rerun_queries"] > 0
+        else:
+            assert s["rerun_queries"] == 0
+        # The per-lane traversal takes the other path (with the radius, short
+        # queries flagged and re-run by the plain traversal): same rows.
+        same(run(t, qs, 32, cuda, form=knn.FORM_LEARNED, per_lane=True, learned_radius=radius), want,
+             f"learned {kind} per-lane radius={radius}")
 
 
 @pytest.mark.parametrize("kind", ["kd", "ball"])
@@ -92,7 +102,7 @@ def test_learned_second_pass_keeps_rows_exact(oracle, cuda, kind):
     for md in (np.inf, 40.0, -1.0):
         for heap in (False, True):
             want = oracle.bf_knn(pts, qs, 16, max_d2=md, heap=heap)
-            got = run(t, qs, 16, cuda, md=md, USE_HEAP_TRACKING=heap, form=knn.FORM_LEARNED)
+            got = run(t, qs, 16, cuda, md=md, USE_HEAP_TRACKING=heap, form=knn.FORM_LEARNED, learned_radius=True)
             same(got, want, f"{kind} md={md} heap={heap}")
             if md == np.inf and not heap:
                 assert t.last_query_stats()["rerun_queries"] > 100
