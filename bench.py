@@ -148,6 +148,25 @@ def cpu_baseline(pts, qs, gpu_idx, gpu_dist, sample: int, gpu_rows: dict):
     }
 
 
+def build_roofline(n: int, d: int, kd_info, kd_ms: float, ball_info, ball_ms: float) -> dict:
+    """Builds against the HBM roofline of their minimal model: 12 B per point
+    and level (index read, key gather, index write), the points read once,
+    the leaf slots (4 + 4D B) and nodes written; the ball tree adds 8D B per
+    point and level for its centroid and radius passes."""
+    def model(info, ball: bool) -> float:
+        levels = info["node_depth"]
+        per_level = 12 + (8 * d if ball else 0)
+        node_bytes = (8 * (d + 1) if ball else 4) * info["node_size"]
+        return n * per_level * levels + n * 4 * d + info["index_size"] * (4 + 4 * d) + node_bytes
+    out = {}
+    for name, info, ms, ball in (("kd", kd_info, kd_ms, False), ("ball", ball_info, ball_ms, True)):
+        b = model(info, ball)
+        gbs = b / (ms * 1e-3) / 1e9
+        out[name] = {"model_bytes": int(b), "ms": round(ms, 2), "achieved_gbs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def _same(a, b) -> bool:
     import torch
     return bool(torch.equal(a[0], b[0]) and torch.equal(a[1].view(torch.int32), b[1].view(torch.int32)))
@@ -469,6 +488,8 @@ def main() -> int:
     ball_qps = nq / ((time.perf_counter() - t0) / max(1, args.steps // 2))
     ball_same = bool(torch.equal(ri, ri2) and torch.equal(rd.view(torch.int32), rd2.view(torch.int32)))
     log(f"ball: {ball_qps / 1e6:.2f} M q/s, build {ball_build:.1f} ms, identical to kd: {ball_same}")
+    builds = build_roofline(len(pts), 3, kd.info(), min(kd_build), ball.info(), ball_build)
+    log(f"builds vs the HBM model: {builds}")
     ball.close()
 
     # Plain brute force on the small uniform configuration (65,536 x 16,384, D=3, K=8, heap).
@@ -545,6 +566,7 @@ def main() -> int:
             "kd_wave_issued_gbs": round(issued_bytes / (kavg * 1e-3) / 1e9, 1),
             "ball_qps": round(ball_qps, 1),
             "ball_build_ms": round(ball_build, 2),
+            "build_roofline": builds,
             "ball_identical_to_kd": ball_same,
             "bf_uniform_65536x16384_k8_kernel_ms": round(bf_kernel_ms, 4),
             "bf_uniform_tflops_3D_ops": round(bf_ops / bf_kernel_ms / 1e9, 2),
