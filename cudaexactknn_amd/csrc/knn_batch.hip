@@ -283,6 +283,7 @@ __global__ void __launch_bounds__(256) batch_fallback_kernel(const Bf16Set* __re
     __shared__ uint64_t s_keys[4][64 * 32];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t total = *novf;
+    if (total == 0) return;  // nothing overflowed: leave the work counter alone
     for (;;) {
         uint32_t it = 0;
         if (lane == 0) it = atomicAdd(counter, 1u);
