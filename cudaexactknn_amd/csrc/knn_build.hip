@@ -317,6 +317,193 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
     }
 }
 
+// --- kd levels: lists partitioned when their dimension splits --------------
+// A kd level only reads its split list, and only at the split rank, so a list
+// has to be in the current segmentation only at the levels where its dimension
+// splits.  A list therefore waits: one stable partition by a 1- or 2-bit code
+// moves it down k <= 2 levels at once (from the segmentation of level l - k
+// to that of level l), with four runs per segment instead of two.  The pair
+// map of level m holds, per point id, 2 bits: the left bit of level m (low)
+// and of level m - 1 (high), 16 points per word.  Segment starts and sizes are
+// the deterministic split sizes (left = ceil(n/2) from the root), so a
+// position's segment is found by descending from the root and no per-position
+// segment table is kept.
+
+__device__ __forceinline__ uint32_t spread16(uint32_t x) {
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// The wave's 64 two-bit codes (low bits `lo`, high bits `hi`, lane order) as
+// four words from word w0 (words below nwords only).
+__device__ __forceinline__ void store_codes(uint32_t* __restrict__ out, size_t w0, size_t nwords, uint64_t lo,
+                                            uint64_t hi, uint32_t lane) {
+    if (lane < 4 && w0 + lane < nwords) {
+        const uint32_t a = (uint32_t)(lo >> (16 * lane)) & 0xFFFFu;
+        const uint32_t b = (uint32_t)(hi >> (16 * lane)) & 0xFFFFu;
+        out[w0 + lane] = spread16(a) | (spread16(b) << 1);
+    }
+}
+
+__device__ __forceinline__ uint32_t code_of(const uint32_t* __restrict__ map, size_t i) {
+    return (map[i >> 4] >> (2 * (uint32_t)(i & 15u))) & 3u;
+}
+
+// Left bit per point id for split dim `dim` (rule of flag_kernel) and the
+// level's pair map; `prev` is the previous level's map (null at level 0).
+__global__ void kd_flag_kernel(const float* __restrict__ pts, uint32_t D, uint32_t N, uint32_t dim,
+                               uint32_t* __restrict__ segById, const uint32_t* __restrict__ pivKey,
+                               const uint32_t* __restrict__ pivId, const uint32_t* __restrict__ prev,
+                               uint32_t* __restrict__ pair) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool left = false, before = false;
+    if (i < N) {
+        const uint32_t s = segById[i];
+        const uint32_t key = ordered_bits(pts[(size_t)i * D + dim]);
+        const uint32_t pk = pivKey[s];
+        left = key < pk || (key == pk && i <= pivId[s]);
+        segById[i] = 2 * s + (left ? 0u : 1u);
+        before = prev && (code_of(prev, i) & 1u) != 0u;
+    }
+    const uint64_t lo = __ballot(left), hi = __ballot(before);
+    const uint32_t lane = threadIdx.x & 63u;
+    store_codes(pair, (size_t)(i - lane) >> 4, ((size_t)N + 15) / 16, lo, hi, lane);
+}
+
+// Bucket of a point for k pending levels: the order of the 2^k runs of a
+// segment (left before right at each level, the earlier level first).
+__device__ __forceinline__ uint32_t kd_bucket(uint32_t code, uint32_t k) {
+    return k == 2 ? 3u - code : 1u - (code & 1u);
+}
+
+// Buckets per list position (gathered from the pair map by point id and
+// stored in position order, so the scatter reads them coalesced) and the
+// tile's count of each bucket: cnt[c * nb + tile].
+__global__ void __launch_bounds__(PT_THREADS) kd_count_kernel(const uint32_t* __restrict__ L, uint32_t N, uint32_t k,
+                                                              const uint32_t* __restrict__ pair,
+                                                              uint32_t* __restrict__ cnt, uint32_t nb,
+                                                              uint32_t* __restrict__ pc) {
+    __shared__ uint32_t red[4][PT_THREADS / 64];
+    const size_t base = (size_t)blockIdx.x * PT_TILE;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const size_t nwords = ((size_t)N + 15) / 16;
+    uint32_t cv = 0, c1 = 0, c2 = 0, c3 = 0;  // valid items and buckets 1..3 (wave-uniform)
+    for (uint32_t i = threadIdx.x; i < PT_TILE; i += PT_THREADS) {
+        const size_t j = base + i;
+        const bool v = j < N;
+        const uint32_t c = v ? kd_bucket(code_of(pair, L[j]), k) : 0u;
+        const uint64_t b0 = __ballot(c & 1u), b1 = __ballot(c >> 1), bv = __ballot(v);
+        cv += (uint32_t)__popcll(bv);
+        c1 += (uint32_t)__popcll(b0 & ~b1);
+        c2 += (uint32_t)__popcll(~b0 & b1);
+        c3 += (uint32_t)__popcll(b0 & b1);
+        store_codes(pc, (j - lane) >> 4, nwords, b0, b1, lane);
+    }
+    if (lane == 0) {
+        red[0][wave] = cv - c1 - c2 - c3;
+        red[1][wave] = c1;
+        red[2][wave] = c2;
+        red[3][wave] = c3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        uint32_t t = 0;
+        for (int w = 0; w < PT_THREADS / 64; ++w) t += red[threadIdx.x][w];
+        cnt[(size_t)threadIdx.x * nb + blockIdx.x] = t;
+    }
+}
+
+// before[c * S' + s] = size of run c of segment s (segment (s << k) + c of
+// the level-l table n), S' = 2^(l - k) segments; exclusive-scanned afterwards
+// so that it cancels the bucket offsets of the scanned tile counts.
+__global__ void kd_runs_kernel(const uint32_t* __restrict__ segN, uint32_t total, uint32_t k,
+                               uint32_t* __restrict__ out) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const uint32_t Sp = total >> k, c = e / Sp, s = e - c * Sp;
+    out[e] = segN[(s << k) + c];
+}
+
+// Stable partition of one list by its position buckets: item j of bucket c
+// in segment s (level lk = l - k) goes to the start of run c plus its rank
+// among the segment's bucket-c items (global bucket rank from the scanned
+// tile counts, minus the bucket-c items of earlier segments).  Items are
+// taken in striped order as in part_scatter_kernel.
+__global__ void __launch_bounds__(PT_THREADS) kd_scatter_kernel(const uint32_t* __restrict__ L,
+                                                                uint32_t* __restrict__ O, uint32_t N, uint32_t k,
+                                                                uint32_t lk, const uint32_t* __restrict__ pc,
+                                                                const uint32_t* __restrict__ off, uint32_t nb,
+                                                                const uint32_t* __restrict__ before) {
+    static_assert(PT_THREADS == 256 && PT_ITEMS * (PT_THREADS / 64) == 64, "one prefix-table entry per lane");
+    __shared__ uint32_t wcnt[4][64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * PT_TILE;
+    uint32_t id[PT_ITEMS], rank[PT_ITEMS];
+    uint32_t cb = 0;  // 2 bits per item
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + (size_t)i * PT_THREADS + tid;
+        id[i] = j < N ? L[j] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + (size_t)i * PT_THREADS + tid;
+        const bool v = j < N;
+        const uint32_t c = v ? code_of(pc, j) : 0u;
+        cb |= c << (2 * i);
+        const uint64_t b0 = __ballot(c & 1u), b1 = __ballot(c >> 1), bv = __ballot(v);
+        const uint64_t m = ((c & 1u) ? b0 : ~b0) & ((c >> 1) ? b1 : ~b1) & bv;
+        rank[i] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (lane < 4) {
+            const uint64_t mt = ((lane & 1u) ? b0 : ~b0) & ((lane >> 1) ? b1 : ~b1) & bv;
+            wcnt[lane][i * (PT_THREADS / 64) + wave] = (uint32_t)__popcll(mt);
+        }
+    }
+    __syncthreads();
+    {
+        // Wave w: exclusive prefix of bucket w over the 64 (iteration, wave)
+        // slots in item order.
+        const uint32_t v = wcnt[wave][lane];
+        const uint32_t inc = wave_incl_scan(v);
+        __builtin_amdgcn_wave_barrier();
+        wcnt[wave][lane] = inc - v;
+    }
+    __syncthreads();
+    uint32_t o4[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o4[c] = off[(size_t)c * nb + blockIdx.x];
+    const uint32_t Sp = 1u << lk;
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + (size_t)i * PT_THREADS + tid;
+        if (j < N) {
+            const uint32_t c = (cb >> (2 * i)) & 3u;
+            const uint32_t g = (c == 0 ? o4[0] : c == 1 ? o4[1] : c == 2 ? o4[2] : o4[3]) +
+                               wcnt[c][i * (PT_THREADS / 64) + wave] + rank[i];
+            // Segment of position j at level lk, then the start of its run c.
+            uint32_t st = 0, n = N, sg = 0;
+            for (uint32_t b = 0; b < lk; ++b) {
+                const uint32_t Lh = (n + 1) >> 1;
+                const bool r = (uint32_t)j - st >= Lh;
+                sg = 2 * sg + (r ? 1u : 0u);
+                st = r ? st + Lh : st;
+                n = r ? n - Lh : Lh;
+            }
+            for (uint32_t b = k; b-- > 0;) {
+                const uint32_t Lh = (n + 1) >> 1;
+                const bool r = ((c >> b) & 1u) != 0u;
+                st = r ? st + Lh : st;
+                n = r ? n - Lh : Lh;
+            }
+            const uint32_t dst = st + (g - before[(size_t)c * Sp + sg]);
+            if (dst < N) O[dst] = id[i];
+        }
+    }
+}
+
 // Coordinates in list-0 order, one array per dimension.
 __global__ void soa_gather_kernel(const float* __restrict__ pts, uint32_t D, const uint32_t* __restrict__ P0,
                                   uint32_t N, float* __restrict__ X) {
@@ -478,7 +665,7 @@ __global__ void leaf_fill_kernel(const float* __restrict__ pts, uint32_t D, cons
 // stable partition of the other lists), so the tree is identical; the global
 // levels cost three launches and a pass over every list each.  Points get
 // local ids (their position in list 0 of the segment) through a point-id ->
-// local-id scratch map, so the lists fit in 16 bits.  Per level and list:
+// local-id hash table in LDS, so the lists fit in 16 bits.  Per level and list:
 // left flag per local id from the split list's positions, a block scan of
 // the flags in position order, and a scatter to the stable position.
 constexpr uint32_t LF_THREADS = 256;
@@ -548,35 +735,97 @@ __device__ __forceinline__ void lf_partition(uint16_t* Lj, const uint8_t* flag, 
         if (p0 + i < n0) Lj[v[i] >> 16] = (uint16_t)(v[i] & 0xFFFFu);
 }
 
-// Local ids of a segment's points: gidOf[p] = list 0 entry p, and every list
-// rewritten as local ids through the point-id -> local-id map `loc` (written
-// and read back by this workgroup only, for its own point ids; the reads go
-// to L2).
+// Point-id -> local-id hash table of a local finish, in LDS: H 16-bit slots
+// (linear probing from a multiplicative hash) in H / 2 words.  A slot holds a
+// local id (0xFFFF: empty) and a probe compares gidOf of that local id with
+// the point id, so no keys are stored.
+__device__ __forceinline__ uint32_t lf_hash(uint32_t g, uint32_t H) {
+    return (uint32_t)(((uint64_t)(g * 0x9E3779B1u) * H) >> 32);
+}
+
+__device__ __forceinline__ uint32_t lf_slot(const uint32_t* tw, uint32_t k) {
+    return (tw[k >> 1] >> ((k & 1u) * 16u)) & 0xFFFFu;
+}
+
+__device__ __forceinline__ void lf_insert(uint32_t* tw, uint32_t g, uint32_t p, uint32_t H) {
+    for (uint32_t k = lf_hash(g, H);; k = k + 1 == H ? 0u : k + 1) {
+        uint32_t* w = tw + (k >> 1);
+        const uint32_t sh = (k & 1u) * 16u;
+        uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        while (((old >> sh) & 0xFFFFu) == 0xFFFFu) {
+            const uint32_t prev = atomicCAS(w, old, (old & ~(0xFFFFu << sh)) | (p << sh));
+            if (prev == old) return;
+            old = prev;
+        }
+    }
+}
+
+// Local id of point g, which the table holds.  An empty slot ends the probe
+// (local id 0): at most M of the H > M slots fill, so one is always reached.
+__device__ __forceinline__ uint32_t lf_lookup(const uint32_t* tw, const uint32_t* gidOf, uint32_t g, uint32_t H) {
+    for (uint32_t k = lf_hash(g, H);; k = k + 1 == H ? 0u : k + 1) {
+        const uint32_t v = lf_slot(tw, k);
+        if (v == 0xFFFFu) return 0u;
+        if (gidOf[v] == g) return v;
+    }
+}
+
+// Local ids of a segment's points: gidOf[p] = list 0 entry p, and every
+// other list rewritten as local ids through the hash table, which takes
+// H = 3M / 2 slots over `tab` (3M bytes: the E and flag arrays, unused until
+// the level loop).  List j is at (bit j of altMask ? P2 : P) + j * N.  Xs:
+// when given, the points' coordinates are copied there in local-id order
+// (Xs[(st0 + p) * D + d], coalesced), so later reads of them stay within the
+// segment's own span.
 template <int IT>
-__device__ __forceinline__ void lf_load_lists(const uint32_t* __restrict__ P, uint32_t N, uint32_t nlists,
-                                              uint32_t* __restrict__ loc, uint32_t st0, uint32_t n0,
-                                              uint32_t* gidOf, uint16_t* lists) {
-    constexpr uint32_t M = LF_THREADS * IT;
+__device__ __forceinline__ void lf_load_lists(const uint32_t* __restrict__ P, const uint32_t* __restrict__ P2,
+                                              uint32_t altMask, uint32_t N, uint32_t nlists,
+                                              uint32_t st0, uint32_t n0, uint32_t* gidOf, uint16_t* lists,
+                                              uint8_t* tab, const float* __restrict__ pts, uint32_t D,
+                                              float* __restrict__ Xs) {
+    constexpr uint32_t M = LF_THREADS * IT, H = 3 * M / 2;
+    static_assert(H % 2 == 0, "whole words of slots");
     const uint32_t tid = threadIdx.x;
+    uint32_t* tw = reinterpret_cast<uint32_t*>(tab);
+    for (uint32_t w = tid; w < H / 2; w += LF_THREADS) tw[w] = 0xFFFFFFFFu;
+    const uint32_t* L0 = (altMask & 1u) ? P2 : P;
     for (uint32_t p = tid; p < n0; p += LF_THREADS) {
-        const uint32_t g = P[st0 + p];
+        const uint32_t g = L0[st0 + p];
         gidOf[p] = g;
         lists[p] = (uint16_t)p;
-        loc[g] = p;
+        if (Xs)
+            for (uint32_t d = 0; d < D; ++d) Xs[(size_t)(st0 + p) * D + d] = pts[(size_t)g * D + d];
     }
-    __threadfence();
+    if (Xs) __threadfence();
     __syncthreads();
-    for (uint32_t j = 1; j < nlists; ++j)
-        for (uint32_t p = tid; p < n0; p += LF_THREADS)
-            lists[j * M + p] = (uint16_t)__hip_atomic_load(&loc[P[(size_t)j * N + st0 + p]], __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t p = tid; p < n0; p += LF_THREADS) lf_insert(tw, gidOf[p], p, H);
     __syncthreads();
+    for (uint32_t j = 1; j < nlists; ++j) {
+        const uint32_t* Lj = (((altMask >> j) & 1u) ? P2 : P) + (size_t)j * N;
+        uint32_t g[IT];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t p = tid + (uint32_t)i * LF_THREADS;
+            g[i] = p < n0 ? Lj[st0 + p] : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const uint32_t p = tid + (uint32_t)i * LF_THREADS;
+            if (p < n0) lists[j * M + p] = (uint16_t)lf_lookup(tw, gidOf, g[i], H);
+        }
+    }
+    __syncthreads();
+}
+
+// A coordinate that this workgroup wrote to Xs (read through L2).
+__device__ __forceinline__ float lf_coord(const float* Xs, size_t i) {
+    return __hip_atomic_load(Xs + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <int IT>
 __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
-    const float* __restrict__ pts, uint32_t D, uint32_t SD, uint32_t N, uint32_t* __restrict__ P,
-    uint32_t* __restrict__ loc, const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
+    const float* __restrict__ pts, uint32_t D, uint32_t SD, uint32_t N, const uint32_t* __restrict__ P,
+    const uint32_t* __restrict__ P2, uint32_t altMask, float* __restrict__ Xs, const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
     uint32_t d0, uint32_t nl, float* __restrict__ nodes, uint32_t skips, float* __restrict__ leaves) {
     constexpr uint32_t M = LF_THREADS * IT;
     extern __shared__ __align__(16) uint8_t lds_raw[];
@@ -588,7 +837,7 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
     const uint32_t s0 = blockIdx.x, tid = threadIdx.x;
     const uint32_t st0 = segStart[s0], n0 = segN[s0];
 
-    lf_load_lists<IT>(P, N, SD, loc, st0, n0, gidOf, lists);
+    lf_load_lists<IT>(P, P2, altMask, N, SD, st0, n0, gidOf, lists, reinterpret_cast<uint8_t*>(E), pts, D, Xs);
 
     // Thread tid owns positions tid * IT + i; ist / inn = start and size of
     // the segment holding that position at the current level.
@@ -609,7 +858,7 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
             uint32_t st, n;
             lf_walk(q, l, n0, st, n);
             const uint32_t L = (n + 1) >> 1;
-            nodes[S - 1 + (s0 << l) + q] = L ? pts[(size_t)gidOf[Ld[st + L - 1]] * D + dim] : 0.0f;
+            nodes[S - 1 + (s0 << l) + q] = L ? lf_coord(Xs, (size_t)(st0 + Ld[st + L - 1]) * D + dim) : 0.0f;
         }
         __syncthreads();
         for (uint32_t j = 0; j < SD; ++j)
@@ -638,9 +887,9 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
         const uint32_t k = z & kmask;
         float* e = out + (size_t)z * (1 + D);
         if (k < n) {
-            const uint32_t id = gidOf[Lf[st + k]];
-            e[0] = __int_as_float((int32_t)id);
-            for (uint32_t d = 0; d < D; ++d) e[1 + d] = pts[(size_t)id * D + d];
+            const uint32_t u = Lf[st + k];
+            e[0] = __int_as_float((int32_t)gidOf[u]);
+            for (uint32_t d = 0; d < D; ++d) e[1 + d] = lf_coord(Xs, (size_t)(st0 + u) * D + d);
         } else {
             e[0] = __int_as_float(-1);
             for (uint32_t d = 0; d < D; ++d) e[1 + d] = 0.0f;
@@ -682,7 +931,7 @@ __device__ __forceinline__ uint32_t blf_lanes(uint32_t N, uint32_t S2) {
 
 template <int IT, int DM>
 __global__ void __launch_bounds__(LF_THREADS) ball_local_finish_kernel(
-    const float* __restrict__ pts, uint32_t D, uint32_t N, uint32_t* __restrict__ P, uint32_t* __restrict__ loc,
+    const float* __restrict__ X, uint32_t D, uint32_t N, const uint32_t* __restrict__ P,
     const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN, const double* __restrict__ var0,
     uint32_t d0, uint32_t nl, uint32_t* __restrict__ nodeDim, float* __restrict__ ballNodes, float inflate,
     uint32_t skips, float* __restrict__ leaves) {
@@ -700,9 +949,10 @@ __global__ void __launch_bounds__(LF_THREADS) ball_local_finish_kernel(
     const uint32_t s0 = blockIdx.x, tid = threadIdx.x;
     const uint32_t st0 = segStart[s0], n0 = segN[s0];
 
-    lf_load_lists<IT>(P, N, D, loc, st0, n0, gidOf, lists);
+    lf_load_lists<IT>(P, P, 0u, N, D, st0, n0, gidOf, lists, reinterpret_cast<uint8_t*>(E), nullptr, D, nullptr);
+    // X holds the coordinates in list-0 order, so local id p is X[d * N + st0 + p].
     for (uint32_t p = tid; p < n0; p += LF_THREADS)
-        for (uint32_t d = 0; d < D; ++d) xs[d * M + p] = pts[(size_t)gidOf[p] * D + d];
+        for (uint32_t d = 0; d < D; ++d) xs[d * M + p] = X[(size_t)d * N + st0 + p];
     if (tid == 0) {
         // first maximum of the segment's deviation sums (strict >, from 0)
         uint32_t best = 0;
@@ -872,6 +1122,7 @@ struct LevelBufs {
     DevBuf<uint32_t> pivKey, pivId;      // split pivot (ordered key, id) per segment
     DevBuf<uint8_t> flag;                // left bit per point id (bitmap of 32-bit words)
     DevBuf<uint32_t> posFlag;            // left bit per list position, one bitmap per partitioned list
+    DevBuf<uint32_t> pair;               // kd: pair maps of two levels (2 bits per point id)
     DevBuf<uint32_t> start, n, start2, n2;  // segment tables (cur / next level)
     DevBuf<uint32_t> Lcount, leftBefore, segDim;
     DevBuf<uint32_t> cnt, temp;          // partition block counts, scan temp
@@ -880,7 +1131,7 @@ struct LevelBufs {
     DevBuf<float> cent, cent2;           // ball: centroids per segment
     DevBuf<double> var, var2, partial;   // ball: deviation sums, chunk partials
     DevBuf<float> pmax;                  // ball: chunk radius maxima
-    DevBuf<float> X, X2;                 // ball: coordinates in list-0 order (D arrays of N)
+    DevBuf<float> X, X2;                 // kd: local-finish coordinate copy (N x D); ball: list-0 order (D x N)
 };
 
 namespace {
@@ -911,7 +1162,9 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
     B.pivKey.ensure(maxSeg);
     B.pivId.ensure(maxSeg);
     B.flag.ensure(((size_t)N + 31) / 32 * 4);
-    B.posFlag.ensure((size_t)nl * (((size_t)N + 31) / 32));
+    // Ball: one bitmap per list; kd: 2-bit run codes of one list.
+    B.posFlag.ensure((size_t)std::max(nl, 2u) * (((size_t)N + 31) / 32));
+    B.pair.ensure(2 * (((size_t)N + 15) / 16));
     B.start.ensure(maxSeg);
     B.n.ensure(maxSeg);
     B.start2.ensure(maxSeg);
@@ -920,8 +1173,9 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
     B.leftBefore.ensure(maxSeg);
     B.segDim.ensure(maxSeg);
     const uint32_t nb = nblk(N, PT_TILE);
-    B.cnt.ensure((size_t)nb * nl);
-    size_t tw = scan_temp_words((uint32_t)((size_t)nb * nl));
+    // Tile counts: one per list and tile (ball), four per tile (kd runs).
+    B.cnt.ensure((size_t)nb * std::max(nl, 4u));
+    size_t tw = scan_temp_words((uint32_t)((size_t)nb * std::max(nl, 4u)));
     size_t tw2 = scan_temp_words(maxSeg);
     size_t tw3 = radix_temp_words(N);
     B.temp.ensure(std::max(tw, std::max(tw2, tw3)));
@@ -929,6 +1183,8 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
     B.keys2.ensure(N);
     B.vals2.ensure(N);
     B.ext.ensure(2 * (size_t)D);
+    // kd: the local finish's coordinate copy; ball: coordinates in list-0 order.
+    B.X.ensure((size_t)D * N);
     if (ball) {
         B.cent.ensure((size_t)maxSeg * D);
         B.cent2.ensure((size_t)maxSeg * D);
@@ -938,7 +1194,6 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
         const size_t items = (size_t)maxSeg + N / 4096 + 2;
         B.partial.ensure(items * 4);
         B.pmax.ensure(items);
-        B.X.ensure((size_t)D * N);
         B.X2.ensure((size_t)D * N);
     }
 }
@@ -967,6 +1222,22 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
                                hipMemcpyDeviceToDevice, st));
     std::swap(B.P.p, B.P2.p);
     if (coordD) std::swap(B.X.p, B.X2.p);
+}
+
+// Moves kd list `L` (in the segmentation of level l - k) to that of level l
+// by the pair map of level l - 1, writing it to `O`.  segN: the level-l
+// segment sizes (2^l of them).
+void kd_flush_list(LevelBufs& B, const uint32_t* L, uint32_t* O, uint32_t N, uint32_t l, uint32_t k,
+                   const uint32_t* pairPrev, const uint32_t* segN, hipStream_t st) {
+    const uint32_t nb = nblk(N, PT_TILE);
+    hipLaunchKernelGGL(kd_count_kernel, dim3(nb), dim3(PT_THREADS), 0, st, L, N, k, pairPrev, B.cnt.get(), nb,
+                       B.posFlag.get());
+    exclusive_scan_u32(B.cnt.get(), B.cnt.get(), 4 * nb, B.temp.get(), st);
+    const uint32_t total = 1u << l;
+    hipLaunchKernelGGL(kd_runs_kernel, dim3(nblk(total, 256)), dim3(256), 0, st, segN, total, k, B.Lcount.get());
+    exclusive_scan_u32(B.Lcount.get(), B.leftBefore.get(), total, B.temp.get(), st);
+    hipLaunchKernelGGL(kd_scatter_kernel, dim3(nb), dim3(PT_THREADS), 0, st, L, O, N, k, l - k, B.posFlag.get(),
+                       B.cnt.get(), nb, B.leftBefore.get());
 }
 
 void level_common(LevelBufs& B, const float* pts, uint32_t D, uint32_t N, uint32_t S, uint32_t uniformDim,
@@ -1071,12 +1342,22 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         return;
     }
     presort(B, pts, N, D, SD, st);
-    launch_fill_u32(B.segOf.get(), N, 0u, st);
     launch_fill_u32(B.segId.get(), N, 0u, st);
     launch_fill_u32(B.start.get(), 1, 0u, st);
     launch_fill_u32(B.n.get(), 1, N, st);
     uint32_t *start = B.start.get(), *n = B.n.get(), *start2 = B.start2.get(), *n2 = B.n2.get();
-    uint32_t *segOf = B.segOf.get(), *segOf2 = B.segOf2.get();
+    // List j lives in P or P2 (bit j of `alt`) and waits pend[j] levels
+    // (at most 2) for its partition; a level first partitions its split list
+    // and every list that waits 2 levels.
+    std::vector<uint32_t> pend(SD, 0u), alt(SD, 0u);
+    auto list = [&](uint32_t j) { return (alt[j] ? B.P2.get() : B.P.get()) + (size_t)j * N; };
+    auto other = [&](uint32_t j) { return (alt[j] ? B.P.get() : B.P2.get()) + (size_t)j * N; };
+    uint32_t* pairMap[2] = {B.pair.get(), B.pair.get() + ((size_t)N + 15) / 16};
+    auto flush = [&](uint32_t j, uint32_t l) {
+        kd_flush_list(B, list(j), other(j), N, l, pend[j], pairMap[(l - 1) & 1u], n, st);
+        alt[j] ^= 1u;
+        pend[j] = 0;
+    };
     // Global levels until every segment fits one workgroup's LDS, then the
     // local finish for the rest.
     const uint32_t lfM = kd_local_finish_points(SD);
@@ -1088,26 +1369,36 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     for (uint32_t depth = 0; depth < d0; ++depth) {
         const uint32_t S = 1u << depth;
         const uint32_t dim = depth % SD;
-        hipLaunchKernelGGL(kd_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, pts, D, B.P.get() + (size_t)dim * N,
-                           dim, start, n, S, h.nodes.get(), start2, n2, B.Lcount.get(), B.pivKey.get(),
-                           B.pivId.get());
-        level_common(B, pts, D, N, S, dim, false, start, n, segOf, segOf2, st);
-        partition_lists(B, N, SD, dim, S, start, n, segOf, st);
-        std::swap(segOf, segOf2);
+        for (uint32_t j = 0; j < SD; ++j)
+            if (pend[j] == 2 || (j == dim && pend[j] > 0)) flush(j, depth);
+        hipLaunchKernelGGL(kd_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, pts, D, list(dim), dim, start, n,
+                           S, h.nodes.get(), start2, n2, B.Lcount.get(), B.pivKey.get(), B.pivId.get());
+        hipLaunchKernelGGL(kd_flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, D, N, dim, B.segId.get(),
+                           B.pivKey.get(), B.pivId.get(), depth ? pairMap[(depth - 1) & 1u] : nullptr,
+                           pairMap[depth & 1u]);
+        for (uint32_t j = 0; j < SD; ++j)
+            if (j != dim) ++pend[j];
         std::swap(start, start2);
         std::swap(n, n2);
     }
     if (d0 < I.node_depth) {
+        uint32_t altMask = 0;  // the local finish runs for SD <= 28 lists
+        for (uint32_t j = 0; j < SD; ++j) {
+            if (pend[j] > 0) flush(j, d0);
+            altMask |= alt[j] << j;
+        }
         const uint32_t nl = I.node_depth - d0;
         const size_t lds = lf_lds_bytes(lfM, SD);
         auto* fn = lfM == 4096 ? kd_local_finish_kernel<16> : lfM == 2048 ? kd_local_finish_kernel<8>
                                                                           : kd_local_finish_kernel<4>;
-        hipLaunchKernelGGL(fn, dim3(1u << d0), dim3(LF_THREADS), lds, st, pts, D, SD, N, B.P.get(), B.keys.get(),
-                           start, n, d0, nl, h.nodes.get(), I.depth_skips, h.leaves.get());
+        hipLaunchKernelGGL(fn, dim3(1u << d0), dim3(LF_THREADS), lds, st, pts, D, SD, N, B.P.get(), B.P2.get(),
+                           altMask, B.X.get(), start, n, d0, nl, h.nodes.get(), I.depth_skips, h.leaves.get());
     } else {
+        // Only the last split list is read; it is in the final segmentation.
         const uint32_t lastDim = (I.node_depth - 1) % SD;
-        hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N,
-                           lastDim, nullptr, I.node_size, start, n, I.depth_skips, I.index_size, h.leaves.get());
+        hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D,
+                           list(lastDim) - (size_t)lastDim * N, N, lastDim, nullptr, I.node_size, start, n,
+                           I.depth_skips, I.index_size, h.leaves.get());
     }
     KNN_CHECK_LAUNCH();
     compute_bbox_ws(pts, N, D, h.bbox.get(), B.ext.get(), st);
@@ -1181,8 +1472,8 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
                    : lfM == 4096 ? ball_local_finish_kernel<16, 4>
                    : lfM == 2048 ? ball_local_finish_kernel<8, 4>
                                  : ball_local_finish_kernel<4, 4>;
-        hipLaunchKernelGGL(fn, dim3(1u << d0), dim3(LF_THREADS), blf_lds_bytes(lfM, D), st, pts, D, N, B.P.get(),
-                           B.keys.get(), start, n, var, d0, nl, h.dims.get(), h.nodes.get(), inflate, I.depth_skips,
+        hipLaunchKernelGGL(fn, dim3(1u << d0), dim3(LF_THREADS), blf_lds_bytes(lfM, D), st, B.X.get(), D, N,
+                           B.P.get(), start, n, var, d0, nl, h.dims.get(), h.nodes.get(), inflate, I.depth_skips,
                            h.leaves.get());
     } else {
         hipLaunchKernelGGL(leaf_fill_kernel, dim3(nblk(I.index_size, 256)), dim3(256), 0, st, pts, D, B.P.get(), N,
