@@ -32,10 +32,15 @@ namespace knn {
 
 static inline uint32_t nblk(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
+// keep: an optional second copy that the sort does not overwrite.
 __global__ void ordered_keys_kernel(const float* __restrict__ pts, uint32_t n, uint32_t D, uint32_t dim,
-                                    uint32_t* __restrict__ keys) {
+                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ keep) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = ordered_bits(pts[i * D + dim]);
+    if (i < n) {
+        const uint32_t k = ordered_bits(pts[i * D + dim]);
+        keys[i] = k;
+        if (keep) keep[i] = k;
+    }
 }
 
 // --- bounding box (query ordering) -----------------------------------------
@@ -352,9 +357,10 @@ __device__ __forceinline__ uint32_t code_of(const uint32_t* __restrict__ map, si
     return (map[i >> 4] >> (2 * (uint32_t)(i & 15u))) & 3u;
 }
 
-// Left bit per point id for split dim `dim` (rule of flag_kernel) and the
-// level's pair map; `prev` is the previous level's map (null at level 0).
-__global__ void kd_flag_kernel(const float* __restrict__ pts, uint32_t D, uint32_t N, uint32_t dim,
+// Left bit per point id for split dim `dim` (rule of flag_kernel; okeys: the
+// dim's ordered keys by point id) and the level's pair map; `prev` is the
+// previous level's map (null at level 0).
+__global__ void kd_flag_kernel(const uint32_t* __restrict__ okeys, uint32_t N,
                                uint32_t* __restrict__ segById, const uint32_t* __restrict__ pivKey,
                                const uint32_t* __restrict__ pivId, const uint32_t* __restrict__ prev,
                                uint32_t* __restrict__ pair) {
@@ -362,7 +368,7 @@ __global__ void kd_flag_kernel(const float* __restrict__ pts, uint32_t D, uint32
     bool left = false, before = false;
     if (i < N) {
         const uint32_t s = segById[i];
-        const uint32_t key = ordered_bits(pts[(size_t)i * D + dim]);
+        const uint32_t key = okeys[i];
         const uint32_t pk = pivKey[s];
         left = key < pk || (key == pk && i <= pivId[s]);
         segById[i] = 2 * s + (left ? 0u : 1u);
@@ -1123,6 +1129,7 @@ struct LevelBufs {
     DevBuf<uint8_t> flag;                // left bit per point id (bitmap of 32-bit words)
     DevBuf<uint32_t> posFlag;            // left bit per list position, one bitmap per partitioned list
     DevBuf<uint32_t> pair;               // kd: pair maps of two levels (2 bits per point id)
+    DevBuf<uint32_t> okeys;              // kd: ordered keys per split dim, by point id
     DevBuf<uint32_t> start, n, start2, n2;  // segment tables (cur / next level)
     DevBuf<uint32_t> Lcount, leftBefore, segDim;
     DevBuf<uint32_t> cnt, temp;          // partition block counts, scan temp
@@ -1136,11 +1143,14 @@ struct LevelBufs {
 
 namespace {
 
-// Sorted lists P_d, d < nl, by (ordered coordinate bits, id).
-void presort(LevelBufs& B, const float* pts, uint32_t N, uint32_t D, uint32_t nl, hipStream_t st) {
+// Sorted lists P_d, d < nl, by (ordered coordinate bits, id).  keep: when
+// given, the ordered keys of dim d by point id are left at keep + d * N.
+void presort(LevelBufs& B, const float* pts, uint32_t N, uint32_t D, uint32_t nl, hipStream_t st,
+             uint32_t* keep = nullptr) {
     for (uint32_t d = 0; d < nl; ++d) {
         uint32_t* vals = B.P.get() + (size_t)d * N;
-        hipLaunchKernelGGL(ordered_keys_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, N, D, d, B.keys.get());
+        hipLaunchKernelGGL(ordered_keys_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, N, D, d, B.keys.get(),
+                           keep ? keep + (size_t)d * N : nullptr);
         launch_iota(vals, N, st);
         bool alt = radix_sort_pairs(B.keys.get(), vals, B.keys2.get(), B.vals2.get(), N, 0, 32, B.temp.get(), st);
         if (alt) KNN_HIP(hipMemcpyAsync(vals, B.vals2.get(), sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, st));
@@ -1165,6 +1175,7 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
     // Ball: one bitmap per list; kd: 2-bit run codes of one list.
     B.posFlag.ensure((size_t)std::max(nl, 2u) * (((size_t)N + 31) / 32));
     B.pair.ensure(2 * (((size_t)N + 15) / 16));
+    if (!ball) B.okeys.ensure((size_t)nl * N);
     B.start.ensure(maxSeg);
     B.n.ensure(maxSeg);
     B.start2.ensure(maxSeg);
@@ -1341,7 +1352,7 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         compute_bbox_ws(pts, 0, D, h.bbox.get(), B.ext.get(), st);
         return;
     }
-    presort(B, pts, N, D, SD, st);
+    presort(B, pts, N, D, SD, st, B.okeys.get());
     launch_fill_u32(B.segId.get(), N, 0u, st);
     launch_fill_u32(B.start.get(), 1, 0u, st);
     launch_fill_u32(B.n.get(), 1, N, st);
@@ -1373,9 +1384,9 @@ void build_kd_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
             if (pend[j] == 2 || (j == dim && pend[j] > 0)) flush(j, depth);
         hipLaunchKernelGGL(kd_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, pts, D, list(dim), dim, start, n,
                            S, h.nodes.get(), start2, n2, B.Lcount.get(), B.pivKey.get(), B.pivId.get());
-        hipLaunchKernelGGL(kd_flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, D, N, dim, B.segId.get(),
-                           B.pivKey.get(), B.pivId.get(), depth ? pairMap[(depth - 1) & 1u] : nullptr,
-                           pairMap[depth & 1u]);
+        hipLaunchKernelGGL(kd_flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, B.okeys.get() + (size_t)dim * N, N,
+                           B.segId.get(), B.pivKey.get(), B.pivId.get(),
+                           depth ? pairMap[(depth - 1) & 1u] : nullptr, pairMap[depth & 1u]);
         for (uint32_t j = 0; j < SD; ++j)
             if (j != dim) ++pend[j];
         std::swap(start, start2);

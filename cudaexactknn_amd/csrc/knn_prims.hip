@@ -54,6 +54,29 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_tile_kernel(const uint32_t*
     }
 }
 
+// One workgroup of 1024 threads for arrays of up to 16K entries (one launch
+// instead of three).  In-place safe like scan_tile_kernel.
+constexpr int SCAN_SMALL_THREADS = 1024;
+constexpr uint32_t SCAN_SMALL_MAX = SCAN_SMALL_THREADS * SCAN_ITEMS;
+__global__ void __launch_bounds__(SCAN_SMALL_THREADS) scan_small_kernel(const uint32_t* in, uint32_t* out, uint32_t n) {
+    __shared__ uint32_t lw[SCAN_SMALL_THREADS / 64 + 1];
+    const uint32_t base = threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = base + i < n ? in[base + i] : 0u;
+        s += v[i];
+    }
+    uint32_t total;
+    uint32_t run = block_excl_scan(s, lw, &total);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+}
+
 static uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 size_t scan_temp_words(uint32_t n) {
@@ -70,6 +93,10 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t*
     uint32_t nb = div_up(n, SCAN_TILE);
     if (nb == 1) {
         hipLaunchKernelGGL(scan_tile_kernel, dim3(1), dim3(SCAN_THREADS), 0, st, in, out, n, nullptr);
+        return;
+    }
+    if (n <= SCAN_SMALL_MAX) {
+        hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(SCAN_SMALL_THREADS), 0, st, in, out, n);
         return;
     }
     uint32_t* sums = temp;
@@ -100,12 +127,20 @@ __global__ void __launch_bounds__(RS_THREADS) radix_hist_kernel(const uint32_t* 
     hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
+// Ranks a tile's keys stably by digit, reorders them in LDS (digit-major),
+// then writes each digit's run out with consecutive lanes on consecutive
+// addresses, so the stores of a wave cover a few contiguous runs instead of
+// up to 64 scattered words.
 __global__ void __launch_bounds__(RS_THREADS) radix_scatter_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                                    uint32_t* __restrict__ okeys, uint32_t* __restrict__ ovals,
                                                                    uint32_t n, int shift, const uint32_t* __restrict__ offs,
                                                                    uint32_t nb) {
-    __shared__ uint16_t ch[RS_CHUNKS][256];
-    __shared__ uint32_t goff[256];
+    // Ranking: per-chunk digit counts ch[c][digit]; then the reordered tile
+    // (keys, then values) over the same bytes.
+    __shared__ uint32_t sbuf[2 * RS_TILE];
+    static_assert(sizeof(uint16_t) * RS_CHUNKS * 256 == sizeof(uint32_t) * 2 * RS_TILE, "shared staging");
+    __shared__ uint32_t goff[256], tdo[256], lw[RS_THREADS / 64 + 1];
+    uint16_t(*ch)[256] = reinterpret_cast<uint16_t(*)[256]>(sbuf);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     goff[threadIdx.x] = offs[(size_t)threadIdx.x * nb + blockIdx.x];
@@ -135,25 +170,43 @@ __global__ void __launch_bounds__(RS_THREADS) radix_scatter_kernel(const uint32_
         if (valid && (m & lt) == 0) ch[c][dg] = (uint16_t)__popcll(m);
     }
     __syncthreads();
-    {
-        uint32_t run = 0;
-        for (int c = 0; c < RS_CHUNKS; ++c) {
-            uint32_t t = ch[c][threadIdx.x];
-            ch[c][threadIdx.x] = (uint16_t)run;
-            run += t;
-        }
+    uint32_t run = 0;
+    for (int c = 0; c < RS_CHUNKS; ++c) {
+        uint32_t t = ch[c][threadIdx.x];
+        ch[c][threadIdx.x] = (uint16_t)run;
+        run += t;
     }
+    // Thread t holds digit t's count in the tile: tdo = the digit's first
+    // slot in the reordered tile.
+    uint32_t tot;
+    tdo[threadIdx.x] = block_excl_scan(run, lw, &tot);
     __syncthreads();
+    uint32_t lpos[RS_CHUNKS_PER_WAVE];
 #pragma unroll
     for (int i = 0; i < RS_CHUNKS_PER_WAVE; ++i) {
         const int c = wave * RS_CHUNKS_PER_WAVE + i;
-        const size_t j = base + (size_t)c * 64 + lane;
-        if (j < n) {
-            const uint32_t dg = (k[i] >> shift) & 255u;
-            const uint32_t pos = goff[dg] + ch[c][dg] + rank[i];
-            okeys[pos] = k[i];
-            ovals[pos] = v[i];
+        const uint32_t dg = (k[i] >> shift) & 255u;
+        lpos[i] = tdo[dg] + ch[c][dg] + rank[i];
+    }
+    __syncthreads();  // the counts are read; the buffer now takes the tile
+    uint32_t* sk = sbuf;
+    uint32_t* sv = sbuf + RS_TILE;
+#pragma unroll
+    for (int i = 0; i < RS_CHUNKS_PER_WAVE; ++i) {
+        const int c = wave * RS_CHUNKS_PER_WAVE + i;
+        if (base + (size_t)c * 64 + lane < n) {
+            sk[lpos[i]] = k[i];
+            sv[lpos[i]] = v[i];
         }
+    }
+    __syncthreads();
+    const uint32_t cnt = (uint32_t)min((size_t)RS_TILE, (size_t)n - base);
+    for (uint32_t i = threadIdx.x; i < cnt; i += RS_THREADS) {
+        const uint32_t kk = sk[i];
+        const uint32_t dg = (kk >> shift) & 255u;
+        const uint32_t pos = goff[dg] + (i - tdo[dg]);
+        okeys[pos] = kk;
+        ovals[pos] = sv[i];
     }
 }
 
