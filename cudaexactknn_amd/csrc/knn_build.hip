@@ -170,8 +170,7 @@ __global__ void ball_node_kernel(const double* __restrict__ var, uint32_t D, con
     Lcount[s] = L;
 }
 
-// Left/right bit per point id (bit id of the bitmap, set for left) and
-// next-level segment per position.  The bitmap (N / 8 bytes: 2 MiB at 2^24
+// Left/right bit per point id (bit id of the bitmap, set for left).  The bitmap (N / 8 bytes: 2 MiB at 2^24
 // points) stays in L2 for the partition kernels' random gathers.
 __device__ __forceinline__ uint32_t left_bit(const uint32_t* __restrict__ bits, uint32_t id) {
     return (bits[id >> 5] >> (id & 31u)) & 1u;
@@ -184,16 +183,12 @@ __device__ __forceinline__ uint32_t left_bit(const uint32_t* __restrict__ bits, 
 // Points are visited in id order, so each wave's ballot is two whole bitmap
 // words (no atomics, no clearing); segById carries each point's segment.
 __global__ void flag_kernel(const float* __restrict__ pts, uint32_t D, uint32_t N, uint32_t uniformDim,
-                            const uint32_t* __restrict__ segDim, const uint32_t* __restrict__ segOf,
-                            const uint32_t* __restrict__ segStart, const uint32_t* __restrict__ segN,
-                            uint32_t* __restrict__ segOfNext, uint32_t* __restrict__ segById,
+                            const uint32_t* __restrict__ segDim, uint32_t* __restrict__ segById,
                             const uint32_t* __restrict__ pivKey, const uint32_t* __restrict__ pivId,
                             uint32_t* __restrict__ bits) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool left = false;
     if (i < N) {
-        const uint32_t sp = segOf[i];
-        segOfNext[i] = 2 * sp + (i - segStart[sp] < (segN[sp] + 1) / 2 ? 0u : 1u);
         const uint32_t s = segById[i];
         const uint32_t d = segDim ? segDim[s] : uniformDim;
         const uint32_t key = ordered_bits(pts[(size_t)i * D + d]);
@@ -257,9 +252,7 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
                                                                   uint32_t N, uint32_t skip,
                                                                   const uint32_t* __restrict__ pbits, uint32_t nwords,
                                                                   const uint32_t* __restrict__ off, uint32_t nb,
-                                                                  const uint32_t* __restrict__ segOf,
-                                                                  const uint32_t* __restrict__ segStart,
-                                                                  const uint32_t* __restrict__ segN,
+                                                                  uint32_t lvl,
                                                                   const uint32_t* __restrict__ leftBefore,
                                                                   const uint32_t* __restrict__ Lcount, uint32_t S,
                                                                   const float* __restrict__ Xin,
@@ -307,9 +300,16 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
     for (int i = 0; i < PT_ITEMS; ++i) {
         const size_t j = base + (size_t)i * PT_THREADS + tid;
         if (j < N) {
-            const uint32_t s = segOf[j];
-            const uint32_t st = segStart[s];
-            const uint32_t Ls = (segN[s] + 1) / 2;
+            // Segment of position j at level lvl (split sizes from the root).
+            uint32_t st = 0, n = N, s = 0;
+            for (uint32_t b = 0; b < lvl; ++b) {
+                const uint32_t Lh = (n + 1) >> 1;
+                const bool r = (uint32_t)j - st >= Lh;
+                s = 2 * s + (r ? 1u : 0u);
+                st = r ? st + Lh : st;
+                n = r ? n - Lh : Lh;
+            }
+            const uint32_t Ls = (n + 1) >> 1;
             const uint32_t g = g0 + wcnt[i * (PT_THREADS / 64) + wave] + rank[i];
             const uint32_t lb = g - leftBefore[s];
             const bool f = (fl >> i) & 1u;
@@ -431,6 +431,44 @@ __global__ void kd_runs_kernel(const uint32_t* __restrict__ segN, uint32_t total
     if (e >= total) return;
     const uint32_t Sp = total >> k, c = e / Sp, s = e - c * Sp;
     out[e] = segN[(s << k) + c];
+}
+
+// kd_runs_kernel and the exclusive scan in one workgroup, for up to
+// KD_BEFORE_MAX entries (LDS-staged like the small device scan).
+constexpr int KD_BEFORE_THREADS = 1024;
+constexpr uint32_t KD_BEFORE_MAX = KD_BEFORE_THREADS * 16;
+__global__ void __launch_bounds__(KD_BEFORE_THREADS) kd_before_kernel(const uint32_t* __restrict__ segN,
+                                                                     uint32_t total, uint32_t k,
+                                                                     uint32_t* __restrict__ out) {
+    __shared__ uint32_t buf[KD_BEFORE_MAX];
+    __shared__ uint32_t lw[KD_BEFORE_THREADS / 64 + 1];
+    const uint32_t Sp = total >> k;
+    for (uint32_t e = threadIdx.x; e < KD_BEFORE_MAX; e += KD_BEFORE_THREADS) {
+        uint32_t v = 0;
+        if (e < total) {
+            const uint32_t c = e / Sp, s = e - c * Sp;
+            v = segN[(s << k) + c];
+        }
+        buf[e] = v;
+    }
+    __syncthreads();
+    const uint32_t base = threadIdx.x * 16;
+    uint32_t v[16];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        v[i] = buf[base + i];
+        sum += v[i];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan(sum, lw, &tot);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        buf[base + i] = run;
+        run += v[i];
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < total; e += KD_BEFORE_THREADS) out[e] = buf[e];
 }
 
 // Stable partition of one list by its position buckets: item j of bucket c
@@ -1123,7 +1161,6 @@ namespace {
 // builds reuse it in stream order.
 struct LevelBufs {
     DevBuf<uint32_t> P, P2;              // D (or SD) lists of N ids, double-buffered
-    DevBuf<uint32_t> segOf, segOf2;      // segment per position
     DevBuf<uint32_t> segId;              // segment per point id
     DevBuf<uint32_t> pivKey, pivId;      // split pivot (ordered key, id) per segment
     DevBuf<uint8_t> flag;                // left bit per point id (bitmap of 32-bit words)
@@ -1166,8 +1203,6 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
     if (N == 0) N = 1;
     B.P.ensure((size_t)nl * N);
     B.P2.ensure((size_t)nl * N);
-    B.segOf.ensure(N);
-    B.segOf2.ensure(N);
     B.segId.ensure(N);
     B.pivKey.ensure(maxSeg);
     B.pivId.ensure(maxSeg);
@@ -1211,10 +1246,10 @@ void alloc_levels(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t maxSeg, uint32
 
 namespace {
 
-// Partition every list except `skip` by the point flags (segment tables of
-// the current level: start, n, segOf).
-void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint32_t S, const uint32_t* start,
-                     const uint32_t* n, const uint32_t* segOf, hipStream_t st, uint32_t coordD = 0) {
+// Partition every list except `skip` by the point flags of level lvl (S
+// segments).
+void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint32_t S, uint32_t lvl, hipStream_t st,
+                     uint32_t coordD = 0) {
     const uint32_t nlists = (skip < nl) ? nl - 1 : nl;
     if (nlists == 0) return;
     const uint32_t nb = nblk(N, PT_TILE);
@@ -1223,8 +1258,7 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
                        reinterpret_cast<const uint32_t*>(B.flag.get()), B.cnt.get(), nb, B.posFlag.get(), nwords);
     exclusive_scan_u32(B.cnt.get(), B.cnt.get(), nb * nlists, B.temp.get(), st);
     hipLaunchKernelGGL(part_scatter_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), B.P2.get(), N, skip,
-                       B.posFlag.get(), nwords, B.cnt.get(), nb, segOf, start, n,
-                       B.leftBefore.get(), B.Lcount.get(), S, coordD ? B.X.get() : nullptr,
+                       B.posFlag.get(), nwords, B.cnt.get(), nb, lvl, B.leftBefore.get(), B.Lcount.get(), S, coordD ? B.X.get() : nullptr,
                        coordD ? B.X2.get() : nullptr, coordD);
     // The partitioned lists live in P2; the skipped list (already in order)
     // is carried over so that P2 is complete before the swap.
@@ -1245,20 +1279,24 @@ void kd_flush_list(LevelBufs& B, const uint32_t* L, uint32_t* O, uint32_t N, uin
                        B.posFlag.get());
     exclusive_scan_u32(B.cnt.get(), B.cnt.get(), 4 * nb, B.temp.get(), st);
     const uint32_t total = 1u << l;
-    hipLaunchKernelGGL(kd_runs_kernel, dim3(nblk(total, 256)), dim3(256), 0, st, segN, total, k, B.Lcount.get());
-    exclusive_scan_u32(B.Lcount.get(), B.leftBefore.get(), total, B.temp.get(), st);
+    if (total <= KD_BEFORE_MAX) {
+        hipLaunchKernelGGL(kd_before_kernel, dim3(1), dim3(KD_BEFORE_THREADS), 0, st, segN, total, k,
+                           B.leftBefore.get());
+    } else {
+        hipLaunchKernelGGL(kd_runs_kernel, dim3(nblk(total, 256)), dim3(256), 0, st, segN, total, k,
+                           B.Lcount.get());
+        exclusive_scan_u32(B.Lcount.get(), B.leftBefore.get(), total, B.temp.get(), st);
+    }
     hipLaunchKernelGGL(kd_scatter_kernel, dim3(nb), dim3(PT_THREADS), 0, st, L, O, N, k, l - k, B.posFlag.get(),
                        B.cnt.get(), nb, B.leftBefore.get());
 }
 
 void level_common(LevelBufs& B, const float* pts, uint32_t D, uint32_t N, uint32_t S, uint32_t uniformDim,
-                  bool perSegDim, const uint32_t* start, const uint32_t* n, const uint32_t* segOf, uint32_t* segOf2,
-                  hipStream_t st) {
+                  bool perSegDim, hipStream_t st) {
     exclusive_scan_u32(B.Lcount.get(), B.leftBefore.get(), S, B.temp.get(), st);
     uint32_t* bits = reinterpret_cast<uint32_t*>(B.flag.get());
     hipLaunchKernelGGL(flag_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, D, N, uniformDim,
-                       perSegDim ? B.segDim.get() : nullptr, segOf, start, n, segOf2, B.segId.get(), B.pivKey.get(),
-                       B.pivId.get(), bits);
+                       perSegDim ? B.segDim.get() : nullptr, B.segId.get(), B.pivKey.get(), B.pivId.get(), bits);
 }
 
 struct Timer {
@@ -1438,12 +1476,10 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
     }
     presort(B, pts, N, D, D, st);
     hipLaunchKernelGGL(soa_gather_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, pts, D, B.P.get(), N, B.X.get());
-    launch_fill_u32(B.segOf.get(), N, 0u, st);
     launch_fill_u32(B.segId.get(), N, 0u, st);
     launch_fill_u32(B.start.get(), 1, 0u, st);
     launch_fill_u32(B.n.get(), 1, N, st);
     uint32_t *start = B.start.get(), *n = B.n.get(), *start2 = B.start2.get(), *n2 = B.n2.get();
-    uint32_t *segOf = B.segOf.get(), *segOf2 = B.segOf2.get();
     float *cent = B.cent.get(), *cent2 = B.cent2.get();
     double *var = B.var.get(), *var2 = B.var2.get();
     // Root centroid and deviation sums (no BallNode entry for the root).
@@ -1462,9 +1498,8 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         hipLaunchKernelGGL(ball_node_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, var, D, start, n, S, h.dims.get(),
                            B.segDim.get(), start2, n2, B.Lcount.get(), pts, B.P.get(), N, B.pivKey.get(),
                            B.pivId.get());
-        level_common(B, pts, D, N, S, 0, true, start, n, segOf, segOf2, st);
-        partition_lists(B, N, D, 0xFFFFFFFFu, S, start, n, segOf, st, D);
-        std::swap(segOf, segOf2);
+        level_common(B, pts, D, N, S, 0, true, st);
+        partition_lists(B, N, D, 0xFFFFFFFFu, S, depth, st, D);
         std::swap(start, start2);
         std::swap(n, n2);
         // Children (level depth + 1): centroids, then deviation sums + radii.

@@ -55,26 +55,33 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_tile_kernel(const uint32_t*
 }
 
 // One workgroup of 1024 threads for arrays of up to 16K entries (one launch
-// instead of three).  In-place safe like scan_tile_kernel.
+// instead of three).  The array is staged through LDS so that the global
+// loads and stores are lane-consecutive.  In-place safe.
 constexpr int SCAN_SMALL_THREADS = 1024;
 constexpr uint32_t SCAN_SMALL_MAX = SCAN_SMALL_THREADS * SCAN_ITEMS;
 __global__ void __launch_bounds__(SCAN_SMALL_THREADS) scan_small_kernel(const uint32_t* in, uint32_t* out, uint32_t n) {
+    __shared__ uint32_t buf[SCAN_SMALL_MAX];
     __shared__ uint32_t lw[SCAN_SMALL_THREADS / 64 + 1];
+    for (uint32_t i = threadIdx.x; i < SCAN_SMALL_MAX; i += SCAN_SMALL_THREADS) buf[i] = i < n ? in[i] : 0u;
+    __syncthreads();
     const uint32_t base = threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        v[i] = base + i < n ? in[base + i] : 0u;
+        v[i] = buf[base + i];
         s += v[i];
     }
+    // The scan's barriers order every read of buf above before the writes.
     uint32_t total;
     uint32_t run = block_excl_scan(s, lw, &total);
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        if (base + i < n) out[base + i] = run;
+        buf[base + i] = run;
         run += v[i];
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += SCAN_SMALL_THREADS) out[i] = buf[i];
 }
 
 static uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
