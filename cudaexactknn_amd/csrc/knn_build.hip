@@ -586,31 +586,51 @@ __global__ void seg_reduce_kernel(const float* __restrict__ X, uint32_t N, uint3
         const uint32_t b0 = st + k * len;
         const uint32_t e0 = min(st + n, b0 + len);
         const float* c = cent ? cent + (size_t)s * D : nullptr;
-        for (uint32_t j = b0 + r; j < e0; j += G) {
-            const float* p = X + j;  // p[d * N] is dimension d
+        // One position's terms, added in position order.
+        auto add = [&](const float (&x)[DB], uint32_t j) {
             if (!c) {
 #pragma unroll
                 for (int b = 0; b < DB; ++b)
-                    if (dbase + b < D) acc[b] += (double)p[(size_t)(dbase + b) * N];
+                    if (dbase + b < D) acc[b] += (double)x[b];
             } else {
 #pragma unroll
                 for (int b = 0; b < DB; ++b)
                     if (dbase + b < D) {
-                        const double u = (double)p[(size_t)(dbase + b) * N] - (double)c[dbase + b];
+                        const double u = (double)x[b] - (double)c[dbase + b];
                         acc[b] += u * u;
                     }
                 if (pmax) {
                     // calcDistance2(centroid, point), src/BallTree.cuh:81
-                    float tt = c[0] - p[0];
+                    // (pmax is only asked for with dbase = 0).
+                    float tt = c[0] - x[0];
                     float d2 = tt * tt;
                     for (uint32_t d = 1; d < D; ++d) {
-                        const float u = c[d] - p[(size_t)d * N];
+                        const float u = c[d] - (d < (uint32_t)DB ? x[d] : X[(size_t)d * N + j]);
                         const float sq = u * u;
                         d2 = d2 + sq;
                     }
                     mx = d2 > mx ? d2 : mx;
                 }
             }
+        };
+        auto load = [&](float (&x)[DB], uint32_t j) {
+#pragma unroll
+            for (int b = 0; b < DB; ++b) x[b] = dbase + b < D ? X[(size_t)(dbase + b) * N + j] : 0.0f;
+        };
+        // Four positions' loads in flight per lane; the additions keep the
+        // order of the plain loop.
+        uint32_t j = b0 + r;
+        for (; j + 3 * G < e0; j += 4 * G) {
+            float x[4][DB];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) load(x[u], j + u * G);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) add(x[u], j + u * G);
+        }
+        for (; j < e0; j += G) {
+            float x[DB];
+            load(x, j);
+            add(x, j);
         }
     }
     for (uint32_t o = G >> 1; o >= 1; o >>= 1) {
