@@ -224,9 +224,20 @@ __global__ void __launch_bounds__(PT_THREADS) part_count_kernel(const uint32_t* 
     const size_t base = (size_t)blockIdx.x * PT_TILE;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t c = 0;
-    for (uint32_t i = threadIdx.x; i < PT_TILE; i += PT_THREADS) {
-        const size_t j = base + i;
-        const uint32_t f = j < N ? left_bit(bits, L[j]) : 0u;
+    // All of the thread's list loads, then all of its bitmap gathers, are
+    // issued before the first use.
+    uint32_t fv[PT_ITEMS];
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + (size_t)i * PT_THREADS + threadIdx.x;
+        fv[i] = j < N ? L[j] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) fv[i] = left_bit(bits, fv[i]);
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + (size_t)i * PT_THREADS + threadIdx.x;
+        const uint32_t f = j < N ? fv[i] : 0u;
         c += f;
         const uint64_t m = __ballot(f != 0);
         const size_t w = (j - lane) >> 5;  // first of the wave's two words
@@ -397,10 +408,21 @@ __global__ void __launch_bounds__(PT_THREADS) kd_count_kernel(const uint32_t* __
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const size_t nwords = ((size_t)N + 15) / 16;
     uint32_t cv = 0, c1 = 0, c2 = 0, c3 = 0;  // valid items and buckets 1..3 (wave-uniform)
-    for (uint32_t i = threadIdx.x; i < PT_TILE; i += PT_THREADS) {
-        const size_t j = base + i;
+    // All of the thread's list loads, then all of its map gathers, are
+    // issued before the first use.
+    uint32_t cw[PT_ITEMS];
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + (size_t)i * PT_THREADS + threadIdx.x;
+        cw[i] = j < N ? L[j] : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) cw[i] = code_of(pair, cw[i]);
+#pragma unroll
+    for (int i = 0; i < PT_ITEMS; ++i) {
+        const size_t j = base + (size_t)i * PT_THREADS + threadIdx.x;
         const bool v = j < N;
-        const uint32_t c = v ? kd_bucket(code_of(pair, L[j]), k) : 0u;
+        const uint32_t c = v ? kd_bucket(cw[i], k) : 0u;
         const uint64_t b0 = __ballot(c & 1u), b1 = __ballot(c >> 1), bv = __ballot(v);
         cv += (uint32_t)__popcll(bv);
         c1 += (uint32_t)__popcll(b0 & ~b1);
@@ -661,9 +683,23 @@ __global__ void seg_finalize_kernel(int mode, uint32_t S, uint32_t chunks, uint3
     double v[DB];
 #pragma unroll
     for (int b = 0; b < DB; ++b) v[b] = 0.0;
-    for (uint32_t k = 0; k < chunks; ++k) {
+    // Eight chunks' loads in flight; the additions keep chunk order.
+    const double* ps = partial + (size_t)s * chunks * DB;
+    uint32_t k = 0;
+    for (; k + 8 <= chunks; k += 8) {
+        double t[8][DB];
 #pragma unroll
-        for (int b = 0; b < DB; ++b) v[b] += partial[((size_t)s * chunks + k) * DB + b];
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int b = 0; b < DB; ++b) t[u][b] = ps[(size_t)(k + u) * DB + b];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int b = 0; b < DB; ++b) v[b] += t[u][b];
+    }
+    for (; k < chunks; ++k) {
+#pragma unroll
+        for (int b = 0; b < DB; ++b) v[b] += ps[(size_t)k * DB + b];
     }
     float* nodeBase = nullptr;
     if (ballNodes) {
@@ -689,8 +725,9 @@ __global__ void seg_finalize_kernel(int mode, uint32_t S, uint32_t chunks, uint3
         }
         if (pmax && nodeBase) {
             float m = 0.0f;
-            for (uint32_t k = 0; k < chunks; ++k) {
-                const float x = pmax[(size_t)s * chunks + k];
+#pragma unroll 8
+            for (uint32_t c = 0; c < chunks; ++c) {
+                const float x = pmax[(size_t)s * chunks + c];
                 m = x > m ? x : m;
             }
             // Certified upper bound of the exact radius: every computed d2 is
