@@ -180,14 +180,23 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
     float maxcap = ((mb >> 31) == 0 && mb < 0x7F800001u) ? S.maxd2 : INFINITY;
     if (!SEED && S.seed && active) maxcap = fminf(maxcap, __uint_as_float(S.seed[qi]));
 
-    // This lane's K smallest upper bounds, ascending; KC-K leading -inf pads
-    // keep the K-th at a static slot.  The K slots start at the cap (the
-    // seeding pass's bound when there is one), itself an upper bound of the
-    // K-th distance: the running bound is then min(cap, K-th smallest bound
-    // seen), still an upper bound, and only pairs below the cap are inserted.
-    float ul[KC];
+    // This lane's H = ceil(K/2) smallest upper bounds, ascending; HC-H
+    // leading -inf pads keep the H-th at a static slot.  The two lanes of a
+    // query see disjoint points, so the K-th smallest of the union of their
+    // two lists bounds the K-th distance: max(a_H, b_H) for even K and
+    // min(max(a_H-1, b_H), max(a_H, b_H-1)) for odd K (a, b: the two lists).
+    // That is about the K-th smallest bound of the pairs seen, where each
+    // lane keeping K bounds and taking the smaller K-th is about the 2K-th.
+    // The H slots start at the cap (the seeding pass's bound when there is
+    // one), itself an upper bound of the K-th distance, so the running bound
+    // stays one with the cap in place of missing bounds, and only pairs below
+    // the lane's own H-th are inserted.
+    constexpr int HC = (KC + 1) / 2;
+    const int H = (int)(S.K + 1) / 2;
+    const bool oddK = (S.K & 1u) != 0u;
+    float ul[HC];
 #pragma unroll
-    for (int i = 0; i < KC; ++i) ul[i] = (i >= KC - (int)S.K) ? maxcap : -INFINITY;
+    for (int i = 0; i < HC; ++i) ul[i] = (i >= HC - H) ? maxcap : -INFINITY;
     float lane_tau = maxcap;
     float tau = maxcap;
     // This lane's candidate list and its length.
@@ -313,12 +322,12 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                             if (B[r] < Ithr) {
                                 float key = (B[r] + EtB) * S.P.up;
 #pragma unroll
-                                for (int i = 0; i < KC; ++i) {
+                                for (int i = 0; i < HC; ++i) {
                                     const float e = ul[i];
                                     ul[i] = fminf(e, key);
                                     key = fmaxf(e, key);
                                 }
-                                lane_tau = ul[KC - 1];
+                                lane_tau = ul[HC - 1];
                                 Ithr = lane_tau * S.P.inv_up - EtB;
                                 gate = SEED ? Ithr : (!(Tb <= Ithr) ? Tb : Ithr);
                             }
@@ -330,13 +339,22 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
                 else
                     pairs(std::false_type{});
             }
-            // Both lanes of a query hold upper bounds of the same K-th
-            // distance: swap halves (lane l <-> l ^ 32) without LDS; the min
-            // of the two swapped copies is min(own, other) on every lane.
+            // Combine the two lanes' lists (lane l <-> l ^ 32, without LDS).
+            // v_permlane32_swap of a value with itself gives lanes 0-31
+            // (own, other) and lanes 32-63 (other, own); max is symmetric.
             {
                 const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(lane_tau),
                                                                  __float_as_uint(lane_tau), false, false);
-                tau = fminf(fminf(__uint_as_float(sw[0]), __uint_as_float(sw[1])), maxcap);
+                float t = fmaxf(__uint_as_float(sw[0]), __uint_as_float(sw[1]));
+                if (oddK) {
+                    const float a0 = HC >= 2 ? ul[HC >= 2 ? HC - 2 : 0] : -INFINITY;
+                    const auto s0 = __builtin_amdgcn_permlane32_swap(__float_as_uint(a0), __float_as_uint(a0),
+                                                                     false, false);
+                    const float b1 = __uint_as_float(half ? sw[0] : sw[1]);
+                    const float b0 = __uint_as_float(half ? s0[0] : s0[1]);
+                    t = fminf(fmaxf(a0, b1), fmaxf(lane_tau, b0));
+                }
+                tau = fminf(t, maxcap);
             }
             acc = acc_next;
         }

@@ -639,15 +639,15 @@ __global__ void seg_reduce_kernel(const float* __restrict__ X, uint32_t N, uint3
 #pragma unroll
             for (int b = 0; b < DB; ++b) x[b] = dbase + b < D ? X[(size_t)(dbase + b) * N + j] : 0.0f;
         };
-        // Four positions' loads in flight per lane; the additions keep the
+        // Eight positions' loads in flight per lane; the additions keep the
         // order of the plain loop.
         uint32_t j = b0 + r;
-        for (; j + 3 * G < e0; j += 4 * G) {
-            float x[4][DB];
+        for (; j + 7 * G < e0; j += 8 * G) {
+            float x[8][DB];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) load(x[u], j + u * G);
+            for (int u = 0; u < 8; ++u) load(x[u], j + u * G);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) add(x[u], j + u * G);
+            for (int u = 0; u < 8; ++u) add(x[u], j + u * G);
         }
         for (; j < e0; j += G) {
             float x[DB];

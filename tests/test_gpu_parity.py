@@ -368,7 +368,7 @@ def test_bf16_form_equals_plain(oracle, cuda, d):
     qs[1] = pts[7] + np.float32(1e-6)      # near-coincident
     qs[2] = pts[8] * np.float32(1 + 2.0 ** -20)
     bf = knn.BruteForceGpu(dev(pts, cuda))
-    for k in (1, 8, 32):
+    for k in (1, 5, 8, 31, 32):
         for heap in (False, True):
             for md in (np.inf, float(np.float32(np.sum(scale ** 2)))):
                 want = oracle.bf_knn(pts, qs, k, max_d2=md, heap=heap)
