@@ -267,97 +267,111 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
             }
             return a;
         };
-        f32x16 acc = tile_dot(0), acc_next = acc;
-        for (uint32_t u = 0; u < nt; ++u) {
+        // Epilogue of tile u (accumulators a).  The common case, in which no
+        // lane's pair passes the gate, is straight-line code (one compare per
+        // pair and one wave vote), so the compiler can interleave it with the
+        // next tile's MFMAs; the pair loop runs only when some lane passes or
+        // the tile has rows beyond n.
+        auto epilogue = [&](const f32x16& a, uint32_t u) {
             const uint32_t t = (tb + u) * ts;
-            if (u + 1 < nt) acc_next = tile_dot(u + 1);
-            if (active) {
-                // This lane's 16 rows are 8j + 4 half + i (j, i < 4): four
-                // 16-byte LDS reads, all issued before any use.
-                float4 n4[4];
+            // This lane's 16 rows are 8j + 4 half + i (j, i < 4): four
+            // 16-byte LDS reads, all issued before any use.
+            float4 n4[4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    n4[j] = __builtin_bit_cast(float4, *reinterpret_cast<const uint4*>(s_np + u * 32 + 8 * j + 4 * half));
-                const float cq = qn + s_npmax[u];
-                const float Et = S.P.cA * cq + S.P.absE;
-                const float T = (tau * S.P.inv_dn + Et) * 0x1.000004p+0f;  // (1 + 4u)
-                // Per pair only B = |p|^2 - 2 s (one FMA) is formed; the
-                // tests move qn to the thresholds.  With A = fl(fl(qn + |p|^2)
-                // - 2 s), |A - (B + qn)| <= 5.01 u cq (|s| <= (qn + |p|^2) / 2
-                // up to the screen's own 1e-4 error), so the margins below
-                // (>= 14 u (cq + T) after their own roundings; 2^-100 where
-                // the scaled terms would be subnormal) keep every A <= T pair
-                // at B <= Tb and keep (B + EtB) up >= (A + Et) up.
-                const float mK = cq * 0x1.0p-20f + 0x1.0p-100f;
-                const float Tb = (T - qn) + ((cq + T) * 0x1.0p-20f + 0x1.0p-100f);
-                const float EtB = (Et + qn) + mK;
-                float Ithr = lane_tau * S.P.inv_up - EtB;
-                float B[16];
+            for (int j = 0; j < 4; ++j)
+                n4[j] = __builtin_bit_cast(float4, *reinterpret_cast<const uint4*>(s_np + u * 32 + 8 * j + 4 * half));
+            const float cq = qn + s_npmax[u];
+            const float Et = S.P.cA * cq + S.P.absE;
+            const float T = (tau * S.P.inv_dn + Et) * 0x1.000004p+0f;  // (1 + 4u)
+            // Per pair only B = |p|^2 - 2 s (one FMA) is formed; the
+            // tests move qn to the thresholds.  With A = fl(fl(qn + |p|^2)
+            // - 2 s), |A - (B + qn)| <= 5.01 u cq (|s| <= (qn + |p|^2) / 2
+            // up to the screen's own 1e-4 error), so the margins below
+            // (>= 14 u (cq + T) after their own roundings; 2^-100 where
+            // the scaled terms would be subnormal) keep every A <= T pair
+            // at B <= Tb and keep (B + EtB) up >= (A + Et) up.
+            const float mK = cq * 0x1.0p-20f + 0x1.0p-100f;
+            const float Tb = (T - qn) + ((cq + T) * 0x1.0p-20f + 0x1.0p-100f);
+            const float EtB = (Et + qn) + mK;
+            float Ithr = lane_tau * S.P.inv_up - EtB;
+            float B[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const float4 v = n4[r >> 2];
+                const float np = (r & 3) == 0 ? v.x : (r & 3) == 1 ? v.y : (r & 3) == 2 ? v.z : v.w;
+                B[r] = __builtin_fmaf(-2.0f, a[r], np);  // |p|^2 - 2 s, rounded once
+            }
+            // Rows at or beyond n exist only in the last tile.
+            const uint32_t nvalid = S.n - t * 32;
+            // One gate per pair, against the larger of the two thresholds
+            // (NaN when Tb is: a NaN B or a NaN Tb passes it, as each
+            // passes the emit test); the two tests run only behind it.
+            float gate = SEED ? Ithr : (!(Tb <= Ithr) ? Tb : Ithr);
+            bool any = false;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) any |= !(B[r] > gate);
+            if (__ballot(active && any) == 0ull && nvalid >= 32) return;
+            if (!active) return;
+            auto pairs = [&](auto all_valid) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const float4 v = n4[r >> 2];
-                    const float np = (r & 3) == 0 ? v.x : (r & 3) == 1 ? v.y : (r & 3) == 2 ? v.z : v.w;
-                    B[r] = __builtin_fmaf(-2.0f, acc[r], np);  // |p|^2 - 2 s, rounded once
-                }
-                // Rows at or beyond n exist only in the last tile: the row
-                // test is compiled into a separate copy of the loop, chosen by
-                // a tile-uniform branch.
-                const uint32_t nvalid = S.n - t * 32;
-                // One gate per pair, against the larger of the two thresholds
-                // (NaN when Tb is: a NaN B or a NaN Tb passes it, as each
-                // passes the emit test); the two tests run only behind it.
-                float gate = SEED ? Ithr : (!(Tb <= Ithr) ? Tb : Ithr);
-                auto pairs = [&](auto all_valid) {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
-                        const bool valid = decltype(all_valid)::value || row < nvalid;
-                        if (!(B[r] > gate) && valid) {
-                            if constexpr (!SEED) {
-                                if (!(B[r] > Tb)) {
-                                    if (count < S.cap) mine[count] = t * 32 + row;
-                                    ++count;
-                                }
-                            }
-                            if (B[r] < Ithr) {
-                                float key = (B[r] + EtB) * S.P.up;
-#pragma unroll
-                                for (int i = 0; i < HC; ++i) {
-                                    const float e = ul[i];
-                                    ul[i] = fminf(e, key);
-                                    key = fmaxf(e, key);
-                                }
-                                lane_tau = ul[HC - 1];
-                                Ithr = lane_tau * S.P.inv_up - EtB;
-                                gate = SEED ? Ithr : (!(Tb <= Ithr) ? Tb : Ithr);
+                    const uint32_t row = 8 * (r >> 2) + 4 * half + (r & 3);
+                    const bool valid = decltype(all_valid)::value || row < nvalid;
+                    if (!(B[r] > gate) && valid) {
+                        if constexpr (!SEED) {
+                            if (!(B[r] > Tb)) {
+                                if (count < S.cap) mine[count] = t * 32 + row;
+                                ++count;
                             }
                         }
+                        if (B[r] < Ithr) {
+                            float key = (B[r] + EtB) * S.P.up;
+#pragma unroll
+                            for (int i = 0; i < HC; ++i) {
+                                const float e = ul[i];
+                                ul[i] = fminf(e, key);
+                                key = fmaxf(e, key);
+                            }
+                            lane_tau = ul[HC - 1];
+                            Ithr = lane_tau * S.P.inv_up - EtB;
+                            gate = SEED ? Ithr : (!(Tb <= Ithr) ? Tb : Ithr);
+                        }
                     }
-                };
-                if (nvalid >= 32)
-                    pairs(std::true_type{});
-                else
-                    pairs(std::false_type{});
-            }
-            // Combine the two lanes' lists (lane l <-> l ^ 32, without LDS).
-            // v_permlane32_swap of a value with itself gives lanes 0-31
-            // (own, other) and lanes 32-63 (other, own); max is symmetric.
-            {
-                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(lane_tau),
-                                                                 __float_as_uint(lane_tau), false, false);
-                float t = fmaxf(__uint_as_float(sw[0]), __uint_as_float(sw[1]));
-                if (oddK) {
-                    const float a0 = HC >= 2 ? ul[HC >= 2 ? HC - 2 : 0] : -INFINITY;
-                    const auto s0 = __builtin_amdgcn_permlane32_swap(__float_as_uint(a0), __float_as_uint(a0),
-                                                                     false, false);
-                    const float b1 = __uint_as_float(half ? sw[0] : sw[1]);
-                    const float b0 = __uint_as_float(half ? s0[0] : s0[1]);
-                    t = fminf(fmaxf(a0, b1), fmaxf(lane_tau, b0));
                 }
-                tau = fminf(t, maxcap);
+            };
+            if (nvalid >= 32)
+                pairs(std::true_type{});
+            else
+                pairs(std::false_type{});
+        };
+        // Combine the two lanes' lists (lane l <-> l ^ 32, without LDS).
+        // v_permlane32_swap of a value with itself gives lanes 0-31
+        // (own, other) and lanes 32-63 (other, own); max is symmetric.
+        auto combine = [&]() {
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(lane_tau),
+                                                             __float_as_uint(lane_tau), false, false);
+            float t = fmaxf(__uint_as_float(sw[0]), __uint_as_float(sw[1]));
+            if (oddK) {
+                const float a0 = HC >= 2 ? ul[HC >= 2 ? HC - 2 : 0] : -INFINITY;
+                const auto s0 = __builtin_amdgcn_permlane32_swap(__float_as_uint(a0), __float_as_uint(a0),
+                                                                 false, false);
+                const float b1 = __uint_as_float(half ? sw[0] : sw[1]);
+                const float b0 = __uint_as_float(half ? s0[0] : s0[1]);
+                t = fminf(fmaxf(a0, b1), fmaxf(lane_tau, b0));
             }
+            tau = fminf(t, maxcap);
+        };
+        // The last tile of the round is peeled, so that the next tile's
+        // MFMAs are issued unconditionally in the loop body.
+        f32x16 acc = tile_dot(0);
+        for (uint32_t u = 0; u + 1 < nt; ++u) {
+            const f32x16 acc_next = tile_dot(u + 1);
+            epilogue(acc, u);
+            combine();
             acc = acc_next;
         }
+        epilogue(acc, nt - 1);
+        combine();
     }
     if constexpr (SEED) {
         // tau >= 0 (upper bounds of distances) or +inf: its bits order as floats.
