@@ -396,9 +396,7 @@ __device__ __forceinline__ bool bf16_rerank_list(const Bf16Set& S, uint32_t qi, 
     if (ok) {
         const float* q = S.qs + (size_t)qi * S.D;
         const uint32_t* list = S.cand + li * S.cap;
-        for (uint32_t j = 0; j < c; ++j) {
-            const uint32_t pt = list[j];
-            const float d2 = dist2_rows(q, S.pts + (size_t)pt * S.D, S.D);
+        auto offer = [&](float d2, uint32_t pt) {
             if (f2u(d2) <= whi) {
                 const uint64_t key = make_key(d2, pt);
                 if (key < worst) {
@@ -406,6 +404,19 @@ __device__ __forceinline__ bool bf16_rerank_list(const Bf16Set& S, uint32_t qi, 
                     whi = key_hi(worst);
                 }
             }
+        };
+        // Two candidates at a time (two rows in flight), offered in list order.
+        uint32_t j = 0;
+        for (; j + 1 < c; j += 2) {
+            const uint32_t p0 = list[j], p1 = list[j + 1];
+            float d0, d1;
+            dist2_rows2(q, S.pts + (size_t)p0 * S.D, S.pts + (size_t)p1 * S.D, S.D, d0, d1);
+            offer(d0, p0);
+            offer(d1, p1);
+        }
+        if (j < c) {
+            const uint32_t pt = list[j];
+            offer(dist2_rows(q, S.pts + (size_t)pt * S.D, S.D), pt);
         }
     }
     buf.write_keys(out, S.K);
@@ -424,19 +435,30 @@ __device__ __forceinline__ bool bf16_rerank_one(const Bf16Set& S, uint32_t qi) {
     uint64_t worst = buf.init(empty, S.K);
     uint32_t whi = key_hi(worst);
     const float* q = S.qs + (size_t)qi * S.D;
+    auto offer = [&](float d2, uint32_t pt) {
+        if (f2u(d2) <= whi) {
+            const uint64_t key = make_key(d2, pt);
+            if (key < worst) {
+                worst = buf.consider(key);
+                whi = key_hi(worst);
+            }
+        }
+    };
     for (uint32_t l = 0; l < S.lists; ++l) {
         const uint32_t* list = S.cand + ((size_t)qi * S.lists + l) * S.cap;
         const uint32_t c = cnt[l];
-        for (uint32_t j = 0; j < c; ++j) {
+        // Two candidates at a time (two rows in flight), offered in list order.
+        uint32_t j = 0;
+        for (; j + 1 < c; j += 2) {
+            const uint32_t p0 = list[j], p1 = list[j + 1];
+            float d0, d1;
+            dist2_rows2(q, S.pts + (size_t)p0 * S.D, S.pts + (size_t)p1 * S.D, S.D, d0, d1);
+            offer(d0, p0);
+            offer(d1, p1);
+        }
+        if (j < c) {
             const uint32_t pt = list[j];
-            const float d2 = dist2_rows(q, S.pts + (size_t)pt * S.D, S.D);
-            if (f2u(d2) <= whi) {
-                const uint64_t key = make_key(d2, pt);
-                if (key < worst) {
-                    worst = buf.consider(key);
-                    whi = key_hi(worst);
-                }
-            }
+            offer(dist2_rows(q, S.pts + (size_t)pt * S.D, S.D), pt);
         }
     }
     buf.write(S.oidx + (size_t)qi * S.K, S.odist + (size_t)qi * S.K, S.K);

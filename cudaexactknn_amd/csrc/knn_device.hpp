@@ -93,6 +93,60 @@ __device__ __forceinline__ float dist2_rows(const float* __restrict__ q, const f
     return dist2_rt(q, 1, p, D);
 }
 
+// dist2_rows of two points against one query, the two sums formed side by
+// side (each in the same order as dist2_rows, so each equals it bit for bit):
+// two rows' loads in flight and each query load used twice.
+__device__ __forceinline__ void dist2_rows2(const float* __restrict__ q, const float* __restrict__ p0,
+                                            const float* __restrict__ p1, uint32_t D, float& r0, float& r1) {
+#pragma clang fp contract(off)
+    if (((D & 3u) | (((uintptr_t)q | (uintptr_t)p0 | (uintptr_t)p1) & 15u)) == 0) {
+        const float4* __restrict__ q4 = reinterpret_cast<const float4*>(q);
+        const float4* __restrict__ a4 = reinterpret_cast<const float4*>(p0);
+        const float4* __restrict__ b4 = reinterpret_cast<const float4*>(p1);
+        float4 c = q4[0], a = a4[0], b = b4[0];
+        float t = c.x - a.x, u = c.x - b.x;
+        float x = t * t, y = u * u;
+        t = c.y - a.y;
+        u = c.y - b.y;
+        x = x + t * t;
+        y = y + u * u;
+        t = c.z - a.z;
+        u = c.z - b.z;
+        x = x + t * t;
+        y = y + u * u;
+        t = c.w - a.w;
+        u = c.w - b.w;
+        x = x + t * t;
+        y = y + u * u;
+        for (uint32_t i = 1; i < (D >> 2); ++i) {
+            c = q4[i];
+            a = a4[i];
+            b = b4[i];
+            t = c.x - a.x;
+            u = c.x - b.x;
+            x = x + t * t;
+            y = y + u * u;
+            t = c.y - a.y;
+            u = c.y - b.y;
+            x = x + t * t;
+            y = y + u * u;
+            t = c.z - a.z;
+            u = c.z - b.z;
+            x = x + t * t;
+            y = y + u * u;
+            t = c.w - a.w;
+            u = c.w - b.w;
+            x = x + t * t;
+            y = y + u * u;
+        }
+        r0 = x;
+        r1 = y;
+        return;
+    }
+    r0 = dist2_rows(q, p0, D);
+    r1 = dist2_rows(q, p1, D);
+}
+
 // Select q[dim] for a runtime dim without dynamic register indexing.
 template <int D>
 __device__ __forceinline__ float select_dim(const float (&q)[D], uint32_t dim) {
