@@ -316,11 +316,11 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     S.odist = a.odist;
     // Seeding pass over every 16th tile when the point set is large and
     // D > 16: it gives each query a starting bound near the K-th distance of
-    // a 1/16 sample, so the full pass emits about 3x fewer candidates while
-    // its own bound tightens (measured: D = 128, 151 -> 134 ms).  At D <= 16
-    // candidates are cheap to rerank and the extra pass costs more than it
-    // saves (45 -> 51 ms).
-    constexpr uint32_t kSeedStride = 16;
+    // a 1/32 sample, so the full pass emits fewer candidates while its own
+    // bound tightens (D = 128: 1/32 62.5 ms, 1/16 63.6 ms, 1/8 65.3 ms).  At
+    // D <= 16 candidates are cheap to rerank and the extra pass costs more
+    // than it saves.
+    constexpr uint32_t kSeedStride = 32;
     const bool seeded = tiles >= kSeedStride * 64 && D16 >= 2;
     EventTimer timer(a.timed, a.stream);
     h.stats[5] = seeded ? 1.0f : 0.0f;

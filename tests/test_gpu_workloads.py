@@ -130,5 +130,5 @@ def test_bf16_seeded_screen_against_oracle(oracle, cuda, d):
             same(got, want, f"bf16 seeded d={d} heap={heap} md={radius}")
             st = bf.last_query_stats()
             assert st["bf16_seeded"] == 1.0, st
-    for k in (1, 16):
+    for k in (1, 5, 16):
         same(run(bf, qs, k, cuda, form=knn.FORM_BF16), oracle.bf_knn(pts, qs, k), f"bf16 seeded d={d} k={k}")
