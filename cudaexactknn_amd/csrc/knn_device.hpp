@@ -94,8 +94,11 @@ __device__ __forceinline__ float dist2_rows(const float* __restrict__ q, const f
 }
 
 // dist2_rows of two points against one query, the two sums formed side by
-// side (each in the same order as dist2_rows, so each equals it bit for bit):
-// two rows' loads in flight and each query load used twice.
+// side (each in the same order as dist2_rows, so each equals it bit for bit;
+// a sum started at +0 equals one started at the first square, which is never
+// -0).  The rows are read 128 bytes (one cache line) at a time, all eight
+// 16-byte loads of a line issued together, so a line is used whole as soon
+// as it arrives instead of being fetched again after an eviction.
 __device__ __forceinline__ void dist2_rows2(const float* __restrict__ q, const float* __restrict__ p0,
                                             const float* __restrict__ p1, uint32_t D, float& r0, float& r1) {
 #pragma clang fp contract(off)
@@ -103,27 +106,10 @@ __device__ __forceinline__ void dist2_rows2(const float* __restrict__ q, const f
         const float4* __restrict__ q4 = reinterpret_cast<const float4*>(q);
         const float4* __restrict__ a4 = reinterpret_cast<const float4*>(p0);
         const float4* __restrict__ b4 = reinterpret_cast<const float4*>(p1);
-        float4 c = q4[0], a = a4[0], b = b4[0];
-        float t = c.x - a.x, u = c.x - b.x;
-        float x = t * t, y = u * u;
-        t = c.y - a.y;
-        u = c.y - b.y;
-        x = x + t * t;
-        y = y + u * u;
-        t = c.z - a.z;
-        u = c.z - b.z;
-        x = x + t * t;
-        y = y + u * u;
-        t = c.w - a.w;
-        u = c.w - b.w;
-        x = x + t * t;
-        y = y + u * u;
-        for (uint32_t i = 1; i < (D >> 2); ++i) {
-            c = q4[i];
-            a = a4[i];
-            b = b4[i];
-            t = c.x - a.x;
-            u = c.x - b.x;
+        const uint32_t n4 = D >> 2;
+        float x = 0.0f, y = 0.0f;
+        auto step = [&](const float4& c, const float4& a, const float4& b) {
+            float t = c.x - a.x, u = c.x - b.x;
             x = x + t * t;
             y = y + u * u;
             t = c.y - a.y;
@@ -138,7 +124,19 @@ __device__ __forceinline__ void dist2_rows2(const float* __restrict__ q, const f
             u = c.w - b.w;
             x = x + t * t;
             y = y + u * u;
+        };
+        uint32_t i = 0;
+        for (; i + 8 <= n4; i += 8) {
+            float4 a[8], b[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                a[k] = a4[i + k];
+                b[k] = b4[i + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) step(q4[i + k], a[k], b[k]);
         }
+        for (; i < n4; ++i) step(q4[i], a4[i], b4[i]);
         r0 = x;
         r1 = y;
         return;
