@@ -62,16 +62,16 @@ __global__ void bf16_norms_kernel(const float* __restrict__ pts, uint32_t n, uin
     if ((i & 31) == 0 && i / 32 < tiles) out[n + i / 32] = r;
 }
 
-// grid (query blocks of 128, point chunks); chunk y screens tiles
+// grid (query blocks of 32 NW, point chunks); chunk y screens tiles
 // [y * tiles_per_chunk, ...) of the `vtiles` tiles v * ts.
-template <int D16, int KC, bool SEED>
-__global__ void __launch_bounds__(256) bf16_screen_kernel(Bf16Set S, uint32_t tiles_per_chunk, uint32_t stage,
-                                                          uint32_t ts, uint32_t vtiles) {
+template <int D16, int KC, bool SEED, int NW>
+__global__ void __launch_bounds__(64 * NW) bf16_screen_kernel(Bf16Set S, uint32_t tiles_per_chunk, uint32_t stage,
+                                                              uint32_t ts, uint32_t vtiles) {
     // Dynamic LDS of bf16_stage_bytes(D16, stage).
     extern __shared__ __align__(16) uint8_t lds_stage[];
     const uint32_t t0 = blockIdx.y * tiles_per_chunk;
     const uint32_t t1 = min(vtiles, t0 + tiles_per_chunk);
-    bf16_screen_run<D16, KC, SEED>(S, blockIdx.x, blockIdx.y, t0, t1, stage, lds_stage, ts);
+    bf16_screen_run<D16, KC, SEED, NW>(S, blockIdx.x, blockIdx.y, t0, t1, stage, lds_stage, ts);
 }
 
 // Exact rerank, one lane per (query, candidate list): each lane writes its
@@ -140,11 +140,18 @@ void prepare_bf16(Handle& h, hipStream_t st) {
 
 using ScreenFn = void (*)(Bf16Set, uint32_t, uint32_t, uint32_t, uint32_t);
 
+// Waves (of 32 queries each) per screen workgroup.  At D > 64 eight waves
+// share each staged tile (halving the L2 -> LDS staging per query; D = 128:
+// 58.7 -> 56.7 ms); below, four keep the launch bound from capping the
+// registers at two waves per SIMD (D = 64 with eight: 36.6 -> 39.0 ms).
+constexpr uint32_t bf16_screen_waves(uint32_t D16) { return D16 >= 8 ? 8u : 4u; }
+
 template <int D16, bool SEED>
 static ScreenFn screen_for(uint32_t K) {
-    if (K <= 8) return bf16_screen_kernel<D16, 8, SEED>;
-    if (K <= 16) return bf16_screen_kernel<D16, 16, SEED>;
-    return bf16_screen_kernel<D16, 32, SEED>;
+    constexpr int NW = (int)bf16_screen_waves(D16);
+    if (K <= 8) return bf16_screen_kernel<D16, 8, SEED, NW>;
+    if (K <= 16) return bf16_screen_kernel<D16, 16, SEED, NW>;
+    return bf16_screen_kernel<D16, 32, SEED, NW>;
 }
 
 using RerankFn = void (*)(Bf16Set, uint64_t*, uint32_t*, uint32_t*, uint32_t*);
@@ -175,7 +182,7 @@ Bf16Params bf16_params(uint32_t D, uint32_t D16) {
 }
 
 struct Bf16Config {
-    uint32_t D16, tiles, stage, want;
+    uint32_t D16, tiles, stage, want, threads;
     size_t stage_lds;
     ScreenFn fn, sfn;
 };
@@ -186,6 +193,7 @@ static Bf16Config bf16_config(const Handle& h, uint32_t K, uint32_t tile) {
     Bf16Config c{};
     c.D16 = bf16_d16_of(h.info.dims);
     c.tiles = (h.info.n + 31) / 32;
+    c.threads = 64 * bf16_screen_waves(c.D16);
     uint32_t max_stage = 1;
     switch (c.D16) {
         case 1: c.fn = screen_for<1, false>(K); c.sfn = screen_for<1, true>(K); max_stage = bf16_stage_tiles_wide<1>(); break;
@@ -207,8 +215,8 @@ static Bf16Config bf16_config(const Handle& h, uint32_t K, uint32_t tile) {
     // fraction of their occupancy (3 resident at D <= 64 gave a 4-workgroup
     // CU a second round at a third of its waves).
     int per_cu = 0;
-    KNN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(c.fn), 256,
-                                                         c.stage_lds));
+    KNN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(c.fn),
+                                                         (int)c.threads, c.stage_lds));
     c.want = 2 * device_limits().cu_count * (uint32_t)(per_cu > 0 ? per_cu : 2);
     return c;
 }
@@ -218,9 +226,9 @@ struct Bf16Split {
     size_t bytes;  // scratch for nq queries
 };
 
-static Bf16Split bf16_split(uint32_t n, uint32_t nq, uint32_t K, uint32_t tiles, uint32_t want) {
+static Bf16Split bf16_split(uint32_t n, uint32_t nq, uint32_t K, uint32_t tiles, uint32_t want, uint32_t qpb) {
     Bf16Split r{};
-    const uint32_t qblocks = (nq + 127) / 128;
+    const uint32_t qblocks = (nq + qpb - 1) / qpb;
     uint32_t S = 1;
     if (qblocks < want) {
         S = (want + qblocks - 1) / qblocks;
@@ -257,7 +265,7 @@ void bf16_reserve(Handle& h, uint32_t nq, uint32_t K, uint32_t tile, hipStream_t
     const uint32_t blocks = (nq + 127) / 128;
     for (uint32_t g = 1; g <= blocks; ++g) {
         const uint32_t m = (uint32_t)std::min<size_t>(nq, (size_t)g * 128);
-        most = std::max(most, bf16_split(n, m, K, c.tiles, c.want).bytes);
+        most = std::max(most, bf16_split(n, m, K, c.tiles, c.want, c.threads / 2).bytes);
     }
     h.q_scratch.ensure(most);
 }
@@ -278,8 +286,9 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     const size_t words = (size_t)tiles * D16 * 64;
     const uint4* phi = reinterpret_cast<const uint4*>(h.pts_bf16.get());
     const uint4* plo = phi + words;
-    const uint32_t qblocks = (a.nq + 127) / 128;
-    const Bf16Split sp = bf16_split(n, a.nq, a.K, tiles, c.want);
+    const uint32_t qpb = c.threads / 2;  // 32 queries per wave
+    const uint32_t qblocks = (a.nq + qpb - 1) / qpb;
+    const Bf16Split sp = bf16_split(n, a.nq, a.K, tiles, c.want, qpb);
     const uint32_t S_ = sp.S, lists = sp.lists, cap = sp.cap;
     const uint32_t tpc = (tiles + S_ - 1) / S_;
     const size_t partial_words = (size_t)a.nq * lists * a.K;
@@ -314,7 +323,7 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
     S.lists = lists;
     S.oidx = a.oidx;
     S.odist = a.odist;
-    // Seeding pass over every 16th tile when the point set is large and
+    // Seeding pass over every 32nd tile when the point set is large and
     // D > 16: it gives each query a starting bound near the K-th distance of
     // a 1/32 sample, so the full pass emits fewer candidates while its own
     // bound tightens (D = 128: 1/32 62.5 ms, 1/16 63.6 ms, 1/8 65.3 ms).  At
@@ -329,9 +338,10 @@ void query_bf_bf16(Handle& h, const QueryArgs& a) {
         launch_fill_u32(seed, a.nq, 0x7F800000u, a.stream);  // +inf
         const uint32_t vt = (tiles + kSeedStride - 1) / kSeedStride;
         const uint32_t vpc = (vt + S_ - 1) / S_;
-        hipLaunchKernelGGL(sfn, dim3(qblocks, S_), dim3(256), stage_lds, a.stream, S, vpc, stage, kSeedStride, vt);
+        hipLaunchKernelGGL(sfn, dim3(qblocks, S_), dim3(c.threads), stage_lds, a.stream, S, vpc, stage, kSeedStride,
+                           vt);
     }
-    hipLaunchKernelGGL(fn, dim3(qblocks, S_), dim3(256), stage_lds, a.stream, S, tpc, stage, 1u, tiles);
+    hipLaunchKernelGGL(fn, dim3(qblocks, S_), dim3(c.threads), stage_lds, a.stream, S, tpc, stage, 1u, tiles);
     const size_t lanes = (size_t)a.nq * lists;
     hipLaunchKernelGGL(rerank_for(a.K), dim3((uint32_t)((lanes + 255) / 256)), dim3(256), 0, a.stream, S, partial,
                        ovf_flag, ovf, novf);

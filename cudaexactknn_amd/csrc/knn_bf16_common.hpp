@@ -130,21 +130,23 @@ __device__ __forceinline__ void dma_to_lds(const void* g, void* lds) {
         __builtin_amdgcn_global_load_lds((const global_void_t*)g, (lds_void_t*)lds, 4, 0, 0);
 }
 
-// Screen tiles [t0, t1) for the 128 queries of `qblock`, `stage` tiles per
-// barrier pair (s_hi, s_lo, s_np sized for it: bf16_stage_bytes).  Every thread of the
-// 256-wide workgroup must call it (it synchronises the workgroup).
+// Screen tiles [t0, t1) for the 32 NW queries of `qblock` (NW waves of 32
+// queries), `stage` tiles per barrier pair (s_hi, s_lo, s_np sized for it:
+// bf16_stage_bytes).  Every thread of the 64 NW-wide workgroup must call it
+// (it synchronises the workgroup).  More waves per workgroup share each
+// staged tile among more queries.
 // Tile v of the range is point tile v * ts.  SEED: a pass over every ts-th
 // tile that emits nothing and only lowers S.seed[q] to the query's running
 // bound: the K-th smallest upper bound over a subset of the points is an
 // upper bound of the K-th distance, so the full pass may start from it.
-template <int D16, int KC, bool SEED = false>
+template <int D16, int KC, bool SEED = false, int NW = 4>
 __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qblock, uint32_t chunk, uint32_t t0,
                                                 uint32_t t1, uint32_t stage, uint8_t* __restrict__ lds,
                                                 uint32_t ts = 1) {
 #pragma clang fp contract(off)
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t qcol = lane & 31, half = lane >> 5;
-    const uint32_t qi = qblock * 128 + wave * 32 + qcol;
+    const uint32_t qi = qblock * (32 * NW) + wave * 32 + qcol;
     const bool active = qi < S.nq;
     const uint32_t D = S.D;
 
@@ -215,7 +217,7 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
     uint4* __restrict__ lo0 = hi0 + 2 * (size_t)stage * W;
     float* __restrict__ np0 = reinterpret_cast<float*>(lo0 + 2 * (size_t)stage * W);
     // Issue the copies of the round starting at tile tb into buffer b: wave w
-    // copies words w * 64 + 256 j (a round's word count is a multiple of 64,
+    // copies words w * 64 + 64 NW j (a round's word count is a multiple of 64,
     // so a wave copies whole 64-word runs); norms past n are not read (those
     // rows are never tested).
     auto issue = [&](uint32_t tb, uint32_t b) {
@@ -223,7 +225,7 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
         const uint32_t words = nt * W;
         uint4* dh = hi0 + (size_t)b * stage * W;
         uint4* dl = lo0 + (size_t)b * stage * W;
-        for (uint32_t i0 = wave * 64; i0 < words; i0 += 256) {
+        for (uint32_t i0 = wave * 64; i0 < words; i0 += 64 * NW) {
             const uint32_t i = i0 + lane;
             const uint32_t u = i / W;
             const size_t src = (size_t)(tb + u) * ts * W + (i - u * W);
@@ -231,13 +233,13 @@ __device__ __forceinline__ void bf16_screen_run(const Bf16Set& S, uint32_t qbloc
             dma_to_lds<16>(S.plo + src, dl + i0);
         }
         float* dn = np0 + (size_t)b * bf16_np_stride(stage);
-        for (uint32_t i0 = wave * 64; i0 < nt * 32; i0 += 256) {
+        for (uint32_t i0 = wave * 64; i0 < nt * 32; i0 += 64 * NW) {
             const uint32_t i = i0 + lane;
             const uint32_t p = (tb + (i >> 5)) * ts * 32 + (i & 31);
             if (i < nt * 32 && p < S.n) dma_to_lds<4>(S.pnorm + p, dn + i0);
         }
         // Tile maxima: lane u of the last wave copies tile u's (nt <= 64).
-        if (wave == 3 && lane < nt) dma_to_lds<4>(S.ptmax + (size_t)(tb + lane) * ts, dn + stage * 32);
+        if (wave == NW - 1 && lane < nt) dma_to_lds<4>(S.ptmax + (size_t)(tb + lane) * ts, dn + stage * 32);
     };
     if (t0 < t1) issue(t0, 0);
     uint32_t buf = 0;
