@@ -96,12 +96,14 @@ struct Bf16Set {
 // must not lower; in the single-dataset screen, whose occupancy is set by
 // its registers, the depth measured fastest (1M x 65k, K = 32, ms for 1 / 2
 // / 4 / 8 / 16 tiles): D = 128: 94 / 88; D = 64: - / 69 / 64; D = 32:
-// - / - / 52 / 54; D = 16: - / - / - / 43 / 45.
+// - / - / 52 / 54; D = 16: - / - / - / 43 / 45.  With eight-wave workgroups
+// at D = 128 (one per CU) four tiles (132 KiB) beat two: 55.4 / 57.3 ms.
+// The autotuned table picks the depth per shape within this maximum.
 template <int D16>
 constexpr uint32_t bf16_stage_tiles() { return D16 >= 8 ? 1u : 8u / D16; }
 template <int D16>
 constexpr uint32_t bf16_stage_tiles_wide() {
-    return D16 == 1 ? 8u : D16 == 2 ? 4u : D16 == 4 ? 4u : D16 == 8 ? 2u : 1u;
+    return D16 == 1 ? 8u : D16 == 2 ? 4u : D16 == 4 ? 4u : D16 == 8 ? 4u : 1u;
 }
 // Floats per buffer of norms (stage x 32) and tile maxima (stage, padded to
 // a multiple of 4 so the next buffer's norms stay 16-byte aligned).
