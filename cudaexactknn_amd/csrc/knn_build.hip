@@ -861,9 +861,11 @@ __device__ __forceinline__ void lf_insert(uint32_t* tw, uint32_t g, uint32_t p, 
     }
 }
 
-// Local id of point g, which the table holds.  An empty slot ends the probe
-// (local id 0): at most M of the H > M slots fill, so one is always reached.
-__device__ __forceinline__ uint32_t lf_lookup(const uint32_t* tw, const uint32_t* gidOf, uint32_t g, uint32_t H) {
+// Local id of point g, which the table holds (gidOf: point id per local id).
+// An empty slot ends the probe (local id 0): at most M of the H > M slots
+// fill, so one is always reached.
+__device__ __forceinline__ uint32_t lf_lookup(const uint32_t* tw, const uint32_t* __restrict__ gidOf, uint32_t g,
+                                              uint32_t H) {
     for (uint32_t k = lf_hash(g, H);; k = k + 1 == H ? 0u : k + 1) {
         const uint32_t v = lf_slot(tw, k);
         if (v == 0xFFFFu) return 0u;
@@ -871,34 +873,39 @@ __device__ __forceinline__ uint32_t lf_lookup(const uint32_t* tw, const uint32_t
     }
 }
 
-// Local ids of a segment's points: gidOf[p] = list 0 entry p, and every
-// other list rewritten as local ids through the hash table, which takes
+// Local ids of a segment's points: local id p is list 0's entry p, and every
+// other list is rewritten as local ids through the hash table, which takes
 // H = 3M / 2 slots over `tab` (3M bytes: the E and flag arrays, unused until
 // the level loop).  List j is at (bit j of altMask ? P2 : P) + j * N.  Xs:
 // when given, the points' coordinates are copied there in local-id order
 // (Xs[(st0 + p) * D + d], coalesced), so later reads of them stay within the
-// segment's own span.
-template <int IT>
+// segment's own span.  LDS_IDS: the point id of every local id is copied to
+// gidLds (M words of LDS) and the probes read it there; otherwise they
+// read the segment's span of list 0 itself, through L2 (the ball kernel,
+// whose LDS then fits one more workgroup per CU: 16.9 -> 16.0 ms; the kd
+// kernel was slower that way, 7.9 -> 8.7 ms).
+template <int IT, bool LDS_IDS>
 __device__ __forceinline__ void lf_load_lists(const uint32_t* __restrict__ P, const uint32_t* __restrict__ P2,
                                               uint32_t altMask, uint32_t N, uint32_t nlists,
-                                              uint32_t st0, uint32_t n0, uint32_t* gidOf, uint16_t* lists,
+                                              uint32_t st0, uint32_t n0, uint16_t* lists,
                                               uint8_t* tab, const float* __restrict__ pts, uint32_t D,
-                                              float* __restrict__ Xs) {
+                                              float* __restrict__ Xs, uint32_t* gidLds) {
     constexpr uint32_t M = LF_THREADS * IT, H = 3 * M / 2;
     static_assert(H % 2 == 0, "whole words of slots");
     const uint32_t tid = threadIdx.x;
     uint32_t* tw = reinterpret_cast<uint32_t*>(tab);
     for (uint32_t w = tid; w < H / 2; w += LF_THREADS) tw[w] = 0xFFFFFFFFu;
-    const uint32_t* L0 = (altMask & 1u) ? P2 : P;
+    const uint32_t* L0 = ((altMask & 1u) ? P2 : P) + st0;
     for (uint32_t p = tid; p < n0; p += LF_THREADS) {
-        const uint32_t g = L0[st0 + p];
-        gidOf[p] = g;
+        const uint32_t g = L0[p];
+        if constexpr (LDS_IDS) gidLds[p] = g;
         lists[p] = (uint16_t)p;
         if (Xs)
             for (uint32_t d = 0; d < D; ++d) Xs[(size_t)(st0 + p) * D + d] = pts[(size_t)g * D + d];
     }
     if (Xs) __threadfence();
     __syncthreads();
+    const uint32_t* gidOf = LDS_IDS ? gidLds : L0;
     for (uint32_t p = tid; p < n0; p += LF_THREADS) lf_insert(tw, gidOf[p], p, H);
     __syncthreads();
     for (uint32_t j = 1; j < nlists; ++j) {
@@ -938,7 +945,7 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
     const uint32_t s0 = blockIdx.x, tid = threadIdx.x;
     const uint32_t st0 = segStart[s0], n0 = segN[s0];
 
-    lf_load_lists<IT>(P, P2, altMask, N, SD, st0, n0, gidOf, lists, reinterpret_cast<uint8_t*>(E), pts, D, Xs);
+    lf_load_lists<IT, true>(P, P2, altMask, N, SD, st0, n0, lists, reinterpret_cast<uint8_t*>(E), pts, D, Xs, gidOf);
 
     // Thread tid owns positions tid * IT + i; ist / inn = start and size of
     // the segment holding that position at the current level.
@@ -1012,7 +1019,7 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
 constexpr uint32_t BLF_MAXD = 8;
 
 __host__ __device__ constexpr size_t blf_lds_bytes(uint32_t M, uint32_t D) {
-    return (size_t)D * M * 4 + (size_t)M * 4 + (size_t)M * 2 + (size_t)M * 3 + 32 + (size_t)D * M * 2;
+    return (size_t)D * M * 4 + (size_t)M * 2 + (size_t)M * 3 + 32 + (size_t)D * M * 2;
 }
 
 inline uint32_t ball_local_finish_points(uint32_t D) {
@@ -1040,8 +1047,7 @@ __global__ void __launch_bounds__(LF_THREADS) ball_local_finish_kernel(
     constexpr uint32_t M = LF_THREADS * IT;
     extern __shared__ __align__(16) uint8_t lds_raw[];
     float* xs = reinterpret_cast<float*>(lds_raw);                 // D x M: coordinates per local id
-    uint32_t* gidOf = reinterpret_cast<uint32_t*>(xs + D * M);     // M: point id per local id
-    uint16_t* E = reinterpret_cast<uint16_t*>(gidOf + M);          // M: left flags before a position
+    uint16_t* E = reinterpret_cast<uint16_t*>(xs + D * M);         // M: left flags before a position
     uint8_t* flag = reinterpret_cast<uint8_t*>(E + M);             // M: left flag per local id
     uint8_t* dimCur = flag + M;                                    // M: split dim per segment, this level
     uint8_t* dimNext = dimCur + M;                                 // M: split dim per segment, next level
@@ -1050,7 +1056,9 @@ __global__ void __launch_bounds__(LF_THREADS) ball_local_finish_kernel(
     const uint32_t s0 = blockIdx.x, tid = threadIdx.x;
     const uint32_t st0 = segStart[s0], n0 = segN[s0];
 
-    lf_load_lists<IT>(P, P, 0u, N, D, st0, n0, gidOf, lists, reinterpret_cast<uint8_t*>(E), nullptr, D, nullptr);
+    lf_load_lists<IT, false>(P, P, 0u, N, D, st0, n0, lists, reinterpret_cast<uint8_t*>(E), nullptr, D, nullptr,
+                             nullptr);
+    const uint32_t* gidOf = P + st0;  // point id per local id (list 0's span)
     // X holds the coordinates in list-0 order, so local id p is X[d * N + st0 + p].
     for (uint32_t p = tid; p < n0; p += LF_THREADS)
         for (uint32_t d = 0; d < D; ++d) xs[d * M + p] = X[(size_t)d * N + st0 + p];
@@ -1570,7 +1578,7 @@ void build_ball_ws(Handle& h, const float* pts, hipStream_t st, LevelBufs& B) {
         const uint32_t nl = I.node_depth - d0;
         const float inflate = 1.0f + (float)(D + 8) * 0x1.0p-23f;
         // ball_local_finish_points gives 4096 points for D = 1, 2048 for
-        // D = 2, 3 and 1024 for D = 4 .. 8.
+        // D = 2 .. 4 and 1024 for D = 5 .. 8.
         auto* fn = D > 4           ? ball_local_finish_kernel<4, 8>
                    : lfM == 4096 ? ball_local_finish_kernel<16, 4>
                    : lfM == 2048 ? ball_local_finish_kernel<8, 4>
