@@ -340,14 +340,16 @@ __device__ __forceinline__ void kd_packet_run(const TreeArgs& A, uint32_t i, uin
                      prime_slots + (lead ? nslots : 0u));
 }
 
-// Minimum waves per SIMD asked of the compiler: 6 for the register K-buffers
-// of K <= 16 at D <= 4.  The four-slot leaf scan needs more registers than
-// the one-slot scan did; asking for 8 waves made the compiler keep ~130
-// bytes per lane in scratch (headline kd traversal 16.0 ms at 8 waves, 11.4
-// at 7, 10.9 at 6, 10.9 at 5 and at 4).  Other forms keep the default.
+// Minimum waves per SIMD asked of the compiler: 5 for the register K-buffers
+// of K <= 16 at D <= 4 (90 VGPRs, no scratch).  The four-slot leaf scan
+// needs more registers than the one-slot scan did; asking for 8 waves made
+// the compiler keep ~130 bytes per lane in scratch (headline kd traversal
+// 16.0 ms at 8 waves, 11.4 at 7), and 6 waves still spilled 12 VGPRs
+// (40 bytes per lane) for the same time as 5 (10.04 / 10.05 ms median).
+// Other forms keep the default.
 template <int D, int KIND>
 constexpr int kd_packet_min_waves() {
-    return (D <= 4 && (KIND == BUF_REG8 || KIND == BUF_REG16 || KIND == BUF_REGH8 || KIND == BUF_REGH16)) ? 6 : 1;
+    return (D <= 4 && (KIND == BUF_REG8 || KIND == BUF_REG16 || KIND == BUF_REGH8 || KIND == BUF_REGH16)) ? 5 : 1;
 }
 
 template <int D, int KIND, bool COUNT, bool LEARN = false, bool PAD = false>
