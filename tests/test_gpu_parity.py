@@ -144,12 +144,16 @@ def test_kd_structure_matches_oracle(oracle, cuda, d, sd, skips):
 
 
 @pytest.mark.parametrize("n,d,sd,skips", [(4096, 3, 3, 0), (4097, 3, 3, 0), (50000, 3, 3, 0), (50000, 4, 3, 2),
-                                          (50000, 8, 8, 1), (40000, 16, 16, 0), (20000, 30, 30, 0)])
+                                          (50000, 8, 8, 1), (40000, 16, 16, 0), (20000, 30, 30, 0),
+                                          (50000, 2, 1, 1), (50000, 2, 2, 0), (70000, 6, 5, 0), (70000, 6, 4, 1)])
 def test_kd_structure_global_then_local_levels(oracle, cuda, n, d, sd, skips):
     """Sizes where the build runs some levels over the whole point set and
     finishes each segment inside one workgroup (segments of at most 4096,
     2048 or 1024 points by the number of split lists; 30 lists: every level
-    global).  Ties on the first coordinate and duplicate points included."""
+    global).  A global level partitions a list only when its dimension
+    splits next or when it has waited two levels, so one, two and four or
+    more split lists take different partition schedules.  Ties on the first
+    coordinate and duplicate points included."""
     g = np.random.Generator(np.random.Philox(n + d * 7 + skips))
     pts = g.random((n, d), dtype=np.float32)
     pts[: n // 8, 0] = 0.5
