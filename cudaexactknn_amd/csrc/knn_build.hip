@@ -797,17 +797,53 @@ inline uint32_t kd_local_finish_points(uint32_t SD) {
 // Stable partition of one LDS list inside every segment by the left flags of
 // its local ids.  Thread tid owns positions p0 + i (p0 = tid * IT), whose
 // segment starts at ist[i] with inn[i] points.  Contains barriers.
+// A thread's IT consecutive 16-bit LDS entries (p 16-byte aligned for IT >= 8,
+// 8-byte aligned for IT = 4) in one or two vector accesses instead of IT.
+template <int IT>
+__device__ __forceinline__ void lds_load_u16(const uint16_t* p, uint32_t (&v)[IT]) {
+    static_assert(IT == 4 || IT % 8 == 0, "vector width");
+    if constexpr (IT == 4) {
+        const uint2 x = *reinterpret_cast<const uint2*>(p);
+        v[0] = x.x & 0xFFFFu; v[1] = x.x >> 16; v[2] = x.y & 0xFFFFu; v[3] = x.y >> 16;
+    } else {
+#pragma unroll
+        for (int w = 0; w < IT / 8; ++w) {
+            const uint4 x = reinterpret_cast<const uint4*>(p)[w];
+            v[8 * w + 0] = x.x & 0xFFFFu; v[8 * w + 1] = x.x >> 16;
+            v[8 * w + 2] = x.y & 0xFFFFu; v[8 * w + 3] = x.y >> 16;
+            v[8 * w + 4] = x.z & 0xFFFFu; v[8 * w + 5] = x.z >> 16;
+            v[8 * w + 6] = x.w & 0xFFFFu; v[8 * w + 7] = x.w >> 16;
+        }
+    }
+}
+template <int IT>
+__device__ __forceinline__ void lds_store_u16(uint16_t* p, const uint32_t (&v)[IT]) {
+    static_assert(IT == 4 || IT % 8 == 0, "vector width");
+    auto pk = [&](int i) { return (v[i] & 0xFFFFu) | (v[i + 1] << 16); };
+    if constexpr (IT == 4) {
+        *reinterpret_cast<uint2*>(p) = make_uint2(pk(0), pk(2));
+    } else {
+#pragma unroll
+        for (int w = 0; w < IT / 8; ++w)
+            reinterpret_cast<uint4*>(p)[w] = make_uint4(pk(8 * w), pk(8 * w + 2), pk(8 * w + 4), pk(8 * w + 6));
+    }
+}
+
+// Stable partition of one LDS list inside every segment by the left flags of
+// its local ids.  Thread tid owns positions p0 + i (p0 = tid * IT), whose
+// segment starts at ist[i] with inn[i] points; its list entries and left
+// ranks move in vector LDS accesses.  Entries at or past n0 are read but not
+// used (their left-rank slots are scratch).  Contains barriers.
 template <int IT>
 __device__ __forceinline__ void lf_partition(uint16_t* Lj, const uint8_t* flag, uint16_t* E, uint32_t* scan_w,
                                              const uint32_t (&ist)[IT], const uint32_t (&inn)[IT], uint32_t p0,
                                              uint32_t n0) {
-    uint32_t v[IT];
+    uint32_t v[IT], r[IT];
+    lds_load_u16<IT>(Lj + p0, v);
     uint32_t fl = 0, cnt = 0;
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const uint32_t p = p0 + i;
-        v[i] = p < n0 ? Lj[p] : 0u;
-        const uint32_t f = p < n0 ? flag[v[i]] : 0u;
+        const uint32_t f = p0 + i < n0 ? flag[v[i]] : 0u;
         fl |= f << i;
         cnt += f;
     }
@@ -815,17 +851,17 @@ __device__ __forceinline__ void lf_partition(uint16_t* Lj, const uint8_t* flag, 
     uint32_t run = block_excl_scan(cnt, scan_w, &total);
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
-        const uint32_t p = p0 + i;
-        if (p < n0) E[p] = (uint16_t)run;
+        r[i] = run;
         run += (fl >> i) & 1u;
     }
+    lds_store_u16<IT>(E + p0, r);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const uint32_t p = p0 + i;
         if (p < n0) {
             const uint32_t L = (inn[i] + 1) >> 1;
-            const uint32_t lb = (uint32_t)E[p] - (uint32_t)E[ist[i]];
+            const uint32_t lb = r[i] - (uint32_t)E[ist[i]];
             const uint32_t pos = ((fl >> i) & 1u) ? ist[i] + lb : ist[i] + L + (p - ist[i] - lb);
             v[i] |= pos << 16;
         }
@@ -956,10 +992,14 @@ __global__ void __launch_bounds__(LF_THREADS) kd_local_finish_kernel(
     for (uint32_t l = 0; l < nl; ++l) {
         const uint32_t depth = d0 + l, dim = depth % SD;
         const uint16_t* Ld = lists + dim * M;
+        {
+            uint32_t lv[IT];
+            lds_load_u16<IT>(Ld + p0, lv);
 #pragma unroll
-        for (int i = 0; i < IT; ++i) {
-            const uint32_t p = p0 + i;
-            if (p < n0) flag[Ld[p]] = (p - ist[i]) < ((inn[i] + 1) >> 1) ? 1 : 0;
+            for (int i = 0; i < IT; ++i) {
+                const uint32_t p = p0 + i;
+                if (p < n0) flag[lv[i]] = (p - ist[i]) < ((inn[i] + 1) >> 1) ? 1 : 0;
+            }
         }
         const uint32_t nseg = 1u << l, S = 1u << depth;
         for (uint32_t q = tid; q < nseg; q += LF_THREADS) {
