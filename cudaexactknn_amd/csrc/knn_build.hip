@@ -211,6 +211,50 @@ constexpr uint32_t PT_TILE = PT_THREADS * PT_ITEMS;
 
 __device__ __forceinline__ uint32_t list_of(uint32_t y, uint32_t skip) { return y + (y >= skip ? 1u : 0u); }
 
+// Segment (index s, start st, size n) of position j at level lvl: the split
+// sizes (left = ceil(n/2)) descended from the root.
+__device__ __forceinline__ void seg_of(uint32_t j, uint32_t N, uint32_t lvl, uint32_t& st, uint32_t& n, uint32_t& s) {
+    st = 0;
+    n = N;
+    s = 0;
+    for (uint32_t b = 0; b < lvl; ++b) {
+        const uint32_t Lh = (n + 1) >> 1;
+        const bool r = j - st >= Lh;
+        s = 2 * s + (r ? 1u : 0u);
+        st = r ? st + Lh : st;
+        n = r ? n - Lh : Lh;
+    }
+}
+
+// A tile's segments at level lvl.  The sizes of a level are floor or ceil of
+// N / 2^lvl, so when N >> lvl >= PT_TILE every segment spans at least a
+// tile and the tile's positions lie in the segments of its first and last
+// positions (two descents per thread instead of one per item).
+struct TileSegs {
+    uint32_t stA, nA, sA, stB, nB, sB;
+    bool wide;
+};
+__device__ __forceinline__ TileSegs tile_segs(size_t base, uint32_t N, uint32_t lvl) {
+    TileSegs t{};
+    t.wide = (N >> lvl) >= PT_TILE;
+    if (t.wide) {
+        seg_of((uint32_t)base, N, lvl, t.stA, t.nA, t.sA);
+        seg_of((uint32_t)min(base + PT_TILE, (size_t)N) - 1u, N, lvl, t.stB, t.nB, t.sB);
+    }
+    return t;
+}
+__device__ __forceinline__ void seg_in_tile(const TileSegs& t, uint32_t j, uint32_t N, uint32_t lvl, uint32_t& st,
+                                            uint32_t& n, uint32_t& s) {
+    if (t.wide) {
+        const bool b = j - t.stA >= t.nA;
+        st = b ? t.stB : t.stA;
+        n = b ? t.nB : t.nA;
+        s = b ? t.sB : t.sA;
+    } else {
+        seg_of(j, N, lvl, st, n, s);
+    }
+}
+
 // Left bits per list position: the count pass gathers them by point id and
 // stores them in position order (pbits: one bitmap of nwords words per list
 // of the launch), so the scatter pass reads them coalesced.
@@ -307,19 +351,14 @@ __global__ void __launch_bounds__(PT_THREADS) part_scatter_kernel(const uint32_t
     }
     __syncthreads();
     const uint32_t g0 = off[(size_t)y * nb + blockIdx.x] - y * totalLeft;
+    const TileSegs ts = tile_segs(base, N, lvl);
 #pragma unroll
     for (int i = 0; i < PT_ITEMS; ++i) {
         const size_t j = base + (size_t)i * PT_THREADS + tid;
         if (j < N) {
-            // Segment of position j at level lvl (split sizes from the root).
-            uint32_t st = 0, n = N, s = 0;
-            for (uint32_t b = 0; b < lvl; ++b) {
-                const uint32_t Lh = (n + 1) >> 1;
-                const bool r = (uint32_t)j - st >= Lh;
-                s = 2 * s + (r ? 1u : 0u);
-                st = r ? st + Lh : st;
-                n = r ? n - Lh : Lh;
-            }
+            // Segment of position j at level lvl.
+            uint32_t st, n, s;
+            seg_in_tile(ts, (uint32_t)j, N, lvl, st, n, s);
             const uint32_t Ls = (n + 1) >> 1;
             const uint32_t g = g0 + wcnt[i * (PT_THREADS / 64) + wave] + rank[i];
             const uint32_t lb = g - leftBefore[s];
@@ -542,6 +581,7 @@ __global__ void __launch_bounds__(PT_THREADS) kd_scatter_kernel(const uint32_t* 
 #pragma unroll
     for (int c = 0; c < 4; ++c) o4[c] = off[(size_t)c * nb + blockIdx.x];
     const uint32_t Sp = 1u << lk;
+    const TileSegs ts = tile_segs(base, N, lk);
 #pragma unroll
     for (int i = 0; i < PT_ITEMS; ++i) {
         const size_t j = base + (size_t)i * PT_THREADS + tid;
@@ -550,14 +590,8 @@ __global__ void __launch_bounds__(PT_THREADS) kd_scatter_kernel(const uint32_t* 
             const uint32_t g = (c == 0 ? o4[0] : c == 1 ? o4[1] : c == 2 ? o4[2] : o4[3]) +
                                wcnt[c][i * (PT_THREADS / 64) + wave] + rank[i];
             // Segment of position j at level lk, then the start of its run c.
-            uint32_t st = 0, n = N, sg = 0;
-            for (uint32_t b = 0; b < lk; ++b) {
-                const uint32_t Lh = (n + 1) >> 1;
-                const bool r = (uint32_t)j - st >= Lh;
-                sg = 2 * sg + (r ? 1u : 0u);
-                st = r ? st + Lh : st;
-                n = r ? n - Lh : Lh;
-            }
+            uint32_t st, n, sg;
+            seg_in_tile(ts, (uint32_t)j, N, lk, st, n, sg);
             for (uint32_t b = k; b-- > 0;) {
                 const uint32_t Lh = (n + 1) >> 1;
                 const bool r = ((c >> b) & 1u) != 0u;
