@@ -709,31 +709,52 @@ __global__ void seg_reduce_kernel(const float* __restrict__ X, uint32_t N, uint3
 __global__ void seg_finalize_kernel(int mode, uint32_t S, uint32_t chunks, uint32_t D, uint32_t dbase,
                                     const double* __restrict__ partial, const float* __restrict__ pmax,
                                     const uint32_t* __restrict__ segN, float* __restrict__ cent,
-                                    double* __restrict__ var, float* __restrict__ ballNodes, float inflate) {
+                                    double* __restrict__ var, float* __restrict__ ballNodes, float inflate,
+                                    uint32_t lanes) {
 #pragma clang fp contract(off)
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
+    // lanes = 1: one thread per segment.  lanes = 64 (segments of several
+    // chunks): a wave per segment, lane r summing chunks r, r + 64, ... in
+    // order, then an xor butterfly (a fixed order, so builds stay
+    // deterministic; these levels have no local-finish counterpart to match).
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = t / lanes, r = t % lanes;
+    if (s >= S) return;  // whole waves when lanes = 64
     const uint32_t n = segN[s];
     double v[DB];
 #pragma unroll
     for (int b = 0; b < DB; ++b) v[b] = 0.0;
-    // Eight chunks' loads in flight; the additions keep chunk order.
+    float m = 0.0f;
     const double* ps = partial + (size_t)s * chunks * DB;
-    uint32_t k = 0;
-    for (; k + 8 <= chunks; k += 8) {
-        double t[8][DB];
+    uint32_t k = r;
+    // Eight chunks' loads in flight; the additions keep the lane's order.
+    for (; k + 7 * lanes < chunks; k += 8 * lanes) {
+        double tv[8][DB];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
 #pragma unroll
-            for (int b = 0; b < DB; ++b) t[u][b] = ps[(size_t)(k + u) * DB + b];
+            for (int b = 0; b < DB; ++b) tv[u][b] = ps[(size_t)(k + u * lanes) * DB + b];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
 #pragma unroll
-            for (int b = 0; b < DB; ++b) v[b] += t[u][b];
+            for (int b = 0; b < DB; ++b) v[b] += tv[u][b];
     }
-    for (; k < chunks; ++k) {
+    for (; k < chunks; k += lanes) {
 #pragma unroll
         for (int b = 0; b < DB; ++b) v[b] += ps[(size_t)k * DB + b];
+    }
+    if (pmax)
+        for (uint32_t c = r; c < chunks; c += lanes) {
+            const float x = pmax[(size_t)s * chunks + c];
+            m = x > m ? x : m;
+        }
+    if (lanes > 1) {
+        for (uint32_t o = lanes >> 1; o >= 1; o >>= 1) {
+#pragma unroll
+            for (int b = 0; b < DB; ++b) v[b] += __shfl_xor(v[b], (int)o, 64);
+            const float om = __shfl_xor(m, (int)o, 64);
+            m = om > m ? om : m;
+        }
+        if (r != 0) return;
     }
     float* nodeBase = nullptr;
     if (ballNodes) {
@@ -758,12 +779,6 @@ __global__ void seg_finalize_kernel(int mode, uint32_t S, uint32_t chunks, uint3
             if (d < D) var[(size_t)s * D + d] = v[b];
         }
         if (pmax && nodeBase) {
-            float m = 0.0f;
-#pragma unroll 8
-            for (uint32_t c = 0; c < chunks; ++c) {
-                const float x = pmax[(size_t)s * chunks + c];
-                m = x > m ? x : m;
-            }
             // Certified upper bound of the exact radius: every computed d2 is
             // within a factor (1 +- 2^-24)^(D+2) of the exact value.
             nodeBase[D] = n ? sqrtf(m) * inflate + 0x1.0p-100f : -INFINITY;
@@ -1490,8 +1505,10 @@ void seg_reduce(const float* X, uint32_t N, uint32_t D, const uint32_t* start, c
         const bool withMax = (mode == 1) && dbase == 0;
         hipLaunchKernelGGL(seg_reduce_kernel, dim3(grid), dim3(256), 0, st, X, N, D, dbase, start, n, S, chunks, G,
                            mode == 1 ? cent : nullptr, partial.get(), withMax ? pmax.get() : nullptr);
-        hipLaunchKernelGGL(seg_finalize_kernel, dim3(nblk(S, 256)), dim3(256), 0, st, mode, S, chunks, D, dbase,
-                           partial.get(), withMax ? pmax.get() : nullptr, n, cent, var, ballNodes, inflate);
+        const uint32_t lanes = chunks > 1 ? 64u : 1u;
+        hipLaunchKernelGGL(seg_finalize_kernel, dim3(nblk((uint64_t)S * lanes, 256)), dim3(256), 0, st, mode, S,
+                           chunks, D, dbase, partial.get(), withMax ? pmax.get() : nullptr, n, cent, var, ballNodes,
+                           inflate, lanes);
     }
     KNN_CHECK_LAUNCH();
 }
