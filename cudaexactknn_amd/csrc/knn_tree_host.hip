@@ -46,18 +46,41 @@ __global__ void morton_kernel(const float* __restrict__ qs, uint32_t nq, uint32_
 // Tree order: each query descends the top `levels` levels of the tree (kd:
 // split planes; ball: nearer centroid) and its landing cell, numbered left to
 // right, is its sort key.  Cells follow the data density, unlike a fixed grid.
-__global__ void kd_order_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D, const float* __restrict__ nodes,
-                                uint32_t levels, uint32_t SD, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    uint32_t node = 0, dim = 0;
-    for (uint32_t l = 0; l < levels; ++l) {
-        const float x = qs[(size_t)i * D + dim];
-        node = 2 * node + 1 + (x < nodes[node] ? 0u : 1u);
-        dim = (dim + 1 == SD) ? 0 : dim + 1;
+// The split values of the first `top` levels (2^top - 1 nodes, top <= 12)
+// are staged in LDS once per workgroup of a grid-stride loop, and a query of
+// at most 8 dims is held in registers, so the descent's dependent loads are
+// LDS reads for the top levels and L2 reads below.
+constexpr uint32_t KD_ORDER_TOP = 12;
+__global__ void __launch_bounds__(256) kd_order_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D,
+                                                       const float* __restrict__ nodes, uint32_t levels, uint32_t SD,
+                                                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    __shared__ float tn[(1u << KD_ORDER_TOP) - 1u];
+    const uint32_t top = levels < KD_ORDER_TOP ? levels : KD_ORDER_TOP;
+    const uint32_t ntop = (1u << top) - 1u;
+    for (uint32_t j = threadIdx.x; j < ntop; j += blockDim.x) tn[j] = nodes[j];
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        const float* q = qs + (size_t)i * D;
+        float qv[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) qv[d] = (uint32_t)d < D && D <= 8u ? q[d] : 0.0f;
+        uint32_t node = 0, dim = 0;
+        for (uint32_t l = 0; l < levels; ++l) {
+            float x;
+            if (D <= 8u) {
+                x = qv[0];
+#pragma unroll
+                for (int d = 1; d < 8; ++d) x = dim == (uint32_t)d ? qv[d] : x;
+            } else {
+                x = q[dim];
+            }
+            const float split = node < ntop ? tn[node] : nodes[node];
+            node = 2 * node + 1 + (x < split ? 0u : 1u);
+            dim = (dim + 1 == SD) ? 0 : dim + 1;
+        }
+        keys[i] = node - ((1u << levels) - 1u);
+        vals[i] = i;
     }
-    keys[i] = node - ((1u << levels) - 1u);
-    vals[i] = i;
 }
 
 __global__ void ball_order_kernel(const float* __restrict__ qs, uint32_t nq, uint32_t D,
@@ -128,7 +151,8 @@ void order_queries_into(Handle& h, const QueryArgs& a, const OrderScratch& S, ui
         if (bpd > 10) bpd = 10;
         bits = (int)(bpd * Dm);
     } else if (h.kind == KNN_KIND_KD) {
-        hipLaunchKernelGGL(kd_order_kernel, dim3((n + 255) / 256), dim3(256), 0, a.stream, a.q, n, D, h.nodes.get(),
+        const uint32_t og = std::min<uint32_t>((n + 255) / 256, 8 * device_limits().cu_count);
+        hipLaunchKernelGGL(kd_order_kernel, dim3(og ? og : 1u), dim3(256), 0, a.stream, a.q, n, D, h.nodes.get(),
                            tree_levels, h.info.split_dims, S.keys, S.vals);
         bits = (int)tree_levels;
     } else {
