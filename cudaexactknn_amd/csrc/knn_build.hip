@@ -9,10 +9,15 @@
 //    owns positions [start_s, start_s + n_s) in every list.
 //  - A level needs no sort: in P_dim the left child is simply the first
 //    ceil(n/2) positions of each segment (src/KdTreeGpu.cuh:231-238 rule) and
-//    the split value is the coordinate at rank ceil(n/2)-1.  Every other list
-//    is re-segmented by a stable 1-bit partition (one LSD radix pass on the
-//    left/right digit), computed from a per-point flag with a block-local
-//    ballot scan plus a device-wide scan of block counts.
+//    the split value is the coordinate at rank ceil(n/2)-1.  The other lists
+//    are re-segmented by stable partitions (one LSD radix pass on a left /
+//    right digit per waited level), computed from per-point flags with
+//    block-local ballot scans plus a device-wide scan of block counts.  A kd
+//    list is partitioned only when its own dimension splits next or when it
+//    has waited two levels (a 2-bit digit, four runs per segment); a ball
+//    list, whose segments split on their own dims, at every level.  Segment
+//    starts and sizes are the deterministic split sizes, found by descending
+//    from the root.
 //  - Ball trees pick each node's dim as the first maximum of the per-dim sum
 //    of squared deviations from the node centroid (src/BallTreeGpu.cuh:296-307)
 //    and store, per child, the FP64 mean rounded to FP32 and a certified upper
@@ -1412,8 +1417,8 @@ void partition_lists(LevelBufs& B, uint32_t N, uint32_t nl, uint32_t skip, uint3
                        reinterpret_cast<const uint32_t*>(B.flag.get()), B.cnt.get(), nb, B.posFlag.get(), nwords);
     exclusive_scan_u32(B.cnt.get(), B.cnt.get(), nb * nlists, B.temp.get(), st);
     hipLaunchKernelGGL(part_scatter_kernel, dim3(nb, nlists), dim3(PT_THREADS), 0, st, B.P.get(), B.P2.get(), N, skip,
-                       B.posFlag.get(), nwords, B.cnt.get(), nb, lvl, B.leftBefore.get(), B.Lcount.get(), S, coordD ? B.X.get() : nullptr,
-                       coordD ? B.X2.get() : nullptr, coordD);
+                       B.posFlag.get(), nwords, B.cnt.get(), nb, lvl, B.leftBefore.get(), B.Lcount.get(), S,
+                       coordD ? B.X.get() : nullptr, coordD ? B.X2.get() : nullptr, coordD);
     // The partitioned lists live in P2; the skipped list (already in order)
     // is carried over so that P2 is complete before the swap.
     if (skip < nl)
