@@ -6,7 +6,8 @@
 // Scan: 3-phase tiled (tile reduce, recursive scan of tile sums, tile scan).
 // Sort: per 8-bit digit pass, a tile histogram kernel, a scan of the
 // digit-major histogram, and a scatter kernel that ranks keys stably with
-// wave64 ballots (8 ballots give each lane the set of lanes sharing its digit).
+// wave64 ballots (8 ballots give each lane the set of lanes sharing its digit)
+// and reorders its tile digit-major in LDS before writing it out.
 #include "knn_device.hpp"
 #include "knn_internal.hpp"
 
