@@ -210,20 +210,22 @@ struct RegArray {
     }
     // Shift insertion: with c_i = (s[i] < key), which holds for a prefix of
     // the slots, the new slot i is s[i] where c_i, key where c_{i-1} and not
-    // c_i, else s[i-1]; the last slot drops out (c_{KCAP-1} is false).
-    // Walking i downwards reads each s[i-1] before it is rewritten and
-    // carries c_{i-1} to the next step, so one 64-bit compare and two 64-bit
-    // selects per slot suffice.
+    // c_i, else s[i-1]; the last slot drops out (c_{KCAP-1} is false).  With
+    // u_i = (c_i ? s[i] : key), that is new slot i = (c_{i-1} ? u_i : s[i-1])
+    // and u_{i-1} = (c_{i-1} ? s[i-1] : key): walking i downwards, both
+    // selects of a step use the one mask c_{i-1}, so each slot costs one
+    // 64-bit compare and two 64-bit selects (u_{KCAP-1} = key by the
+    // precondition, and new slot 0 = u_0).
     __device__ __forceinline__ uint64_t consider(uint64_t key) {
-        bool ci = false;  // key < s[KCAP-1] is the caller's precondition
+        uint64_t u = key;  // key < s[KCAP-1] is the caller's precondition
 #pragma unroll
         for (int i = KCAP - 1; i >= 1; --i) {
-            const bool cp = s[i - 1] < key;
-            const uint64_t t = cp ? key : s[i - 1];
-            s[i] = ci ? s[i] : t;
-            ci = cp;
+            const uint64_t lower = s[i - 1];
+            const bool c = lower < key;
+            s[i] = c ? u : lower;
+            u = c ? lower : key;
         }
-        s[0] = ci ? s[0] : key;
+        s[0] = u;
         return s[KCAP - 1];
     }
     // Merges C <= KCAP candidate keys sorted ascending (C a power of two;
