@@ -228,6 +228,33 @@ struct RegArray {
         s[0] = u;
         return s[KCAP - 1];
     }
+    // The shift insertion of slots L..KCAP-1 only, for a key above slot L-1
+    // (c_{L-1} holds, so the new slot L is u_L and slots below L stay).
+    template <int L>
+    __device__ __forceinline__ uint64_t consider_from(uint64_t key) {
+        uint64_t u = key;
+#pragma unroll
+        for (int i = KCAP - 1; i > L; --i) {
+            const uint64_t lower = s[i - 1];
+            const bool c = lower < key;
+            s[i] = c ? u : lower;
+            u = c ? lower : key;
+        }
+        s[L] = u;
+        return s[KCAP - 1];
+    }
+    // consider() for a wave that inserts together (the four-slot leaf scan):
+    // when every active lane's key lies above its slot KCAP/2 - 1, only the
+    // upper half of every buffer shifts, at half the compares and selects.
+    // The branch is wave-uniform.  (A second, quarter-buffer level, and the
+    // same shortcut in the one-slot scans, were measured slower: registers.)
+    __device__ __forceinline__ uint64_t consider_wave(uint64_t key) {
+        if constexpr (KCAP >= 16) {
+            constexpr int H = KCAP / 2;
+            if (__ballot(!(s[H - 1] < key)) == 0ull) return consider_from<H>(key);
+        }
+        return consider(key);
+    }
     // Merges C <= KCAP candidate keys sorted ascending (C a power of two;
     // keys that belong nowhere may be ~0): the buffer becomes the KCAP
     // smallest of buffer and candidates, ascending, which is what inserting
@@ -396,6 +423,14 @@ template <int KCAP, bool DESC> struct IsRegArray<RegArray<KCAP, DESC>> {
     static constexpr bool value = true;
     static constexpr int cap = KCAP;
 };
+
+// Insertion from code whose lanes insert together: the register buffers'
+// consider_wave, the plain consider for the LDS buffers.
+template <class B>
+__device__ __forceinline__ uint64_t consider_wave(B& buf, uint64_t key) {
+    if constexpr (IsRegArray<B>::value) return buf.consider_wave(key);
+    else return buf.consider(key);
+}
 
 template <int KIND> struct BufOf;
 template <> struct BufOf<BUF_REG8> { using T = RegArray<8>; static constexpr bool lds = false; };

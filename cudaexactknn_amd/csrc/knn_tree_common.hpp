@@ -332,7 +332,7 @@ __device__ __forceinline__ void packet_scan4(const float* __restrict__ leaves, u
                     pm &= pm - 1u;
                     const uint64_t key = make_key(dk, __float_as_uint(ik));
                     if (key < worst && (!FLOOR || key >= floor)) {
-                        worst = buf.consider(key);
+                        worst = consider_wave(buf, key);
                         whi = key_hi(worst);
                         whim = clamp_hi(whi);
                     }
