@@ -130,17 +130,24 @@ __global__ void feature_kernel(TreeArgs A, int is_ball, const float* __restrict_
         for (uint32_t l = 0; l < pl; ++l) {
             const float sv = A.nodes[n];
             const bool left = q[dim] < sv;
-            if (dim < 8) {
-                if (left) hi[dim] = fminf(hi[dim], sv); else lo[dim] = fmaxf(lo[dim], sv);
+            // Static indices keep the box in registers.
+#pragma unroll
+            for (uint32_t d = 0; d < 8; ++d) {
+                if (d == dim) {
+                    if (left) hi[d] = fminf(hi[d], sv); else lo[d] = fmaxf(lo[d], sv);
+                }
             }
             n = 2 * n + 1 + (left ? 0u : 1u);
             dim = (dim + 1 == A.split_dims) ? 0 : dim + 1;
         }
         const uint32_t dm = D < 8 ? D : 8;
-        for (uint32_t d = 0; d < dm; ++d) {
-            lsum += safe_log2(hi[d] - lo[d]);
-            const float g = fminf(fabsf(q[d] - lo[d]), fabsf(hi[d] - q[d]));
-            face = fminf(face, g * g);
+#pragma unroll
+        for (uint32_t d = 0; d < 8; ++d) {
+            if (d < dm) {
+                lsum += safe_log2(hi[d] - lo[d]);
+                const float g = fminf(fabsf(q[d] - lo[d]), fabsf(hi[d] - q[d]));
+                face = fminf(face, g * g);
+            }
         }
         lsum /= (float)dm;
     } else {
@@ -159,29 +166,39 @@ __global__ void feature_kernel(TreeArgs A, int is_ball, const float* __restrict_
     // Scan the landing subtree: nearest and K-th smallest squared distance.
     uint32_t s0, cntn;
     leaf_range(A, n, pl, s0, cntn);
+    // best[0..K-1]: the K smallest so far, ascending (slots from K on stay
+    // +inf); kth mirrors best[K-1].  Every index is static, so the list
+    // stays in registers.
     float best[32];
     const uint32_t K = A.K < 32 ? A.K : 32;
+#pragma unroll
     for (uint32_t j = 0; j < 32; ++j) best[j] = INFINITY;
+    float kth = INFINITY;
     float dmin = INFINITY;
     for (uint32_t o = 0; o < cntn; ++o) {
         const float* e = A.leaves + (size_t)(s0 + o) * (1 + D);
         if (__float_as_int(e[0]) == -1) continue;
         const float d2 = dist2_rt(q, 1, e + 1, D);
         dmin = fminf(dmin, d2);
-        if (d2 < best[K - 1]) {
+        if (d2 < kth) {
             float key = d2;
-            for (uint32_t j = 0; j < K; ++j) {
-                const float b = best[j];
-                best[j] = fminf(b, key);
-                key = fmaxf(b, key);
+#pragma unroll
+            for (uint32_t j = 0; j < 32; ++j) {
+                if (j < K) {
+                    const float b = best[j];
+                    best[j] = fminf(b, key);
+                    key = fmaxf(b, key);
+                }
             }
+#pragma unroll
+            for (uint32_t j = 0; j < 32; ++j) kth = (j + 1 == K) ? best[j] : kth;
         }
     }
     f[8] = lsum;
     f[9] = safe_log2(dmin);
-    f[10] = safe_log2(best[K - 1] < INFINITY ? best[K - 1] : dmin);
+    f[10] = safe_log2(kth < INFINITY ? kth : dmin);
     f[11] = safe_log2(face);
-    if (ub) ub[i] = A.K <= 32 ? best[K - 1] : INFINITY;
+    if (ub) ub[i] = A.K <= 32 ? kth : INFINITY;
 }
 
 // --- MLP forward (FP32 or bf16 inference) --------------------------------------
