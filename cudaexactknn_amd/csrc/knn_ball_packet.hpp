@@ -142,7 +142,9 @@ __device__ __forceinline__ void ball_packet_run(const TreeArgs& A, float shrink_
                 leaf_range(A, node, level, slot0, count);
                 const uint32_t s0 = uni_b(slot0);
                 // The four-slot batches of the kd scan were slower here
-                // (18.0 vs 16.4 ms on the 16.7M-point mixture).
+                // (18.0 vs 16.4 ms on the 16.7M-point mixture; 17.2 vs 15.3
+                // ms with the one-compare insertion, at 88 instead of 64
+                // VGPRs).
                 packet_scan<D, LEARN, PAD>(A.leaves, s0, uni_b(count), q, active && s0 - land0 >= landN, buf,
                                            worst, whi, nslots, floor, Dt);
             }
